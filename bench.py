@@ -1,0 +1,159 @@
+#!/usr/bin/env python3
+"""Benchmark of the ConvStencil hot path on one MI355X.
+
+Workload (default): box2d1r (9-point) stencil, FP64, 10240 x 10240 interior
+with a zero halo of width 1, T = 64 time steps, seeded synthetic input (interior
+uniform in [-1, 1), weights uniform in (0, 1) normalised to sum 1).  One bench
+"step" is one full 64-time-step run over the grid, with the grid resident in
+HBM; GStencil/s counts interior points x time steps / s, as the reference does
+(src/2d/gpu.cu:329).
+
+Prints one JSON line.  `roofline` describes the dominant kernel: algorithmic
+bytes (8 B read + 8 B written per interior point per launch) or flops (2 per tap
+per point-step) per launch over its mean launch time measured with HIP events
+on the plan stream; `traffic` is the measured HBM bytes per launch from the
+committed rocprofv3 PMC summary (profiles/), or null.  `cpu_baseline` is the
+CPU oracle (the reference's loop, restated) timed on this host's cores on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+WORKLOADS = {
+    # name: (shape, interior sizes, halo, steps)
+    "box2d1r": ("box2d1r", [10240, 10240], 1, 64),
+    "star2d3r": ("star2d3r", [8192, 8192], 3, 64),
+    "box2d3r": ("box2d3r", [8192, 8192], 3, 64),
+}
+
+
+def load_traffic(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        v = d.get(kernel_key)
+        return None if v is None else float(v["bytes_per_launch"])
+    except (OSError, ValueError, KeyError, TypeError):
+        return None
+
+
+def cpu_baseline(layout, offs, w, seed, budget_s):
+    """Times the oracle (OpenMP over all host cores) on the full grid for as many
+    whole time steps as fit in about budget_s seconds (at least one)."""
+    from oracle import oracle
+    threads = oracle.max_threads()
+    x = oracle.fill_data(layout, seed)
+    t0 = time.perf_counter()
+    oracle.run(layout, offs, w, x, 1, threads=threads)
+    one = time.perf_counter() - t0
+    steps = max(1, int(budget_s / max(one, 1e-6)))
+    t0 = time.perf_counter()
+    oracle.run(layout, offs, w, x, steps, threads=threads)
+    el = time.perf_counter() - t0
+    pts = layout.points
+    return {"value": pts * steps / el / 1e9, "unit": "GStencil/s", "cores": threads, "kind": "port",
+            "sample": f"full grid, {steps} time step(s) of the same stencil, {el:.1f} s, OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="box2d1r", choices=sorted(WORKLOADS))
+    ap.add_argument("--fusion", type=int, default=0, help="fusion depth (0 = library default)")
+    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1234)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import convstencil_amd as cs
+    from convstencil_amd import Layout, Plan, shape_offsets, dense_weights, SHAPES
+
+    shape, sizes, halo, T = WORKLOADS[args.workload]
+    ndim, r, kind = SHAPES[shape]
+    L = Layout.with_halo(sizes, halo)
+    offs = shape_offsets(shape)
+    w = cs.fill_weights_host(len(offs), args.seed)
+    dense = dense_weights(ndim, r, offs, w)
+
+    plan = Plan(L, r, dense, fusion_depth=args.fusion, variant=args.variant, use_graph=not args.no_graph)
+    plan.fill_random(args.seed)
+    desc = plan.describe()
+    # warmup + timed region: HIP events on the plan stream around exactly K runs of T steps
+    t0 = time.perf_counter()
+    rep = plan.time(T, warmup=args.warmup, reps=args.steps)
+    wall = time.perf_counter() - t0
+    ms = rep.elapsed_ms
+    pts = L.points
+    value = pts * T / (ms * 1e-3) / 1e9
+    launches = rep.launches
+    ms_launch = ms / launches
+    kf = rep.fusion_depth
+    alg_bytes = 16.0 * pts
+    alg_flops = 2.0 * len(offs) * pts * kf
+    hbm = alg_bytes / (ms_launch * 1e-3) / 1e9
+    tfl = alg_flops / (ms_launch * 1e-3) / 1e12
+    frac_hbm, frac_fl = hbm / HBM_PEAK_GBS, tfl / FP64_PEAK_TFLOPS
+    kernel_key = f"{args.workload}:variant{rep.variant}"
+    traffic = load_traffic(kernel_key)
+    if frac_hbm >= frac_fl:
+        roof = {"bound": "hbm", "achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(frac_hbm, 4), "traffic": traffic}
+    else:
+        roof = {"bound": "mfma", "achieved": round(tfl, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(frac_fl, 4), "traffic": traffic}
+    roof.update({"kernel": desc.split()[1].split("=", 1)[1], "launches_per_step": launches,
+                 "ms_per_launch": round(ms_launch, 5), "hbm_gbs_alg": round(hbm, 1), "fp64_tflops_alg": round(tfl, 2),
+                 "alg_bytes_per_launch": alg_bytes, "alg_flops_per_launch": alg_flops})
+    plan.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(L, offs, w, args.seed, args.cpu_seconds)
+
+    out = {
+        "metric": "GStencil/s at 1 GPU: FP64 MFMA vs split-precision at stated error; % roofline",
+        "value": round(value, 2),
+        "unit": "GStencil/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64 seeded grid and weights)",
+        "config": {"workload": f"{shape} {'x'.join(map(str, sizes))} halo {halo} T={T} fp64", "time_steps": T,
+                   "fusion_depth": kf, "plan": desc, "wall_s": round(wall, 3)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

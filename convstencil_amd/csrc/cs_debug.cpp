@@ -1,0 +1,49 @@
+// cs_debug.cpp -- host-only table introspection (include/convstencil_debug.h).
+#include "../../include/convstencil_debug.h"
+
+#include <cstring>
+
+#include "cs_internal.h"
+
+extern "C" {
+
+int32_t cs_debug_variant_count(void) {
+  int n = 0;
+  (void)cs::variants_2d_f64(&n);
+  return n;
+}
+
+int32_t cs_debug_variant_info(int32_t i, int64_t *info, char *name, int32_t len) {
+  int n = 0;
+  const cs::Variant *v = cs::variants_2d_f64(&n);
+  if (i < 0 || i >= n || !info) return -1;
+  const cs::Variant &x = v[i];
+  const int64_t vals[10] = {x.id, x.mf, x.R, x.shape, x.kf, x.by, x.bx, x.nw, (int64_t)x.lds_bytes, x.px};
+  memcpy(info, vals, sizeof(vals));
+  if (name && len > 0) {
+    strncpy(name, x.name, (size_t)len - 1);
+    name[len - 1] = 0;
+  }
+  return 0;
+}
+
+int32_t cs_debug_ktable(int32_t mf, int32_t radius, int32_t shape, int32_t *dy, int32_t *c0, int32_t *live,
+                        int32_t cap) {
+  const cs::KTable t = mf == 16 ? cs::make_ktable(radius, shape) : cs::make_ftable(radius, shape);
+  if (t.n > cap) return -1;
+  for (int i = 0; i < t.n; ++i) {
+    dy[i] = t.dy[i];
+    c0[i] = t.c0[i];
+    live[i] = t.live[i];
+  }
+  return t.n;
+}
+
+int32_t cs_debug_bfrag_2d(int32_t mf, int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap) {
+  const int nks = cs::bfrag_ksteps_2d_f64(mf, radius, shape);
+  if (nks * 64 > cap) return -1;
+  cs::build_bfrag_2d_f64(mf, radius, shape, w, bfrag);
+  return nks;
+}
+
+}  // extern "C"

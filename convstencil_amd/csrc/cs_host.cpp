@@ -1,0 +1,509 @@
+// cs_host.cpp -- host runtime and C ABI of libconvstencil (see include/convstencil.h).
+//
+// The reference's host launchers (src/2d/gpu.cu:245-333 and siblings) each build
+// the B matrices, allocate two device buffers, copy the input, run the time loop
+// of single-launch steps and copy the result back.  Here that body is a plan:
+// buffers and B fragments are built once, the time loop is a sequence of fused
+// launches (optionally captured in a hipGraph), and the reference-named
+// launchers at the bottom of this file are thin wrappers that keep the
+// reference's layouts, weight layouts and stdout lines.
+#include "../../include/convstencil.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cs_internal.h"
+#include "cs_rng.h"
+
+using cs::Geo;
+using cs::Variant;
+
+namespace {
+
+#define CS_HIP(call)                                                                                        \
+  do {                                                                                                      \
+    hipError_t e_ = (call);                                                                                 \
+    if (e_ != hipSuccess) {                                                                                 \
+      fprintf(stderr, "convstencil: HIP error %d (%s) at %s:%d: %s\n", (int)e_, hipGetErrorString(e_), __FILE__, \
+              __LINE__, #call);                                                                             \
+      return CS_ERR_DEVICE;                                                                                 \
+    }                                                                                                       \
+  } while (0)
+
+#define CS_TRY(expr)                  \
+  do {                                \
+    cs_status s_ = (expr);            \
+    if (s_ != CS_OK) return s_;       \
+  } while (0)
+
+bool layout_valid(const cs_layout *L, int R) {
+  if (!L || L->ndim < 1 || L->ndim > 3) return false;
+  for (int d = 0; d < 3; ++d) {
+    const bool active = d >= 3 - L->ndim;
+    if (!active) {
+      if (L->ext[d] != 1 || L->n[d] != 1 || L->org[d] != 0) return false;
+      continue;
+    }
+    if (L->ext[d] <= 0 || L->n[d] < 0 || L->org[d] < R) return false;
+    if (L->org[d] + L->n[d] + R > L->ext[d]) return false;
+  }
+  const double cells = (double)L->ext[0] * (double)L->ext[1] * (double)L->ext[2];
+  return cells < 9.0e18 / 8.0;
+}
+
+int64_t cell_count(const cs_layout *L) { return L->ext[0] * L->ext[1] * L->ext[2]; }
+
+// 2-D shape class from the dense weights: a star has no off-axis taps.
+int classify_2d(int R, const double *w) {
+  const int D = 2 * R + 1;
+  for (int dy = -R; dy <= R; ++dy)
+    for (int dx = -R; dx <= R; ++dx)
+      if (dy != 0 && dx != 0 && w[(dy + R) * D + (dx + R)] != 0.0) return cs::SHAPE_BOX;
+  return cs::SHAPE_STAR;
+}
+
+// True when every halo cell of the padded host array is zero.
+bool halo_is_zero(const cs_layout *L, const double *a) {
+  for (int64_t z = 0; z < L->ext[0]; ++z)
+    for (int64_t y = 0; y < L->ext[1]; ++y) {
+      const bool row_in = z >= L->org[0] && z < L->org[0] + L->n[0] && y >= L->org[1] && y < L->org[1] + L->n[1];
+      const double *row = a + (z * L->ext[1] + y) * L->ext[2];
+      for (int64_t x = 0; x < L->ext[2]; ++x) {
+        if (row_in && x >= L->org[2] && x < L->org[2] + L->n[2]) {
+          x = L->org[2] + L->n[2] - 1;
+          continue;
+        }
+        if (row[x] != 0.0) return false;
+      }
+    }
+  return true;
+}
+
+cs_status check_device(int *dev_out) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return CS_ERR_NO_DEVICE;
+  int dev = 0;
+  CS_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  CS_HIP(hipGetDeviceProperties(&prop, dev));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    fprintf(stderr, "convstencil: device %d is %s, this library is built for gfx950\n", dev, prop.gcnArchName);
+    return CS_ERR_NO_DEVICE;
+  }
+  *dev_out = dev;
+  return CS_OK;
+}
+
+}  // namespace
+
+struct cs_plan {
+  cs_layout L{};
+  int ndim = 0, R = 0, shape = 0;
+  std::vector<double> w;  // dense weights
+  cs_options opt{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double *buf[2] = {nullptr, nullptr};
+  double *bfrag = nullptr;  // B fragments for the selected MFMA shape
+  int cur = 0;              // buffer holding the current state
+  int64_t steps_done = 0;
+  bool halo_uniform = true;  // both buffers carry the same halo -> fusion is exact
+  const Variant *vmain = nullptr;
+  std::vector<const Variant *> vrem;  // vrem[r] runs r fused steps (r < vmain->kf), may be null
+  const Variant *vone = nullptr;      // kf == 1 variant of the same family
+  std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
+  int64_t launches = 0;
+};
+
+namespace {
+
+Geo make_geo(const cs_layout &L, const Variant *v) {
+  Geo g{};
+  for (int d = 0; d < 3; ++d) {
+    g.ext[d] = L.ext[d];
+    g.org[d] = L.org[d];
+    g.n[d] = L.n[d];
+  }
+  g.tiles[0] = 1;
+  g.tiles[1] = (L.n[1] + v->by - 1) / v->by;
+  g.tiles[2] = (L.n[2] + v->bx - 1) / v->bx;
+  return g;
+}
+
+const Variant *find_variant(int mf, int R, int shape, int kf, int prefer_id) {
+  int cnt = 0;
+  const Variant *vs = cs::variants_2d_f64(&cnt);
+  const Variant *best = nullptr;
+  for (int i = 0; i < cnt; ++i) {
+    const Variant &v = vs[i];
+    if (prefer_id > 0 && v.id == prefer_id && v.R == R && v.shape == shape) return &v;
+    if (v.mf == mf && v.R == R && v.shape == shape && v.kf == kf && !best) best = &v;
+  }
+  return best;
+}
+
+int max_kf(int mf, int R, int shape) {
+  int cnt = 0, m = 0;
+  const Variant *vs = cs::variants_2d_f64(&cnt);
+  for (int i = 0; i < cnt; ++i)
+    if (vs[i].mf == mf && vs[i].R == R && vs[i].shape == shape) m = std::max(m, vs[i].kf);
+  return m;
+}
+
+cs_status select_variants(cs_plan *p) {
+  if (p->ndim != 2) return CS_ERR_UNSUPPORTED;
+  const int mf = 4;
+  const Variant *v = nullptr;
+  if (p->opt.tile_variant > 0) {
+    v = find_variant(0, p->R, p->shape, 0, p->opt.tile_variant);
+    if (!v) return CS_ERR_INVALID;
+  } else {
+    int kf = p->opt.fusion_depth > 0 ? p->opt.fusion_depth : max_kf(mf, p->R, p->shape);
+    kf = std::min(kf, max_kf(mf, p->R, p->shape));
+    for (; kf >= 1 && !v; --kf) v = find_variant(mf, p->R, p->shape, kf, 0);
+  }
+  if (!v) return CS_ERR_UNSUPPORTED;
+  p->vmain = v;
+  p->vone = find_variant(v->mf, p->R, p->shape, 1, 0);
+  if (!p->vone) return CS_ERR_UNSUPPORTED;
+  p->vrem.assign(v->kf, nullptr);
+  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(v->mf, p->R, p->shape, r, 0);
+  for (const Variant *u : {p->vmain, p->vone}) CS_HIP(u->prepare());
+  for (const Variant *u : p->vrem)
+    if (u) CS_HIP(u->prepare());
+  // B fragments for the family's MFMA shape
+  const int nks = cs::bfrag_ksteps_2d_f64(v->mf, p->R, p->shape);
+  std::vector<double> h((size_t)nks * 64);
+  cs::build_bfrag_2d_f64(v->mf, p->R, p->shape, p->w.data(), h.data());
+  if (p->bfrag) (void)hipFree(p->bfrag);
+  p->bfrag = nullptr;
+  CS_HIP(hipMalloc(&p->bfrag, h.size() * sizeof(double)));
+  CS_HIP(hipMemcpy(p->bfrag, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  return CS_OK;
+}
+
+cs_status launch_one(cs_plan *p, const Variant *v) {
+  const Geo g = make_geo(p->L, v);
+  CS_HIP(v->launch(p->buf[p->cur], p->buf[p->cur ^ 1], p->bfrag, g, p->stream));
+  p->cur ^= 1;
+  p->launches += 1;
+  return CS_OK;
+}
+
+// Issues the launches for `steps` physical steps.  With uniform halos, fused
+// launches of kf steps are exact; otherwise every launch is a single step so that
+// each step reads the halo of its own buffer, as the reference's ping-pong does.
+cs_status issue_steps(cs_plan *p, int64_t steps) {
+  if (steps <= 0) return CS_OK;
+  if (!p->halo_uniform || p->vmain->kf == 1) {
+    for (int64_t t = 0; t < steps; ++t) CS_TRY(launch_one(p, p->vone));
+    return CS_OK;
+  }
+  const int kf = p->vmain->kf;
+  for (int64_t t = 0; t < steps / kf; ++t) CS_TRY(launch_one(p, p->vmain));
+  const int r = (int)(steps % kf);
+  if (r > 0) {
+    if (p->vrem[r])
+      CS_TRY(launch_one(p, p->vrem[r]));
+    else
+      for (int t = 0; t < r; ++t) CS_TRY(launch_one(p, p->vone));
+  }
+  return CS_OK;
+}
+
+// Number of launches issue_steps() makes for `steps` steps.
+int64_t count_launches(const cs_plan *p, int64_t steps) {
+  if (steps <= 0) return 0;
+  if (!p->halo_uniform || p->vmain->kf == 1) return steps;
+  const int kf = p->vmain->kf;
+  const int r = (int)(steps % kf);
+  return steps / kf + (r > 0 ? (p->vrem[r] ? 1 : r) : 0);
+}
+
+cs_status run_steps(cs_plan *p, int64_t steps) {
+  if (steps < 0) return CS_ERR_INVALID;
+  if (steps == 0) return CS_OK;
+  CS_HIP(hipSetDevice(p->device));
+  if (!p->opt.use_graph) {
+    CS_TRY(issue_steps(p, steps));
+    p->steps_done += steps;
+    return CS_OK;
+  }
+  // Graphs are keyed by (step count, starting buffer, fusion rule): the captured
+  // kernel arguments name the buffers directly.
+  const auto key = std::make_pair(steps * 4 + (p->halo_uniform ? 2 : 0), p->cur);
+  auto it = p->graphs.find(key);
+  if (it == p->graphs.end()) {
+    const int cur0 = p->cur;
+    const int64_t launches0 = p->launches;
+    hipGraph_t graph = nullptr;
+    CS_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    cs_status st = issue_steps(p, steps);
+    hipError_t e = hipStreamEndCapture(p->stream, &graph);
+    p->cur = cur0;
+    p->launches = launches0;
+    if (st != CS_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st;
+    }
+    CS_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    CS_HIP(ei);
+    it = p->graphs.emplace(key, exec).first;
+  }
+  CS_HIP(hipGraphLaunch(it->second, p->stream));
+  const int64_t n = count_launches(p, steps);
+  p->cur = (int)((p->cur + n) & 1);
+  p->launches += n;
+  p->steps_done += steps;
+  return CS_OK;
+}
+
+void destroy_graphs(cs_plan *p) {
+  for (auto &kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+  p->graphs.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cs_version(void) { return "convstencil-mi355x 0.1.0 (gfx950)"; }
+
+const char *cs_status_string(cs_status s) {
+  switch (s) {
+    case CS_OK: return "ok";
+    case CS_ERR_INVALID: return "invalid argument";
+    case CS_ERR_UNSUPPORTED: return "unsupported configuration";
+    case CS_ERR_ALLOC: return "allocation failed";
+    case CS_ERR_DEVICE: return "HIP error";
+    case CS_ERR_NO_DEVICE: return "no gfx950 device";
+  }
+  return "unknown status";
+}
+
+cs_status cs_device_count(int32_t *count) {
+  if (!count) return CS_ERR_INVALID;
+  *count = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return CS_OK;
+  for (int d = 0; d < n; ++d) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess && strncmp(prop.gcnArchName, "gfx950", 6) == 0) ++*count;
+  }
+  return CS_OK;
+}
+
+void cs_fill_random_host(double *a, const cs_layout *L, uint64_t seed) {
+  if (!a || !L) return;
+  const int64_t total = cell_count(L);
+  std::fill(a, a + total, 0.0);
+  uint64_t st = cs::stream_seed(seed, 0);
+  for (int64_t z = 0; z < L->n[0]; ++z)
+    for (int64_t y = 0; y < L->n[1]; ++y) {
+      double *row = a + ((L->org[0] + z) * L->ext[1] + (L->org[1] + y)) * L->ext[2] + L->org[2];
+      for (int64_t x = 0; x < L->n[2]; ++x) row[x] = cs::unit_data(cs::splitmix_next(st));
+    }
+}
+
+void cs_fill_weights_host(double *w, int32_t count, uint64_t seed) {
+  if (!w || count <= 0) return;
+  uint64_t st = cs::stream_seed(seed, 1);
+  double s = 0.0;
+  for (int i = 0; i < count; ++i) {
+    w[i] = cs::unit_weight(cs::splitmix_next(st));
+    s = s + w[i];
+  }
+  for (int i = 0; i < count; ++i) w[i] = w[i] / s;
+}
+
+cs_status cs_plan_create(cs_plan **out, const cs_layout *layout, const cs_stencil *st, const cs_options *opt) {
+  if (!out || !layout || !st || !st->weights) return CS_ERR_INVALID;
+  *out = nullptr;
+  if (st->ndim != layout->ndim || st->radius < 1) return CS_ERR_INVALID;
+  if (!layout_valid(layout, st->radius)) return CS_ERR_INVALID;
+  if (st->radius > (st->ndim == 1 ? 4 : 3)) return CS_ERR_UNSUPPORTED;
+  int dev = 0;
+  CS_TRY(check_device(&dev));
+  cs_plan *p = new cs_plan();
+  p->L = *layout;
+  p->ndim = layout->ndim;
+  p->R = st->radius;
+  const int D = 2 * p->R + 1;
+  const int nw = p->ndim == 1 ? D : p->ndim == 2 ? D * D : D * D * D;
+  p->w.assign(st->weights, st->weights + nw);
+  if (opt) p->opt = *opt;
+  p->device = dev;
+  p->shape = p->ndim == 2 ? classify_2d(p->R, p->w.data()) : cs::SHAPE_BOX;
+  cs_status s = select_variants(p);
+  if (s != CS_OK) {
+    cs_plan_destroy(p);
+    return s;
+  }
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+    cs_plan_destroy(p);
+    return CS_ERR_DEVICE;
+  }
+  const size_t bytes = (size_t)cell_count(layout) * sizeof(double);
+  if (hipMalloc(&p->buf[0], bytes) != hipSuccess || hipMalloc(&p->buf[1], bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    cs_plan_destroy(p);
+    return CS_ERR_ALLOC;
+  }
+  *out = p;
+  return CS_OK;
+}
+
+cs_status cs_plan_destroy(cs_plan *p) {
+  if (!p) return CS_ERR_INVALID;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  destroy_graphs(p);
+  for (double *&b : p->buf)
+    if (b) (void)hipFree(b);
+  if (p->bfrag) (void)hipFree(p->bfrag);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return CS_OK;
+}
+
+cs_status cs_plan_upload(cs_plan *p, const double *host_in) {
+  if (!p || !host_in) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(p->device));
+  const size_t n = (size_t)cell_count(&p->L);
+  CS_HIP(hipMemcpyAsync(p->buf[0], host_in, n * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  CS_HIP(cs::launch_zero(p->buf[1], n, p->stream));
+  CS_HIP(hipStreamSynchronize(p->stream));
+  p->cur = 0;
+  p->steps_done = 0;
+  p->halo_uniform = halo_is_zero(&p->L, host_in);
+  return CS_OK;
+}
+
+cs_status cs_plan_fill_random(cs_plan *p, uint64_t seed) {
+  if (!p) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(p->device));
+  Geo g{};
+  for (int d = 0; d < 3; ++d) {
+    g.ext[d] = p->L.ext[d];
+    g.org[d] = p->L.org[d];
+    g.n[d] = p->L.n[d];
+  }
+  CS_HIP(cs::launch_fill_random(p->buf[0], g, cs::stream_seed(seed, 0), p->stream));
+  CS_HIP(cs::launch_zero(p->buf[1], (size_t)cell_count(&p->L), p->stream));
+  CS_HIP(hipStreamSynchronize(p->stream));
+  p->cur = 0;
+  p->steps_done = 0;
+  p->halo_uniform = true;
+  return CS_OK;
+}
+
+cs_status cs_plan_run(cs_plan *p, int64_t steps, cs_report *rep) {
+  if (!p) return CS_ERR_INVALID;
+  const int64_t l0 = p->launches;
+  CS_TRY(run_steps(p, steps));
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->launches = p->launches - l0;
+    rep->fusion_depth = p->halo_uniform ? p->vmain->kf : 1;
+    rep->variant = p->vmain->id;
+  }
+  return CS_OK;
+}
+
+cs_status cs_plan_time(cs_plan *p, int64_t steps, int32_t warmup, int32_t reps, cs_report *rep) {
+  if (!p || reps <= 0 || warmup < 0 || steps <= 0) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(p->device));
+  for (int i = 0; i < warmup; ++i) CS_TRY(run_steps(p, steps));
+  CS_HIP(hipStreamSynchronize(p->stream));
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  const int64_t l0 = p->launches;
+  CS_HIP(hipEventRecord(e0, p->stream));
+  for (int i = 0; i < reps; ++i) CS_TRY(run_steps(p, steps));
+  CS_HIP(hipEventRecord(e1, p->stream));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  CS_HIP(hipEventDestroy(e0));
+  CS_HIP(hipEventDestroy(e1));
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->elapsed_ms = ms / reps;
+    const double pts = (double)p->L.n[0] * (double)p->L.n[1] * (double)p->L.n[2];
+    rep->gstencil_per_s = pts * (double)steps / (rep->elapsed_ms * 1e-3) / 1e9;
+    rep->launches = (p->launches - l0) / reps;
+    rep->fusion_depth = p->halo_uniform ? p->vmain->kf : 1;
+    rep->variant = p->vmain->id;
+  }
+  return CS_OK;
+}
+
+cs_status cs_plan_sync(cs_plan *p) {
+  if (!p) return CS_ERR_INVALID;
+  CS_HIP(hipStreamSynchronize(p->stream));
+  return CS_OK;
+}
+
+cs_status cs_plan_download(cs_plan *p, double *host_out) {
+  if (!p || !host_out) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(p->device));
+  CS_HIP(hipMemcpyAsync(host_out, p->buf[p->cur], (size_t)cell_count(&p->L) * sizeof(double), hipMemcpyDeviceToHost,
+                        p->stream));
+  CS_HIP(hipStreamSynchronize(p->stream));
+  return CS_OK;
+}
+
+cs_status cs_plan_device_buffers(cs_plan *p, double **current, double **other) {
+  if (!p) return CS_ERR_INVALID;
+  if (current) *current = p->buf[p->cur];
+  if (other) *other = p->buf[p->cur ^ 1];
+  return CS_OK;
+}
+
+cs_status cs_plan_describe(cs_plan *p, char *buf, int32_t len) {
+  if (!p || !buf || len <= 0) return CS_ERR_INVALID;
+  snprintf(buf, (size_t)len, "variant=%d name=%s mf=%d R=%d shape=%s kf=%d tile=%dx%d waves=%d lds=%zu halo_uniform=%d",
+           p->vmain->id, p->vmain->name, p->vmain->mf, p->R, p->shape == cs::SHAPE_BOX ? "box" : "star", p->vmain->kf,
+           p->vmain->by, p->vmain->bx, p->vmain->nw, p->vmain->lds_bytes, (int)p->halo_uniform);
+  return CS_OK;
+}
+
+cs_status cs_run(const cs_layout *layout, const cs_stencil *st, const double *host_in, double *host_out, int64_t steps,
+                 const cs_options *opt, cs_report *rep) {
+  if (!host_in || !host_out || steps < 0) return CS_ERR_INVALID;
+  cs_plan *p = nullptr;
+  CS_TRY(cs_plan_create(&p, layout, st, opt));
+  cs_status s = cs_plan_upload(p, host_in);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s == CS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) s = CS_ERR_DEVICE;
+  if (s == CS_OK && hipEventRecord(e0, p->stream) != hipSuccess) s = CS_ERR_DEVICE;
+  if (s == CS_OK) s = run_steps(p, steps);
+  if (s == CS_OK && hipEventRecord(e1, p->stream) != hipSuccess) s = CS_ERR_DEVICE;
+  if (s == CS_OK) s = cs_plan_download(p, host_out);
+  if (s == CS_OK && rep) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    memset(rep, 0, sizeof(*rep));
+    rep->elapsed_ms = ms;
+    const double pts = (double)layout->n[0] * (double)layout->n[1] * (double)layout->n[2];
+    rep->gstencil_per_s = ms > 0 ? pts * (double)steps / (ms * 1e-3) / 1e9 : 0.0;
+    rep->launches = p->launches;
+    rep->fusion_depth = p->halo_uniform ? p->vmain->kf : 1;
+    rep->variant = p->vmain->id;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  cs_plan_destroy(p);
+  return s;
+}
+
+}  // extern "C"

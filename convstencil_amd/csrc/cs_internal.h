@@ -1,0 +1,45 @@
+// cs_internal.h -- types shared by the host runtime and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include "cs_ktable.h"
+
+namespace cs {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Geometry of one padded grid as the kernels see it.  All coordinates slowest first.
+struct Geo {
+  int64_t ext[3];    // padded extents
+  int64_t org[3];    // interior origin
+  int64_t n[3];      // interior sizes
+  int64_t tiles[3];  // tiles per axis for this launch
+};
+
+typedef hipError_t (*LaunchFn)(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s);
+
+// One kernel variant of the FP64 stencil2row families.
+struct Variant {
+  int id;
+  int mf;      // MFMA shape: 16 = v_mfma_f64_16x16x4, 4 = v_mfma_f64_4x4x4
+  int R, shape, kf, by, bx, nw;
+  size_t lds_bytes;
+  int px;  // LDS row pitch in doubles
+  LaunchFn launch;  // one launch = kf fused steps from `in` to `out`
+  hipError_t (*prepare)();  // one-time function attributes (dynamic LDS above 64 KiB)
+  const char *name;
+};
+
+const Variant *variants_2d_f64(int *count);
+
+// Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].
+void build_bfrag_2d_f64(int mf, int R, int shape, const double *w_dense, double *bfrag);
+int bfrag_ksteps_2d_f64(int mf, int R, int shape);
+
+// Device-side synthetic input fill (same bits as the host generator).
+hipError_t launch_fill_random(double *a, const Geo &g, uint64_t stream_seed, hipStream_t s);
+hipError_t launch_zero(double *a, size_t count, hipStream_t s);
+
+}  // namespace cs

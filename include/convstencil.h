@@ -1,0 +1,155 @@
+/*
+ * convstencil.h -- C ABI of the MI355X (gfx950) ConvStencil library.
+ *
+ * Two layers:
+ *
+ * 1. Drop-in launchers with the reference's names and argument meaning.  The
+ *    reference declares them (with C++ linkage and a void return) in
+ *      src/1d/1d_utils.h:75,85   gpu_1d1r, gpu_star_1d2r_step2
+ *      src/2d/2d_utils.h:77-79   gpu_box_2d1r, gpu_box_2d3r
+ *      src/3d/3d_utils.h:75-77   gpu_box_3d1r, gpu_star_3d1r
+ *    and implements them in src/1d/gpu_1r.cu:227-301, src/1d/gpu_2r.cu:85-159,
+ *    src/2d/gpu.cu:245-333,336-425, src/3d/gpu_box.cu:446-536,
+ *    src/3d/gpu_star.cu:238-332.  Same host-pointer arguments, same padded
+ *    layouts, same weight layouts, same stdout lines ("ConvStencil(2D): ",
+ *    "Time = N[ms]", "GStencil/s = X").  `times` counts launches as in the
+ *    reference (each launch applies the given, already time-fused, weights once).
+ *    Where the reference prints a CUDA error and exits, these print the error and
+ *    return a non-zero cs_status; the CLIs turn that into exit code 1.
+ *
+ * 2. A plan API (cs_plan_*) for device-resident time loops: many steps on grids
+ *    that stay in HBM, optional hipGraph capture, split precision, temporal
+ *    fusion, device-pointer access for benchmarking.  This has no single
+ *    counterpart in the reference; it generalises the launchers' bodies
+ *    (B-matrix build, ping-pong buffers, time loop) so that they can be reused.
+ *
+ * All functions are thread-compatible: distinct plans may be used from distinct
+ * threads; a single plan must not be used concurrently.
+ */
+#ifndef CONVSTENCIL_H
+#define CONVSTENCIL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cs_status {
+  CS_OK = 0,
+  CS_ERR_INVALID = 1,     /* bad argument: null pointer, bad size, halo narrower than the radius ... */
+  CS_ERR_UNSUPPORTED = 2, /* valid request this build has no kernel for */
+  CS_ERR_ALLOC = 3,       /* device or host allocation failed */
+  CS_ERR_DEVICE = 4,      /* a HIP call failed */
+  CS_ERR_NO_DEVICE = 5    /* no gfx950 device visible */
+} cs_status;
+
+typedef enum cs_precision {
+  CS_PREC_FP64 = 0,   /* plain FP64: v_mfma_f64_16x16x4 stencil2row */
+  CS_PREC_BF16X3 = 1, /* state/weights split into 3 BF16 slices, BF16 MFMA, FP32 accumulate, FP64 recombine */
+  CS_PREC_FP32X2 = 2  /* state/weights split into FP32 hi/lo, FP32 MFMA, FP64 recombine */
+} cs_precision;
+
+/* Padded array layout.  Dimensions are listed slowest first; a 2-D grid has
+ * ext[0] = n[0] = 1, org[0] = 0 and a 1-D grid also has ext[1] = n[1] = 1.
+ * Element (z, y, x) of the padded array is at ((z * ext[1]) + y) * ext[2] + x.
+ * The interior is [org[d], org[d] + n[d]) on every axis; every other cell is
+ * halo, read by the stencil and never written. */
+typedef struct cs_layout {
+  int32_t ndim;   /* 1, 2 or 3 */
+  int32_t reserved;
+  int64_t ext[3];
+  int64_t org[3];
+  int64_t n[3];
+} cs_layout;
+
+/* A stencil of radius r in ndim dimensions is given by its dense weight cube
+ * w[(2r+1)^ndim] in row-major (z, y, x) order, tap (dz, dy, dx) at index
+ * ((dz + r) * (2r+1) + (dy + r)) * (2r+1) + (dx + r); taps that are not part of
+ * the stencil are 0.  Radius 1..3 is supported. */
+typedef struct cs_stencil {
+  int32_t ndim;
+  int32_t radius;
+  const double *weights;
+} cs_stencil;
+
+typedef struct cs_options {
+  int32_t precision;     /* cs_precision */
+  int32_t fusion_depth;  /* time steps per launch; 0 = choose (autotuner / defaults) */
+  int32_t use_graph;     /* 1 = capture the launch sequence of cs_plan_run in a hipGraph */
+  int32_t tile_variant;  /* 0 = choose; otherwise a kernel-variant id from cs_plan_describe */
+  int32_t verbose;
+} cs_options;
+
+typedef struct cs_report {
+  double elapsed_ms;       /* device time of the time loop (HIP events on the plan stream) */
+  double gstencil_per_s;   /* interior points * steps / elapsed / 1e9 */
+  int64_t launches;        /* kernel launches issued */
+  int32_t fusion_depth;    /* executed steps per launch */
+  int32_t variant;         /* kernel variant used */
+} cs_report;
+
+typedef struct cs_plan cs_plan;
+
+/* ---- library ---- */
+const char *cs_version(void);
+const char *cs_status_string(cs_status s);
+/* Fills *count with the number of visible devices whose gfx name is gfx950. */
+cs_status cs_device_count(int32_t *count);
+
+/* ---- plans: state lives in two device buffers (ping-pong) ---- */
+cs_status cs_plan_create(cs_plan **plan, const cs_layout *layout, const cs_stencil *stencil, const cs_options *opt);
+cs_status cs_plan_destroy(cs_plan *plan);
+/* Copies the padded host array into buffer 0 and zeroes buffer 1 (the reference's
+ * cudaMemcpy + cudaMemset pair, src/2d/gpu.cu:273-277); resets the step count. */
+cs_status cs_plan_upload(cs_plan *plan, const double *host_in);
+/* Fills buffer 0 with the seeded synthetic grid on the device (interior uniform in
+ * [-1,1) from SplitMix64, halo 0; same bits as the host generator) and zeroes buffer 1. */
+cs_status cs_plan_fill_random(cs_plan *plan, uint64_t seed);
+/* Advances the state by `steps` physical steps (asynchronous on the plan stream). */
+cs_status cs_plan_run(cs_plan *plan, int64_t steps, cs_report *report);
+/* Runs `steps` once untimed, then `reps` timed repetitions of `steps` steps; reports
+ * the mean device time per repetition in report->elapsed_ms. */
+cs_status cs_plan_time(cs_plan *plan, int64_t steps, int32_t warmup, int32_t reps, cs_report *report);
+cs_status cs_plan_sync(cs_plan *plan);
+/* Copies the whole padded array holding the current state to the host. */
+cs_status cs_plan_download(cs_plan *plan, double *host_out);
+/* Device pointer of the buffer holding the current state / the other buffer. */
+cs_status cs_plan_device_buffers(cs_plan *plan, double **current, double **other);
+/* Human-readable description of the kernel choice (variant, tile, fusion depth). */
+cs_status cs_plan_describe(cs_plan *plan, char *buf, int32_t len);
+
+/* ---- one-shot host-to-host run (what the reference launchers do, minus printing) ---- */
+cs_status cs_run(const cs_layout *layout, const cs_stencil *stencil, const double *host_in, double *host_out,
+                 int64_t steps, const cs_options *opt, cs_report *report);
+
+/* ---- synthetic input generator (host side; bit-identical to cs_plan_fill_random) ---- */
+void cs_fill_random_host(double *a, const cs_layout *layout, uint64_t seed);
+void cs_fill_weights_host(double *w, int32_t count, uint64_t seed);
+
+/* ---- drop-in launchers (reference names; see the header comment) ---- */
+/* 1d1r: host arrays of input_n + 7 doubles; interior at [4, input_n + 4) of `in`;
+ * 7 fused weights; out[j] = final[j + 1] for j < input_n + 6  (src/1d/gpu_1r.cu:227-301). */
+int gpu_1d1r(const double *in, double *out, const double *params, int times, int input_n);
+/* 1d2r: host arrays of input_n + 8 doubles; interior [4, input_n + 4); 9 fused weights;
+ * out[j] = final[j] for j < input_n + 7  (src/1d/gpu_2r.cu:85-159). */
+int gpu_star_1d2r_step2(const double *in, double *out, const double *params, int times, int input_n);
+/* 2D: host arrays of (input_m + 6) x (input_n + 8); interior rows [3, m+3), cols [4, n+4);
+ * 49 weights row-major over (dy, dx) in [-3,3]^2  (src/2d/gpu.cu:245-333).  Prints with the
+ * 3-steps-per-launch GStencil/s formula of src/2d/gpu.cu:329. */
+int gpu_box_2d1r(const double *in, double *out, const double *params, int times, int input_m, int input_n);
+/* Same computation, 1 step per launch in the GStencil/s line (src/2d/gpu.cu:336-425). */
+int gpu_box_2d3r(const double *in, double *out, const double *params, int times, int input_m, int input_n);
+/* 3D box: (h+6)(m+6)(n+6) arrays, interior [3, .+3); 196 weights = 4 planes of 7x7 for
+ * |dz| = 3, 2, 1, 0 (z-symmetric 343-point stencil)  (src/3d/gpu_box.cu:446-536). */
+int gpu_box_3d1r(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                 int input_n);
+/* 3D star: same layout; 44 weights for the 63-point octahedron |dz|+|dy|+|dx| <= 3, planes
+ * |dz| = 3, 2, 1, 0 with 1, 5, 13, 25 taps, z-symmetric  (src/3d/gpu_star.cu:238-332). */
+int gpu_star_3d1r(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                  int input_n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CONVSTENCIL_H */
