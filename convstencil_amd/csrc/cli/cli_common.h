@@ -1,0 +1,144 @@
+// cli_common.h -- shared pieces of the convstencil_{1d,2d,3d} command lines.
+//
+// The positional grammar, prompts, messages and exit codes are the reference's
+// (src/1d/main.cu:49-100, src/2d/main.cu:117-181, src/3d/main.cu:469-522).  Flags of
+// the form --name=value are additions; they are removed from argv before the
+// reference grammar is applied, so `prog shape sizes... T [--custom]` behaves as in
+// the reference:
+//   --steps=T        physical-stencil mode: the shape's own stencil (1d1r = 3 points,
+//                    box2d1r = 9, ...) with a zero halo of its radius, seeded input,
+//                    T physical time steps, fused in the kernels.  --custom then reads
+//                    the shape's taps (3, 5, 5, 9, 13, 49, 7 or 27 values).
+//   --seed=S         seed of the synthetic grid and weights (physical mode)
+//   --fusion=K       fusion depth (0 = library default)
+//   --graph=0|1      capture the time loop in a hipGraph (default 1)
+//   --json           also print a JSON record (physical mode)
+//   --exact-fusion   2-D reference mode: compose the 3x3 weights exactly instead of
+//                    using the reference's 49-entry table
+#pragma once
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/convstencil.h"
+
+namespace cli {
+
+struct Ext {
+  long long steps = -1;
+  unsigned long long seed = 1234;
+  int fusion = 0;
+  int graph = 1;
+  bool json = false;
+  bool exact_fusion = false;
+};
+
+// Removes the additive flags from argv; returns false (after printing a message)
+// on a malformed one.
+inline bool strip_flags(int &argc, char **argv, Ext &e) {
+  int w = 1;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    auto val = [&](const char *k) -> const char * {
+      const size_t n = strlen(k);
+      return a.compare(0, n, k) == 0 ? argv[i] + n : nullptr;
+    };
+    try {
+      if (const char *v = val("--steps=")) {
+        e.steps = std::stoll(v);
+      } else if (const char *v2 = val("--seed=")) {
+        e.seed = std::stoull(v2);
+      } else if (const char *v3 = val("--fusion=")) {
+        e.fusion = std::stoi(v3);
+      } else if (const char *v4 = val("--graph=")) {
+        e.graph = std::stoi(v4);
+      } else if (a == "--json") {
+        e.json = true;
+      } else if (a == "--exact-fusion") {
+        e.exact_fusion = true;
+      } else {
+        argv[w++] = argv[i];
+      }
+    } catch (const std::exception &) {
+      std::cerr << "Invalid argument: cannot convert the parameter(s) to integer.\n";
+      return false;
+    }
+  }
+  argc = w;
+  argv[argc] = nullptr;
+  return true;
+}
+
+// Reads n doubles from stdin after the reference's prompt; false if one is missing.
+inline bool read_params(int n, double *out) {
+  printf("Please enter %d parameters:\n", n);
+  fflush(stdout);
+  for (int i = 0; i < n; ++i)
+    if (scanf("%lf", &out[i]) != 1) return false;
+  return true;
+}
+
+// Physical-stencil mode: dense weights of radius r in ndim dims, interior sizes
+// (slowest first), zero halo of width r.
+inline int run_physical(const char *tag, int ndim, int r, const std::vector<double> &dense,
+                        const std::vector<long long> &sizes, const Ext &e) {
+  cs_layout L{};
+  L.ndim = ndim;
+  for (int d = 0; d < 3; ++d) {
+    L.ext[d] = 1;
+    L.n[d] = 1;
+    L.org[d] = 0;
+  }
+  for (int k = 0; k < ndim; ++k) {
+    const int d = 3 - ndim + k;
+    L.n[d] = sizes[k];
+    L.org[d] = r;
+    L.ext[d] = sizes[k] + 2 * r;
+  }
+  cs_stencil st{ndim, r, dense.data()};
+  cs_options opt{};
+  opt.fusion_depth = e.fusion;
+  opt.use_graph = e.graph;
+  cs_plan *p = nullptr;
+  cs_status s = cs_plan_create(&p, &L, &st, &opt);
+  if (s != CS_OK) {
+    fprintf(stderr, "convstencil: %s\n", cs_status_string(s));
+    return 1;
+  }
+  s = cs_plan_fill_random(p, e.seed);
+  cs_report rep{};
+  if (s == CS_OK) s = cs_plan_time(p, e.steps, 1, 1, &rep);
+  char desc[512] = {0};
+  cs_plan_describe(p, desc, sizeof desc);
+  cs_plan_destroy(p);
+  if (s != CS_OK) {
+    fprintf(stderr, "convstencil: %s\n", cs_status_string(s));
+    return 1;
+  }
+  printf("ConvStencil(%s): \n", tag);
+  printf("Time = %lld[ms]\n", (long long)rep.elapsed_ms);
+  printf("GStencil/s = %f\n", rep.gstencil_per_s);
+  if (e.json)
+    printf("{\"mode\": \"physical\", \"steps\": %lld, \"seed\": %llu, \"elapsed_ms\": %.6f, \"gstencil_per_s\": %.4f, "
+           "\"launches\": %lld, \"fusion_depth\": %d, \"plan\": \"%s\"}\n",
+           e.steps, e.seed, rep.elapsed_ms, rep.gstencil_per_s, (long long)rep.launches, rep.fusion_depth, desc);
+  fflush(stdout);
+  return 0;
+}
+
+// Seeded physical weights (uniform in (0,1), normalised) unless --custom supplied them.
+inline std::vector<double> physical_taps(int count, bool custom, unsigned long long seed, bool &ok) {
+  std::vector<double> w((size_t)count);
+  ok = true;
+  if (custom)
+    ok = read_params(count, w.data());
+  else
+    cs_fill_weights_host(w.data(), count, seed);
+  return w;
+}
+
+}  // namespace cli

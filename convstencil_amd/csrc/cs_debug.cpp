@@ -3,22 +3,31 @@
 
 #include <cstring>
 
+#include "cs_fused_weights.h"
 #include "cs_internal.h"
 
 extern "C" {
 
-int32_t cs_debug_variant_count(void) {
+static const cs::Variant *family(int32_t ndim, int *n) {
+  *n = 0;
+  if (ndim == 1) return cs::variants_1d_f64(n);
+  if (ndim == 2) return cs::variants_2d_f64(n);
+  if (ndim == 3) return cs::variants_3d_f64(n);
+  return nullptr;
+}
+
+int32_t cs_debug_variant_count(int32_t ndim) {
   int n = 0;
-  (void)cs::variants_2d_f64(&n);
+  (void)family(ndim, &n);
   return n;
 }
 
-int32_t cs_debug_variant_info(int32_t i, int64_t *info, char *name, int32_t len) {
+int32_t cs_debug_variant_info(int32_t ndim, int32_t i, int64_t *info, char *name, int32_t len) {
   int n = 0;
-  const cs::Variant *v = cs::variants_2d_f64(&n);
+  const cs::Variant *v = family(ndim, &n);
   if (i < 0 || i >= n || !info) return -1;
   const cs::Variant &x = v[i];
-  const int64_t vals[10] = {x.id, x.mf, x.R, x.shape, x.kf, x.by, x.bx, x.nw, (int64_t)x.lds_bytes, x.px};
+  const int64_t vals[11] = {x.id, x.mf, x.R, x.shape, x.kf, x.by, x.bx, x.nw, (int64_t)x.lds_bytes, x.px, x.st};
   memcpy(info, vals, sizeof(vals));
   if (name && len > 0) {
     strncpy(name, x.name, (size_t)len - 1);
@@ -44,6 +53,42 @@ int32_t cs_debug_bfrag_2d(int32_t mf, int32_t radius, int32_t shape, const doubl
   if (nks * 64 > cap) return -1;
   cs::build_bfrag_2d_f64(mf, radius, shape, w, bfrag);
   return nks;
+}
+
+int32_t cs_debug_bfrag_1d(int32_t radius, const double *w, double *bfrag, int32_t cap) {
+  const int nks = cs::bfrag_ksteps_1d_f64(radius);
+  if (nks * 64 > cap) return -1;
+  cs::build_bfrag_1d_f64(radius, w, bfrag);
+  return nks;
+}
+
+int32_t cs_debug_ktable3d(int32_t radius, int32_t shape, int32_t *dz, int32_t *dy, int32_t *c, int32_t *live,
+                          int32_t cap) {
+  const cs::KTable3D t = cs::make_ktable3d(radius, shape);
+  if (t.n > cap) return -1;
+  for (int i = 0; i < t.n; ++i) {
+    dz[i] = t.dz[i];
+    for (int k = 0; k < 4; ++k) {
+      dy[4 * i + k] = t.dy[i][k];
+      c[4 * i + k] = t.c[i][k];
+      live[4 * i + k] = t.live[i][k];
+    }
+  }
+  return t.n;
+}
+
+int32_t cs_debug_bfrag_3d(int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap) {
+  const int nks = cs::bfrag_ksteps_3d_f64(radius, shape);
+  if (nks * 64 > cap) return -1;
+  cs::build_bfrag_3d_f64(radius, shape, w, bfrag);
+  return nks;
+}
+
+void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49) {
+  if (exact)
+    cs::fused_2d_exact(p9, out49);
+  else
+    cs::fused_2d_reference(p9, out49);
 }
 
 }  // extern "C"

@@ -68,6 +68,21 @@ int classify_2d(int R, const double *w) {
   return cs::SHAPE_STAR;
 }
 
+// 3-D shape class: star (taps on the axes), diamond (|dz|+|dy|+|dx| <= R) or box.
+int classify_3d(int R, const double *w) {
+  const int D = 2 * R + 1;
+  bool star = true, diamond = true;
+  for (int dz = -R; dz <= R; ++dz)
+    for (int dy = -R; dy <= R; ++dy)
+      for (int dx = -R; dx <= R; ++dx) {
+        if (w[((dz + R) * D + (dy + R)) * D + (dx + R)] == 0.0) continue;
+        const int nz = (dz != 0) + (dy != 0) + (dx != 0);
+        if (nz > 1) star = false;
+        if (std::abs(dz) + std::abs(dy) + std::abs(dx) > R) diamond = false;
+      }
+  return star ? cs::SHAPE3_STAR : diamond ? cs::SHAPE3_DIAMOND : cs::SHAPE3_BOX;
+}
+
 // True when every halo cell of the padded host array is zero.
 bool halo_is_zero(const cs_layout *L, const double *a) {
   for (int64_t z = 0; z < L->ext[0]; ++z)
@@ -136,9 +151,25 @@ Geo make_geo(const cs_layout &L, const Variant *v) {
   return g;
 }
 
-const Variant *find_variant(int mf, int R, int shape, int kf, int prefer_id) {
+// Default fusion depth when the caller does not choose one (the autotuner's
+// cache, when present, overrides this).
+int default_kf(int ndim, int R, int top) {
+  if (ndim == 1) return std::min(top, 32);
+  if (ndim == 3) return 1;
+  return std::min(top, R == 1 ? 4 : 2);
+}
+
+const Variant *family(int ndim, int *cnt) {
+  *cnt = 0;
+  if (ndim == 1) return cs::variants_1d_f64(cnt);
+  if (ndim == 2) return cs::variants_2d_f64(cnt);
+  if (ndim == 3) return cs::variants_3d_f64(cnt);
+  return nullptr;
+}
+
+const Variant *find_variant(int ndim, int mf, int R, int shape, int kf, int prefer_id) {
   int cnt = 0;
-  const Variant *vs = cs::variants_2d_f64(&cnt);
+  const Variant *vs = family(ndim, &cnt);
   const Variant *best = nullptr;
   for (int i = 0; i < cnt; ++i) {
     const Variant &v = vs[i];
@@ -148,39 +179,47 @@ const Variant *find_variant(int mf, int R, int shape, int kf, int prefer_id) {
   return best;
 }
 
-int max_kf(int mf, int R, int shape) {
+int max_kf(int ndim, int mf, int R, int shape) {
   int cnt = 0, m = 0;
-  const Variant *vs = cs::variants_2d_f64(&cnt);
+  const Variant *vs = family(ndim, &cnt);
   for (int i = 0; i < cnt; ++i)
     if (vs[i].mf == mf && vs[i].R == R && vs[i].shape == shape) m = std::max(m, vs[i].kf);
   return m;
 }
 
 cs_status select_variants(cs_plan *p) {
-  if (p->ndim != 2) return CS_ERR_UNSUPPORTED;
+  const int nd = p->ndim;
+  if (nd < 1 || nd > 3) return CS_ERR_UNSUPPORTED;
   const int mf = 4;
   const Variant *v = nullptr;
   if (p->opt.tile_variant > 0) {
-    v = find_variant(0, p->R, p->shape, 0, p->opt.tile_variant);
+    v = find_variant(nd, 0, p->R, p->shape, 0, p->opt.tile_variant);
     if (!v) return CS_ERR_INVALID;
   } else {
-    int kf = p->opt.fusion_depth > 0 ? p->opt.fusion_depth : max_kf(mf, p->R, p->shape);
-    kf = std::min(kf, max_kf(mf, p->R, p->shape));
-    for (; kf >= 1 && !v; --kf) v = find_variant(mf, p->R, p->shape, kf, 0);
+    const int top = max_kf(nd, mf, p->R, p->shape);
+    int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
+    for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, p->shape, kf, 0);
   }
   if (!v) return CS_ERR_UNSUPPORTED;
   p->vmain = v;
-  p->vone = find_variant(v->mf, p->R, p->shape, 1, 0);
+  p->vone = find_variant(nd, v->mf, p->R, p->shape, 1, 0);
   if (!p->vone) return CS_ERR_UNSUPPORTED;
   p->vrem.assign(v->kf, nullptr);
-  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(v->mf, p->R, p->shape, r, 0);
+  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0);
   for (const Variant *u : {p->vmain, p->vone}) CS_HIP(u->prepare());
   for (const Variant *u : p->vrem)
     if (u) CS_HIP(u->prepare());
   // B fragments for the family's MFMA shape
-  const int nks = cs::bfrag_ksteps_2d_f64(v->mf, p->R, p->shape);
+  const int nks = nd == 1   ? cs::bfrag_ksteps_1d_f64(p->R)
+                  : nd == 2 ? cs::bfrag_ksteps_2d_f64(v->mf, p->R, p->shape)
+                            : cs::bfrag_ksteps_3d_f64(p->R, p->shape);
   std::vector<double> h((size_t)nks * 64);
-  cs::build_bfrag_2d_f64(v->mf, p->R, p->shape, p->w.data(), h.data());
+  if (nd == 1)
+    cs::build_bfrag_1d_f64(p->R, p->w.data(), h.data());
+  else if (nd == 2)
+    cs::build_bfrag_2d_f64(v->mf, p->R, p->shape, p->w.data(), h.data());
+  else
+    cs::build_bfrag_3d_f64(p->R, p->shape, p->w.data(), h.data());
   if (p->bfrag) (void)hipFree(p->bfrag);
   p->bfrag = nullptr;
   CS_HIP(hipMalloc(&p->bfrag, h.size() * sizeof(double)));
@@ -196,34 +235,42 @@ cs_status launch_one(cs_plan *p, const Variant *v) {
   return CS_OK;
 }
 
-// Issues the launches for `steps` physical steps.  With uniform halos, fused
-// launches of kf steps are exact; otherwise every launch is a single step so that
-// each step reads the halo of its own buffer, as the reference's ping-pong does.
-cs_status issue_steps(cs_plan *p, int64_t steps) {
-  if (steps <= 0) return CS_OK;
+// Launch sequence for `steps` physical steps.  With uniform halos, fused
+// launches are exact: take the main depth as often as it fits, then the largest
+// compiled depth that fits the rest.  Otherwise every launch is a single step so
+// that each step reads the halo of its own buffer, as the reference's ping-pong
+// does.
+void plan_sequence(const cs_plan *p, int64_t steps, std::vector<const Variant *> *seq) {
+  seq->clear();
+  if (steps <= 0) return;
   if (!p->halo_uniform || p->vmain->kf == 1) {
-    for (int64_t t = 0; t < steps; ++t) CS_TRY(launch_one(p, p->vone));
-    return CS_OK;
+    seq->assign((size_t)steps, p->vone);
+    return;
   }
   const int kf = p->vmain->kf;
-  for (int64_t t = 0; t < steps / kf; ++t) CS_TRY(launch_one(p, p->vmain));
-  const int r = (int)(steps % kf);
-  if (r > 0) {
-    if (p->vrem[r])
-      CS_TRY(launch_one(p, p->vrem[r]));
-    else
-      for (int t = 0; t < r; ++t) CS_TRY(launch_one(p, p->vone));
+  seq->assign((size_t)(steps / kf), p->vmain);
+  int64_t rem = steps % kf;
+  while (rem > 0) {
+    const Variant *best = p->vone;
+    for (const Variant *u : p->vrem)
+      if (u && u->kf <= rem && u->kf > best->kf) best = u;
+    seq->push_back(best);
+    rem -= best->kf;
   }
+}
+
+cs_status issue_steps(cs_plan *p, int64_t steps) {
+  std::vector<const Variant *> seq;
+  plan_sequence(p, steps, &seq);
+  for (const Variant *v : seq) CS_TRY(launch_one(p, v));
   return CS_OK;
 }
 
 // Number of launches issue_steps() makes for `steps` steps.
 int64_t count_launches(const cs_plan *p, int64_t steps) {
-  if (steps <= 0) return 0;
-  if (!p->halo_uniform || p->vmain->kf == 1) return steps;
-  const int kf = p->vmain->kf;
-  const int r = (int)(steps % kf);
-  return steps / kf + (r > 0 ? (p->vrem[r] ? 1 : r) : 0);
+  std::vector<const Variant *> seq;
+  plan_sequence(p, steps, &seq);
+  return (int64_t)seq.size();
 }
 
 cs_status run_steps(cs_plan *p, int64_t steps) {
@@ -342,7 +389,7 @@ cs_status cs_plan_create(cs_plan **out, const cs_layout *layout, const cs_stenci
   p->w.assign(st->weights, st->weights + nw);
   if (opt) p->opt = *opt;
   p->device = dev;
-  p->shape = p->ndim == 2 ? classify_2d(p->R, p->w.data()) : cs::SHAPE_BOX;
+  p->shape = p->ndim == 2 ? classify_2d(p->R, p->w.data()) : p->ndim == 3 ? classify_3d(p->R, p->w.data()) : cs::SHAPE_BOX;
   cs_status s = select_variants(p);
   if (s != CS_OK) {
     cs_plan_destroy(p);

@@ -27,16 +27,23 @@ struct Variant {
   int R, shape, kf, by, bx, nw;
   size_t lds_bytes;
   int px;  // LDS row pitch in doubles
+  int st;  // fused-step staging: 0 = two LDS buffers, 1 = one buffer + registers
   LaunchFn launch;  // one launch = kf fused steps from `in` to `out`
   hipError_t (*prepare)();  // one-time function attributes (dynamic LDS above 64 KiB)
   const char *name;
 };
 
+const Variant *variants_1d_f64(int *count);
 const Variant *variants_2d_f64(int *count);
+const Variant *variants_3d_f64(int *count);
 
 // Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].
 void build_bfrag_2d_f64(int mf, int R, int shape, const double *w_dense, double *bfrag);
 int bfrag_ksteps_2d_f64(int mf, int R, int shape);
+void build_bfrag_1d_f64(int R, const double *w_dense, double *bfrag);
+int bfrag_ksteps_1d_f64(int R);
+void build_bfrag_3d_f64(int R, int shape, const double *w_dense, double *bfrag);
+int bfrag_ksteps_3d_f64(int R, int shape);
 
 // Device-side synthetic input fill (same bits as the host generator).
 hipError_t launch_fill_random(double *a, const Geo &g, uint64_t stream_seed, hipStream_t s);

@@ -99,3 +99,77 @@ constexpr int ktable_overrun(const KTable &t) {
 }
 
 }  // namespace cs
+
+namespace cs {
+
+// ---- 3-D tables (v_mfma_f64_4x4x4, 4 x 16 output strips of one z-plane) ----
+//
+// K is grouped by dz so that every k-step reads a single LDS plane (the plane's
+// ring slot is then uniform over the wave); inside a dz group the (dy, c) entries
+// are flattened as in make_ftable and padded to a multiple of 4.  Shape classes:
+// box (every tap), star (taps on the axes), diamond (|dz| + |dy| + |dx| <= R: the
+// reference's 63-point star3d1r is the radius-3 diamond, src/3d/main.cu:42-113).
+enum Shape3D : int { SHAPE3_BOX = 0, SHAPE3_STAR = 1, SHAPE3_DIAMOND = 2 };
+
+struct KTable3D {
+  int n;             // k-steps
+  int dz[160];       // plane of each k-step
+  int dy[160][4];    // per k-step, per k lane-group: row offset
+  int c[160][4];     // column offset relative to the 4-wide block
+  int live[160][4];  // 0 for padding entries (zero weight)
+};
+
+constexpr void row_span_3d(int R, int shape, int dz, int dy, int &lo, int &hi) {
+  const int adz = dz < 0 ? -dz : dz, ady = dy < 0 ? -dy : dy;
+  lo = 0;
+  hi = -1;
+  if (shape == SHAPE3_BOX) {
+    lo = -R;
+    hi = R;
+  } else if (shape == SHAPE3_STAR) {
+    if (dz == 0 && dy == 0) {
+      lo = -R;
+      hi = R;
+    } else if (dz == 0 || dy == 0) {
+      lo = 0;
+      hi = 0;
+    }
+  } else {
+    const int rem = R - adz - ady;
+    if (rem >= 0) {
+      lo = -rem;
+      hi = rem;
+    }
+  }
+}
+
+constexpr KTable3D make_ktable3d(int R, int shape) {
+  KTable3D t{};
+  t.n = 0;
+  for (int dz = -R; dz <= R; ++dz) {
+    int fdy[128] = {}, fc[128] = {};
+    int m = 0;
+    for (int dy = -R; dy <= R; ++dy) {
+      int lo = 0, hi = -1;
+      row_span_3d(R, shape, dz, dy, lo, hi);
+      for (int c = lo; c <= 3 + hi && lo <= hi; ++c) {
+        fdy[m] = dy;
+        fc[m] = c;
+        ++m;
+      }
+    }
+    for (int q = 0; q < (m + 3) / 4; ++q) {
+      t.dz[t.n] = dz;
+      for (int k = 0; k < 4; ++k) {
+        const int f = 4 * q + k;
+        t.dy[t.n][k] = f < m ? fdy[f] : 0;
+        t.c[t.n][k] = f < m ? fc[f] : 0;
+        t.live[t.n][k] = f < m ? 1 : 0;
+      }
+      ++t.n;
+    }
+  }
+  return t;
+}
+
+}  // namespace cs

@@ -15,12 +15,13 @@
 extern "C" {
 #endif
 
-/* Number of compiled 2-D FP64 kernel variants. */
-int32_t cs_debug_variant_count(void);
-/* info[0..9] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
- * tile rows, tile cols, waves, LDS bytes, LDS row pitch (doubles).  Returns 0, or -1 if i is out
- * of range.  name receives the variant's name (truncated to len). */
-int32_t cs_debug_variant_info(int32_t i, int64_t *info, char *name, int32_t len);
+/* Number of compiled FP64 kernel variants for grids of ndim dimensions. */
+int32_t cs_debug_variant_count(int32_t ndim);
+/* info[0..10] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
+ * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
+ * (doubles), staging of fused steps (0 = two LDS buffers, 1 = one buffer + registers; for 3-D
+ * variants: the z-chunk length).  Returns 0, or -1 if i is out of range.  name receives the variant's name. */
+int32_t cs_debug_variant_info(int32_t ndim, int32_t i, int64_t *info, char *name, int32_t len);
 /* k-step table of an MFMA shape: dy[], c0[], live[] (cap entries each); returns the table length
  * (k-steps for 16x16x4, flattened entries for 4x4x4), or -1 if cap is too small. */
 int32_t cs_debug_ktable(int32_t mf, int32_t radius, int32_t shape, int32_t *dy, int32_t *c0, int32_t *live,
@@ -28,6 +29,20 @@ int32_t cs_debug_ktable(int32_t mf, int32_t radius, int32_t shape, int32_t *dy, 
 /* B fragments bfrag[ks * 64 + lane] for dense weights w[(2r+1)^2]; returns the k-step count,
  * or -1 if cap (doubles) is too small. */
 int32_t cs_debug_bfrag_2d(int32_t mf, int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap);
+/* Same for the 1-D family (v_mfma_f64_4x4x4, weights w[2r+1]). */
+int32_t cs_debug_bfrag_1d(int32_t radius, const double *w, double *bfrag, int32_t cap);
+
+/* 3-D k-step table (grouped by dz): dz[ks], and dy/c/live[4 * ks + k] for lane group k; returns
+ * the k-step count or -1 if cap (k-steps) is too small.  shape: 0 box, 1 star, 2 diamond. */
+int32_t cs_debug_ktable3d(int32_t radius, int32_t shape, int32_t *dz, int32_t *dy, int32_t *c, int32_t *live,
+                          int32_t cap);
+/* 3-D B fragments bfrag[ks * 64 + lane] for dense weights w[(2r+1)^3]. */
+int32_t cs_debug_bfrag_3d(int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap);
+
+/* The 2-D CLI's fused weights: the 49 coefficients of the cube of a row-major 3x3 stencil, either
+ * exactly (exact != 0) or as the reference's hand-expanded table computes them
+ * (src/2d/main.cu:233-281). */
+void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49);
 
 /* Runs the FP64 MFMA lane-layout probe on the current device (needs a GPU).  which = 0:
  * v_mfma_f64_16x16x4, host_out[64 * 64 * 4] = D[lane][item] for A one-hot at lane p (p-major);

@@ -1,0 +1,120 @@
+"""Plain FP64 3-D path on the GPU against the CPU oracle (<= 1e-12 relative max).
+
+Covers star3d1r (7-point) and box3d1r (27-point) on ragged sizes and several
+steps, the BASELINE size (512^3, T = 32) checked on windows whose light cone the
+oracle recomputes exactly, and the reference's own launchers gpu_box_3d1r (343
+points, 196 z-symmetric weights) and gpu_star_3d1r (63-point octahedron, 44
+weights) with the reference's padded layout and a random non-zero halo.
+"""
+import numpy as np
+import pytest
+
+from convstencil_amd import Plan, Layout, shape_offsets, dense_weights, SHAPES, fill_random_host
+from convstencil_amd import api
+from oracle import oracle
+from refimpl import rel_errors
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def _check(L, got, want):
+    rm, _ = rel_errors(L.interior(got), L.interior(want))
+    assert rm <= TOL, rm
+
+
+@pytest.mark.parametrize("name", ["star3d1r", "box3d1r"])
+@pytest.mark.parametrize("sizes", [[7, 9, 11], [33, 40, 70], [3, 64, 128], [40, 33, 200]])
+@pytest.mark.parametrize("steps", [1, 2, 5])
+def test_parity(gpu, name, sizes, steps):
+    print(f"case {name} {sizes} T={steps}", flush=True)
+    ndim, r, kind = SHAPES[name]
+    L = Layout.with_halo(sizes, r)
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), sum(sizes) + steps)
+    x = oracle.fill_data(L, sum(sizes) + steps)
+    with Plan(L, r, dense_weights(3, r, offs, w)) as p:
+        p.upload(x)
+        p.run(steps)
+        got = p.download()
+    _check(L, got, oracle.run(L, offs, w, x, steps))
+
+
+@pytest.mark.parametrize("name", ["star3d1r", "box3d1r"])
+def test_baseline_size_windows(gpu, name):
+    n, steps = 512, 32
+    ndim, r, kind = SHAPES[name]
+    L = Layout.with_halo([n, n, n], r)
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), 77)
+    with Plan(L, r, dense_weights(3, r, offs, w), use_graph=True) as p:
+        p.fill_random(77)
+        p.run(steps)
+        out = p.download()
+    x = fill_random_host(L, 77)
+    cone = steps * r
+    worst = 0.0
+    for (z0, y0, x0) in [(0, 0, 0), (n - 16, n - 16, n - 16), (200, 37, 451), (511 - 20, 100, 0)]:
+        s = 16
+        lo = [max(0, c - cone) for c in (z0, y0, x0)]
+        hi = [min(n, c + s + cone) for c in (z0, y0, x0)]
+        sub = Layout.with_halo([h - l for l, h in zip(lo, hi)], r)
+        xs = np.zeros(sub.cells)
+        sub.interior(xs)[...] = L.interior(x)[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]]
+        ys = sub.interior(oracle.run(sub, offs, w, xs, steps, threads=0))
+        want = ys[z0 - lo[0]:z0 - lo[0] + s, y0 - lo[1]:y0 - lo[1] + s, x0 - lo[2]:x0 - lo[2] + s]
+        got = L.interior(out)[z0:z0 + s, y0:y0 + s, x0:x0 + s]
+        worst = max(worst, rel_errors(got, want)[0])
+    assert worst <= TOL, worst
+
+
+def _ref3d(h, m, n, seed):
+    L = Layout(3, [h + 6, m + 6, n + 6], [3, 3, 3], [h, m, n])
+    x = oracle.fill_data(L, seed)
+    rng = np.random.default_rng(seed)
+    a = x.reshape(L.ext)
+    mask = np.ones(L.ext, dtype=bool)
+    mask[3:3 + h, 3:3 + m, 3:3 + n] = False
+    a[mask] = rng.integers(0, 100, int(mask.sum())).astype(np.float64)
+    return L, a.reshape(-1)
+
+
+def test_reference_gpu_box_3d1r(gpu):
+    h, m, n, times = 9, 20, 70, 3
+    L, x = _ref3d(h, m, n, 3)
+    params = np.arange(196) / 100.0 / 50.0  # the reference's i/100 defaults, scaled to keep values small
+    out = np.zeros(L.cells)
+    api.gpu_box_3d1r(x, out, params, times, h, m, n)
+    offs, w = [], []
+    for dz in range(-3, 4):
+        k = 3 - abs(dz)
+        for dy in range(-3, 4):
+            for dx in range(-3, 4):
+                offs.append((dz, dy, dx))
+                w.append(params[k * 49 + (dy + 3) * 7 + (dx + 3)])
+    want = oracle.run(L, offs, w, x, times)
+    _check(L, out, want)
+    m3 = np.ones(L.ext, dtype=bool)
+    m3[3:3 + h, 3:3 + m, 3:3 + n] = False
+    assert np.array_equal(out.reshape(L.ext)[m3], want.reshape(L.ext)[m3])
+
+
+def test_reference_gpu_star_3d1r(gpu):
+    h, m, n, times = 8, 17, 66, 2
+    L, x = _ref3d(h, m, n, 4)
+    params = np.linspace(0.001, 0.03, 44)
+    out = np.zeros(L.cells)
+    api.gpu_star_3d1r(x, out, params, times, h, m, n)
+    base = {3: 0, 2: 1, 1: 6, 0: 19}
+    offs, w = [], []
+    for dz in range(-3, 4):
+        r = 3 - abs(dz)
+        idx = base[abs(dz)]
+        for dy in range(-r, r + 1):
+            span = r - abs(dy)
+            for dx in range(-span, span + 1):
+                offs.append((dz, dy, dx))
+                w.append(params[idx])
+                idx += 1
+    want = oracle.run(L, offs, w, x, times)
+    _check(L, out, want)

@@ -25,9 +25,12 @@ from oracle import oracle
 def _dbg():
     lib = _lib.load()
     lib.cs_debug_variant_count.restype = ctypes.c_int32
+    lib.cs_debug_variant_count.argtypes = [ctypes.c_int32]
     lib.cs_debug_variant_info.restype = ctypes.c_int32
-    lib.cs_debug_variant_info.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p,
-                                          ctypes.c_int32]
+    lib.cs_debug_variant_info.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.c_char_p, ctypes.c_int32]
+    lib.cs_debug_bfrag_1d.restype = ctypes.c_int32
+    lib.cs_debug_bfrag_1d.argtypes = [ctypes.c_int32, _lib.c_double_p, _lib.c_double_p, ctypes.c_int32]
     ip = ctypes.POINTER(ctypes.c_int32)
     lib.cs_debug_ktable.restype = ctypes.c_int32
     lib.cs_debug_ktable.argtypes = [ctypes.c_int32] * 3 + [ip, ip, ip, ctypes.c_int32]
@@ -36,14 +39,14 @@ def _dbg():
     return lib
 
 
-def variants():
+def variants(ndim=2):
     lib = _dbg()
     out = []
-    for i in range(lib.cs_debug_variant_count()):
-        info = (ctypes.c_int64 * 10)()
+    for i in range(lib.cs_debug_variant_count(ndim)):
+        info = (ctypes.c_int64 * 11)()
         name = ctypes.create_string_buffer(128)
-        assert lib.cs_debug_variant_info(i, info, name, 128) == 0
-        keys = ["id", "mf", "R", "shape", "kf", "by", "bx", "nw", "lds", "px"]
+        assert lib.cs_debug_variant_info(ndim, i, info, name, 128) == 0
+        keys = ["id", "mf", "R", "shape", "kf", "by", "bx", "nw", "lds", "px", "st"]
         d = dict(zip(keys, [int(v) for v in info]))
         d["name"] = name.value.decode()
         out.append(d)
@@ -124,7 +127,7 @@ def emulate(v, layout, dense, x):
     offs = lane_offsets(v, tab)
     EY, EX = by + 2 * kf * R, bx + 2 * kf * R
     BUF = EY * px
-    nbuf = 2 if kf > 1 else 1
+    nbuf = 2 if (kf > 1 and v["st"] == 0) else 1
     assert v["lds"] == nbuf * BUF * 8
     assert v["lds"] <= 160 * 1024
     ext, org, n = layout.ext, layout.org, layout.n
@@ -155,12 +158,13 @@ def emulate(v, layout, dense, x):
                 assert 0 <= gi < layout.cells
                 val = x[gi]
             lds[idx] = val
-            if kf > 1:
+            if nbuf == 2:
                 lds[BUF + idx] = val
         full = Y0 >= 0 and X0 >= 0 and Y0 + EY <= n[1] and X0 + EX <= n[2]
         for s in range(kf):
-            src = (s & 1) * BUF
-            dst = ((s + 1) & 1) * BUF
+            # ST = 0 ping-pongs two buffers; ST = 1 reads and (after a barrier) rewrites one
+            src = (s & 1) * BUF if nbuf == 2 else 0
+            dst = ((s + 1) & 1) * BUF if nbuf == 2 else 0
             H, W = by + 2 * R * (kf - 1 - s), bx + 2 * R * (kf - 1 - s)
             base = R * (s + 1)
             nby, nbx = -(-H // BH), -(-W // 16)
@@ -244,3 +248,218 @@ def test_ktables_cover_every_tap():
                     for n in range(width):
                         hits = [1 for (a, c) in cols if a == tdy and c - n == tdx]
                         assert len(hits) == 1, (mf, R, shape, tdy, tdx, n)
+
+
+# ---------------------------------------------------------------- 1-D family
+
+def emulate_1d(v, layout, w, x):
+    R, kf, L, P = v["R"], v["kf"], v["bx"], v["px"]
+    lib = _dbg()
+    cap = 64 * 16
+    bfr = np.zeros(cap)
+    wd = np.ascontiguousarray(w, dtype=np.float64)
+    nks = lib.cs_debug_bfrag_1d(R, wd.ctypes.data_as(_lib.c_double_p), bfr.ctypes.data_as(_lib.c_double_p), cap)
+    assert nks > 0
+    bf = bfr[:nks * 64].reshape(nks, 64)
+    lanes = np.arange(64)
+    k, b4, i = lanes >> 4, (lanes >> 2) & 3, lanes & 3
+    ne = 4 + 2 * R
+    offs = []
+    for ks in range(nks):
+        f = 4 * ks + k
+        c = np.where(f < ne, f - R, 0)
+        offs.append(16 * i + 4 * b4 + c)
+    offs = np.array(offs)
+    E = L + 2 * kf * R
+    assert P == E + 1
+    nbuf = 2 if kf > 1 else 1
+    assert v["lds"] == nbuf * P * 8 and v["lds"] <= 160 * 1024
+    ext, org, n = layout.ext[2], layout.org[2], layout.n[2]
+    out = np.zeros(layout.cells)
+    for tile in range(-(-n // L)):
+        X0 = tile * L - kf * R
+        ax0 = org + X0
+        lds = np.full(nbuf * P, np.nan)
+        for idx in range(P):
+            ax = ax0 + idx
+            val = x[ax] if (idx < E and 0 <= ax < ext) else 0.0
+            lds[idx] = val
+            if kf > 1:
+                lds[P + idx] = val
+        full = X0 >= 0 and X0 + E <= n
+        for s in range(kf):
+            src, dst = (s & 1) * P, ((s + 1) & 1) * P
+            W = L + 2 * R * (kf - 1 - s)
+            base = R * (s + 1)
+            pend = []
+            for blk in range(-(-W // 64)):
+                xx = base + min(64 * blk, W - 64)
+                idx = src + xx + offs
+                assert idx.min() >= src and idx.max() < src + P
+                a = lds[idx]
+                assert np.isfinite(a).all()
+                acc = np.zeros((4, 4, 4))
+                for ks in range(nks):
+                    A = np.zeros((4, 4, 4))
+                    B = np.zeros((4, 4, 4))
+                    A[b4, i, k] = a[ks]
+                    B[b4, k, lanes & 3] = bf[ks]
+                    acc += A @ B
+                vals = acc[b4, lanes >> 4, lanes & 3]
+                for ln in range(64):
+                    tx = xx + ln
+                    if not (full or 0 <= X0 + tx < n):
+                        continue
+                    if s < kf - 1:
+                        pend.append((dst + tx, vals[ln]))
+                    else:
+                        gi = ax0 + tx
+                        assert org <= gi < org + n
+                        out[gi] = vals[ln]
+            for li, vv in pend:
+                lds[li] = vv
+    return out
+
+
+VARIANTS_1D = variants(1)
+
+
+@pytest.mark.parametrize("v", VARIANTS_1D, ids=[v["name"] for v in VARIANTS_1D])
+def test_variant_emulation_1d(v):
+    R, kf = v["R"], v["kf"]
+    rng = np.random.default_rng(v["id"])
+    w = rng.uniform(0.05, 1.0, 2 * R + 1)
+    w /= w.sum()
+    for layout in [Layout.with_halo([v["bx"] + 77], R), Layout(1, [v["bx"] // 2 + 2 * R + 9], [R + 4],
+                                                                  [v["bx"] // 2 + 3])]:
+        x = oracle.fill_data(layout, v["id"])
+        got = emulate_1d(v, layout, w, x)
+        offs = [(0, 0, d) for d in range(-R, R + 1)]
+        want = oracle.run(layout, offs, w, x, kf)
+        gi, wi = layout.interior(got), layout.interior(want)
+        err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+        assert err < 1e-13, (v["name"], err)
+
+
+# ---------------------------------------------------------------- 3-D family
+
+def ktable3d(R, shape):
+    lib = _dbg()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.cs_debug_ktable3d.restype = ctypes.c_int32
+    lib.cs_debug_ktable3d.argtypes = [ctypes.c_int32, ctypes.c_int32, ip, ip, ip, ip, ctypes.c_int32]
+    cap = 256
+    dz = (ctypes.c_int32 * cap)()
+    dy, c, live = (ctypes.c_int32 * (4 * cap))(), (ctypes.c_int32 * (4 * cap))(), (ctypes.c_int32 * (4 * cap))()
+    n = lib.cs_debug_ktable3d(R, shape, dz, dy, c, live, cap)
+    assert n > 0
+    return (np.array(dz[:n]), np.array(dy[:4 * n]).reshape(n, 4), np.array(c[:4 * n]).reshape(n, 4),
+            np.array(live[:4 * n]).reshape(n, 4))
+
+
+def bfrag3d(R, shape, w):
+    lib = _dbg()
+    lib.cs_debug_bfrag_3d.restype = ctypes.c_int32
+    lib.cs_debug_bfrag_3d.argtypes = [ctypes.c_int32, ctypes.c_int32, _lib.c_double_p, _lib.c_double_p,
+                                      ctypes.c_int32]
+    cap = 64 * 256
+    out = np.zeros(cap)
+    wd = np.ascontiguousarray(w, dtype=np.float64)
+    nks = lib.cs_debug_bfrag_3d(R, shape, wd.ctypes.data_as(_lib.c_double_p), out.ctypes.data_as(_lib.c_double_p),
+                                cap)
+    assert nks > 0
+    return out[:nks * 64].reshape(nks, 64)
+
+
+def emulate_3d(v, layout, dense, x):
+    R, BY, BX, ZC, nw = v["R"], v["by"], v["bx"], v["st"], v["nw"]
+    dz_t, dy_t, c_t, live_t = ktable3d(R, v["shape"])
+    bf = bfrag3d(R, v["shape"], dense)
+    EY, EX = BY + 2 * R, BX + 2 * R
+    PX = v["px"]
+    assert PX >= EX
+    PLANE = EY * PX
+    NP = 2 * R + 1
+    assert v["lds"] == NP * PLANE * 8 and v["lds"] <= 160 * 1024
+    ext, org, n = layout.ext, layout.org, layout.n
+    x3 = x.reshape(ext)
+    out = np.zeros(layout.cells)
+    o3 = out.reshape(ext)
+    tiles = (-(-n[0] // ZC), -(-n[1] // BY), -(-n[2] // BX))
+    lanes = np.arange(64)
+    k, b4, i4 = lanes >> 4, (lanes >> 2) & 3, lanes & 3
+    for zc in range(tiles[0]):
+        for ty in range(tiles[1]):
+            for tx in range(tiles[2]):
+                Y0, X0 = ty * BY - R, tx * BX - R
+                ay0, ax0 = org[1] + Y0, org[2] + X0
+                Z0 = zc * ZC
+                Z1 = min(Z0 + ZC, n[0])
+                ring = np.full((NP, PLANE), np.nan)
+
+                def load(p):
+                    pl = np.zeros(PLANE)
+                    az = org[0] + p
+                    assert 0 <= az < ext[0]
+                    for idx in range(PLANE):
+                        r, c = divmod(idx, PX)
+                        ay, ax = ay0 + r, ax0 + c
+                        if c < EX and ay < ext[1] and ax < ext[2]:
+                            assert ay >= 0 and ax >= 0
+                            pl[idx] = x3[az, ay, ax]
+                    return pl
+                slot = lambda p: (p - (Z0 - R)) % NP
+                for p in range(Z0 - R, Z0 + R + 1):
+                    ring[slot(p)] = load(p)
+                for z in range(Z0, Z1):
+                    for blk in range((BY // 4) * (BX // 16)):
+                        y, xx = R + 4 * (blk // (BX // 16)), R + 16 * (blk % (BX // 16))
+                        acc = np.zeros((4, 4, 4))
+                        for ks in range(len(dz_t)):
+                            pl = ring[slot(z + dz_t[ks])]
+                            idx = (y + i4 + dy_t[ks][k]) * PX + xx + 4 * b4 + c_t[ks][k]
+                            assert idx.min() >= 0 and idx.max() < PLANE
+                            a = pl[idx]
+                            assert np.isfinite(a).all()
+                            A = np.zeros((4, 4, 4))
+                            B = np.zeros((4, 4, 4))
+                            A[b4, i4, k] = a
+                            B[b4, k, lanes & 3] = bf[ks]
+                            acc += A @ B
+                        vals = acc[b4, lanes >> 4, lanes & 3]
+                        for ln in range(64):
+                            tyy, txx = y + (ln >> 4), xx + (ln & 15)
+                            if Y0 + tyy < n[1] and X0 + txx < n[2]:
+                                gi = (org[0] + z, ay0 + tyy, ax0 + txx)
+                                assert all(org[d] <= gi[d] < org[d] + n[d] for d in range(3))
+                                o3[gi] = vals[ln]
+                    if z + 1 < Z1:
+                        ring[slot(z + R + 1)] = load(z + R + 1)
+    return out
+
+
+VARIANTS_3D = variants(3)
+
+
+@pytest.mark.parametrize("v", VARIANTS_3D, ids=[v["name"] for v in VARIANTS_3D])
+def test_variant_emulation_3d(v):
+    R, sh = v["R"], v["shape"]
+    D = 2 * R + 1
+    rng = np.random.default_rng(v["id"])
+    w = rng.uniform(0.05, 1.0, (D, D, D))
+    zz, yy, xx = np.meshgrid(*[np.arange(-R, R + 1)] * 3, indexing="ij")
+    if sh == 1:
+        w[((zz != 0).astype(int) + (yy != 0) + (xx != 0)) > 1] = 0.0
+    elif sh == 2:
+        w[np.abs(zz) + np.abs(yy) + np.abs(xx) > R] = 0.0
+    w /= w.sum()
+    dense = w.reshape(-1)
+    layout = Layout(3, [v["st"] + 2 * R + 4, v["by"] + 2 * R + 5, v["bx"] // 2 + 2 * R + 9], [R + 1, R, R + 2],
+                    [v["st"] + 3, v["by"] + 5, v["bx"] // 2 + 7])
+    x = oracle.fill_data(layout, v["id"])
+    got = emulate_3d(v, layout, dense, x)
+    offs, ww = oracle.dense_to_taps(3, R, dense)
+    want = oracle.run(layout, offs, ww, x, 1)
+    gi, wi = layout.interior(got), layout.interior(want)
+    err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+    assert err < 1e-13, (v["name"], err)
