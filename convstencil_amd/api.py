@@ -13,7 +13,8 @@ from . import _lib
 from ._lib import check
 from .shapes import Layout
 
-PRECISIONS = {"fp64": _lib.CS_PREC_FP64, "bf16x3": _lib.CS_PREC_BF16X3, "fp32x2": _lib.CS_PREC_FP32X2}
+PRECISIONS = {"fp64": _lib.CS_PREC_FP64, "bf16x3": _lib.CS_PREC_BF16X3, "fp32x2": _lib.CS_PREC_FP32X2,
+              "fp16x2": _lib.CS_PREC_FP16X2}
 
 
 def _dptr(a):

@@ -10,6 +10,7 @@ extern "C" {
 
 static const cs::Variant *family(int32_t ndim, int *n) {
   *n = 0;
+  if (ndim == 4) return cs::variants_2d_split(n);
   if (ndim == 1) return cs::variants_1d_f64(n);
   if (ndim == 2) return cs::variants_2d_f64(n);
   if (ndim == 3) return cs::variants_3d_f64(n);
@@ -89,6 +90,10 @@ void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49) {
     cs::fused_2d_exact(p9, out49);
   else
     cs::fused_2d_reference(p9, out49);
+}
+
+int64_t cs_debug_bfrag_split(int32_t precision, int32_t radius, const double *w, void *out, int64_t cap) {
+  return (int64_t)cs::build_bfrag_split(precision, radius, w, out, (size_t)cap);
 }
 
 }  // extern "C"

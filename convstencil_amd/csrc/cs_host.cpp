@@ -159,17 +159,18 @@ int default_kf(int ndim, int R, int top) {
   return std::min(top, R == 1 ? 4 : 2);
 }
 
-const Variant *family(int ndim, int *cnt) {
+const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   *cnt = 0;
+  if (prec != CS_PREC_FP64) return ndim == 2 ? cs::variants_2d_split(cnt) : nullptr;
   if (ndim == 1) return cs::variants_1d_f64(cnt);
   if (ndim == 2) return cs::variants_2d_f64(cnt);
   if (ndim == 3) return cs::variants_3d_f64(cnt);
   return nullptr;
 }
 
-const Variant *find_variant(int ndim, int mf, int R, int shape, int kf, int prefer_id) {
+const Variant *find_variant(int ndim, int mf, int R, int shape, int kf, int prefer_id, int prec = CS_PREC_FP64) {
   int cnt = 0;
-  const Variant *vs = family(ndim, &cnt);
+  const Variant *vs = family(ndim, &cnt, prec);
   const Variant *best = nullptr;
   for (int i = 0; i < cnt; ++i) {
     const Variant &v = vs[i];
@@ -179,9 +180,9 @@ const Variant *find_variant(int ndim, int mf, int R, int shape, int kf, int pref
   return best;
 }
 
-int max_kf(int ndim, int mf, int R, int shape) {
+int max_kf(int ndim, int mf, int R, int shape, int prec = CS_PREC_FP64) {
   int cnt = 0, m = 0;
-  const Variant *vs = family(ndim, &cnt);
+  const Variant *vs = family(ndim, &cnt, prec);
   for (int i = 0; i < cnt; ++i)
     if (vs[i].mf == mf && vs[i].R == R && vs[i].shape == shape) m = std::max(m, vs[i].kf);
   return m;
@@ -189,26 +190,41 @@ int max_kf(int ndim, int mf, int R, int shape) {
 
 cs_status select_variants(cs_plan *p) {
   const int nd = p->ndim;
+  const int prec = p->opt.precision;
   if (nd < 1 || nd > 3) return CS_ERR_UNSUPPORTED;
-  const int mf = 4;
+  if (prec != CS_PREC_FP64 && (prec < CS_PREC_BF16X3 || prec > CS_PREC_FP16X2)) return CS_ERR_INVALID;
+  if (prec != CS_PREC_FP64 && nd != 2) return CS_ERR_UNSUPPORTED;
+  // split-precision variants use the full weight square (Variant::mf holds the precision)
+  if (prec != CS_PREC_FP64) p->shape = cs::SHAPE_BOX;
+  const int mf = prec != CS_PREC_FP64 ? prec : 4;
   const Variant *v = nullptr;
   if (p->opt.tile_variant > 0) {
-    v = find_variant(nd, 0, p->R, p->shape, 0, p->opt.tile_variant);
+    v = find_variant(nd, 0, p->R, p->shape, 0, p->opt.tile_variant, prec);
     if (!v) return CS_ERR_INVALID;
   } else {
-    const int top = max_kf(nd, mf, p->R, p->shape);
+    const int top = max_kf(nd, mf, p->R, p->shape, prec);
     int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
-    for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, p->shape, kf, 0);
+    for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, p->shape, kf, 0, prec);
   }
   if (!v) return CS_ERR_UNSUPPORTED;
   p->vmain = v;
-  p->vone = find_variant(nd, v->mf, p->R, p->shape, 1, 0);
+  p->vone = find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec);
   if (!p->vone) return CS_ERR_UNSUPPORTED;
   p->vrem.assign(v->kf, nullptr);
-  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0);
+  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0, prec);
   for (const Variant *u : {p->vmain, p->vone}) CS_HIP(u->prepare());
   for (const Variant *u : p->vrem)
     if (u) CS_HIP(u->prepare());
+  if (prec != CS_PREC_FP64) {
+    std::vector<unsigned char> hb(1 << 20);
+    const size_t bytes = cs::build_bfrag_split(prec, p->R, p->w.data(), hb.data(), hb.size());
+    if (bytes == 0) return CS_ERR_UNSUPPORTED;
+    if (p->bfrag) (void)hipFree(p->bfrag);
+    p->bfrag = nullptr;
+    CS_HIP(hipMalloc(&p->bfrag, bytes));
+    CS_HIP(hipMemcpy(p->bfrag, hb.data(), bytes, hipMemcpyHostToDevice));
+    return CS_OK;
+  }
   // B fragments for the family's MFMA shape
   const int nks = nd == 1   ? cs::bfrag_ksteps_1d_f64(p->R)
                   : nd == 2 ? cs::bfrag_ksteps_2d_f64(v->mf, p->R, p->shape)

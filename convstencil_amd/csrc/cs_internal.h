@@ -36,6 +36,7 @@ struct Variant {
 const Variant *variants_1d_f64(int *count);
 const Variant *variants_2d_f64(int *count);
 const Variant *variants_3d_f64(int *count);
+const Variant *variants_2d_split(int *count);
 
 // Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].
 void build_bfrag_2d_f64(int mf, int R, int shape, const double *w_dense, double *bfrag);
@@ -43,6 +44,8 @@ int bfrag_ksteps_2d_f64(int mf, int R, int shape);
 void build_bfrag_1d_f64(int R, const double *w_dense, double *bfrag);
 int bfrag_ksteps_1d_f64(int R);
 void build_bfrag_3d_f64(int R, int shape, const double *w_dense, double *bfrag);
+// Split-precision B fragments (prec: 1 BF16x3, 2 FP32x2, 3 FP16x2); returns bytes written, 0 if cap is short.
+size_t build_bfrag_split(int prec, int R, const double *w_dense, void *out, size_t cap);
 int bfrag_ksteps_3d_f64(int R, int shape);
 
 // Device-side synthetic input fill (same bits as the host generator).

@@ -45,9 +45,12 @@ typedef enum cs_status {
 } cs_status;
 
 typedef enum cs_precision {
-  CS_PREC_FP64 = 0,   /* plain FP64: v_mfma_f64_16x16x4 stencil2row */
-  CS_PREC_BF16X3 = 1, /* state/weights split into 3 BF16 slices, BF16 MFMA, FP32 accumulate, FP64 recombine */
-  CS_PREC_FP32X2 = 2  /* state/weights split into FP32 hi/lo, FP32 MFMA, FP64 recombine */
+  CS_PREC_FP64 = 0,   /* plain FP64: stencil2row on FP64 MFMA (v_mfma_f64_4x4x4 / 16x16x4) */
+  CS_PREC_BF16X3 = 1, /* 2-D: state/weights split into 3 BF16 slices, BF16 MFMA, FP32 accumulate per level
+                         (6 of 9 slice products), FP64 recombine; state FP64 in HBM */
+  CS_PREC_FP32X2 = 2, /* 2-D: FP32 hi/lo slices, FP32 MFMA (3 products), FP64 recombine */
+  CS_PREC_FP16X2 = 3  /* 2-D: 2 FP16 slices rescaled by 2^11 (3 products), FP32 accumulate, FP64
+                         recombine; values must stay below 2^15 in magnitude */
 } cs_precision;
 
 /* Padded array layout.  Dimensions are listed slowest first; a 2-D grid has

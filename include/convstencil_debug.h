@@ -15,7 +15,8 @@
 extern "C" {
 #endif
 
-/* Number of compiled FP64 kernel variants for grids of ndim dimensions. */
+/* Number of compiled kernel variants of a family: 1, 2, 3 = FP64 kernels for 1-, 2-, 3-D grids;
+ * 4 = 2-D split-precision kernels (there info[1] holds the cs_precision). */
 int32_t cs_debug_variant_count(int32_t ndim);
 /* info[0..10] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
  * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
@@ -43,6 +44,10 @@ int32_t cs_debug_bfrag_3d(int32_t radius, int32_t shape, const double *w, double
  * exactly (exact != 0) or as the reference's hand-expanded table computes them
  * (src/2d/main.cu:233-281). */
 void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49);
+
+/* Split-precision B fragments as raw slice elements (bf16 / fp32 / fp16 by precision), layout
+ * [weight slice][instruction][lane][element]; returns the byte count, or 0 if cap is too small. */
+int64_t cs_debug_bfrag_split(int32_t precision, int32_t radius, const double *w, void *out, int64_t cap);
 
 /* Runs the FP64 MFMA lane-layout probe on the current device (needs a GPU).  which = 0:
  * v_mfma_f64_16x16x4, host_out[64 * 64 * 4] = D[lane][item] for A one-hot at lane p (p-major);

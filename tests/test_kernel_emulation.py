@@ -463,3 +463,160 @@ def test_variant_emulation_3d(v):
     gi, wi = layout.interior(got), layout.interior(want)
     err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
     assert err < 1e-13, (v["name"], err)
+
+
+# ---------------------------------------------------------------- 2-D split precision
+
+SPLIT = {1: ("bf16", 8, 32, 3, 3, 3, 1.0), 2: ("f32", 1, 4, 2, 2, 2, 1.0), 3: ("f16", 8, 32, 2, 2, 2, 2048.0)}
+
+
+def _to_f32(raw, kind):
+    if kind == "bf16":
+        return (raw.view(np.uint16).astype(np.uint32) << 16).view(np.float32)
+    if kind == "f16":
+        return raw.view(np.float16).astype(np.float32)
+    return raw.view(np.float32)
+
+
+def _round(x, kind):
+    x32 = np.asarray(x, dtype=np.float64).astype(np.float32)
+    if kind == "bf16":
+        b = x32.view(np.uint32).astype(np.uint64)
+        return (((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16).astype(np.uint32).view(np.float32)
+    if kind == "f16":
+        return x32.astype(np.float16).astype(np.float32)
+    return x32
+
+
+def split_np(v, kind, ns, scale):
+    rem = np.asarray(v, dtype=np.float64)
+    out, sc = [], 1.0
+    for _ in range(ns):
+        t = _round(rem * sc, kind)
+        out.append(t)
+        rem = rem - t.astype(np.float64) / sc
+        sc *= scale
+    return out
+
+
+def emulate_split(v, layout, dense, x):
+    prec, R, KF, BY, BX = v["mf"], v["R"], v["kf"], v["by"], v["bx"]
+    kind, CK, KI, NSU, NSW, NL, SCALE = SPLIT[prec]
+    esz = 4 if kind == "f32" else 2
+    lib = _dbg()
+    lib.cs_debug_bfrag_split.restype = ctypes.c_int64
+    lib.cs_debug_bfrag_split.argtypes = [ctypes.c_int32, ctypes.c_int32, _lib.c_double_p, ctypes.c_void_p,
+                                         ctypes.c_int64]
+    raw = np.zeros(1 << 20, dtype=np.uint8)
+    wd = np.ascontiguousarray(dense, dtype=np.float64)
+    nbytes = lib.cs_debug_bfrag_split(prec, R, wd.ctypes.data_as(_lib.c_double_p), raw.ctypes.data, raw.size)
+    assert nbytes > 0
+    el = _to_f32(raw[:nbytes].copy().view(np.uint32 if esz == 4 else np.uint16), kind)
+    RPD = 24 if CK == 8 else ((16 + 2 * R + 3) // 4) * 4
+    NENT = (2 * R + 1) * RPD
+    NINS = -(-NENT // KI)
+    bf = el.reshape(NSW, NINS, 64, CK)
+    EY, EX = BY + 2 * KF * R, BX + 2 * KF * R
+    TOP = (((R - KF * R) % 8) + 8) % 8 + 8 if CK == 8 else 8
+    PR = -(-(TOP + EY + RPD - 2 * R - 1) // 8) * 8
+    assert v["px"] == PR
+    PLANE = EX * PR
+    assert v["lds"] == 2 * NSU * PLANE * esz and v["lds"] <= 160 * 1024
+    lanes = np.arange(64)
+    ai, q = lanes & 15, lanes >> 4
+
+    def ent(e):
+        return (e // RPD - R, e % RPD) if e < NENT else (0, 0)
+    off = np.zeros((NINS, 64), dtype=np.int64)
+    for m in range(NINS):
+        for ln in range(64):
+            dx, rr = ent(m * KI + q[ln] * CK)
+            off[m, ln] = (ai[ln] + dx) * PR + rr
+    ext, org, n = layout.ext, layout.org, layout.n
+    pitch = ext[2]
+    out = np.zeros(layout.cells)
+    x3 = x.reshape(ext)
+    ty_n, tx_n = -(-n[1] // BY), -(-n[2] // BX)
+    for tile in range(ty_n * tx_n):
+        Y0, X0 = (tile // tx_n) * BY - KF * R, (tile % tx_n) * BX - KF * R
+        ay0, ax0 = org[1] + Y0, org[2] + X0
+        full = Y0 >= 0 and X0 >= 0 and Y0 + EY <= n[1] and X0 + EX <= n[2]
+        vals = np.zeros(PLANE)
+        for idx in range(PLANE):
+            c, sr = divmod(idx, PR)
+            r = sr - TOP
+            if 0 <= r < EY:
+                ay, ax = ay0 + r, ax0 + c
+                if 0 <= ay < ext[1] and 0 <= ax < pitch:
+                    vals[idx] = x3[0, ay, ax]
+        sl = split_np(vals, kind, NSU, SCALE)
+        bufs = [np.stack(sl), np.stack(sl)]
+        for st in range(KF):
+            src, dst = bufs[st & 1], bufs[(st + 1) & 1]
+            last = st == KF - 1
+            lo, hi = R * (st + 1), EY - R * (st + 1)
+            W = EX - 2 * R * (st + 1)
+            y0 = lo - (((lo - KF * R) % 8) + 8) % 8
+            nby, nbx = -(-(hi - y0) // 16), -(-W // 16)
+            writes = []
+            for blk in range(nby * nbx):
+                ba, bb = divmod(blk, nbx)
+                y, xx = y0 + 16 * ba, lo + min(16 * bb, W - 16)
+                ubase = xx * PR + (y - R + TOP)
+                acc = [np.zeros((16, 16), dtype=np.float32) for _ in range(NL)]
+                for m in range(NINS):
+                    base = ubase + off[m]
+                    if CK == 8:
+                        assert np.all(base % 8 == 0), "16-byte LDS read misaligned"
+                    assert base.min() >= 0 and base.max() + CK <= PLANE
+                    for s in range(NSU):
+                        a = src[s][base[:, None] + np.arange(CK)[None, :]]  # [lane][e]
+                        A = np.zeros((16, KI), dtype=np.float32)
+                        A[ai[:, None], q[:, None] * CK + np.arange(CK)[None, :]] = a
+                        for t in range(NSW):
+                            if s + t >= NL:
+                                continue
+                            B = np.zeros((KI, 16), dtype=np.float32)
+                            B[q[:, None] * CK + np.arange(CK)[None, :], ai[:, None]] = bf[t, m]
+                            acc[s + t] = (acc[s + t] + (A.astype(np.float64) @ B.astype(np.float64)).astype(
+                                np.float32)).astype(np.float32)
+                for i in range(16):        # output column x + i
+                    for nn in range(16):   # output row y + nn
+                        tyy, txx = y + nn, xx + i
+                        val, sc = 0.0, 1.0
+                        for L in range(NL):
+                            val += float(acc[L][i, nn]) / sc
+                            sc *= SCALE
+                        if not (lo <= tyy < hi):
+                            continue
+                        if not (full or (0 <= Y0 + tyy < n[1] and 0 <= X0 + txx < n[2])):
+                            continue
+                        if last:
+                            gi = (ay0 + tyy) * pitch + ax0 + txx
+                            assert org[1] <= ay0 + tyy < org[1] + n[1] and org[2] <= ax0 + txx < org[2] + n[2]
+                            out[gi] = val
+                        else:
+                            writes.append((txx * PR + tyy + TOP, val))
+            for li, val in writes:
+                assert 0 <= li < PLANE
+                for k, s_ in enumerate(split_np(np.array([val]), kind, NSU, SCALE)):
+                    dst[k][li] = s_[0]
+    return out
+
+
+VARIANTS_SPLIT = variants(4)
+
+
+@pytest.mark.parametrize("v", [v for v in VARIANTS_SPLIT if v["kf"] <= 2],
+                         ids=[v["name"] for v in VARIANTS_SPLIT if v["kf"] <= 2])
+def test_variant_emulation_split(v):
+    R = v["R"]
+    dense = _dense(R, 0, v["id"])
+    layout = Layout(2, [v["by"] // 2 + 2 * R + 3, v["bx"] + 2 * R + 9], [R + 1, R + 2], [v["by"] // 2, v["bx"] + 7])
+    x = oracle.fill_data(layout, v["id"])
+    got = emulate_split(v, layout, dense, x)
+    offs, w = oracle.dense_to_taps(2, R, dense)
+    want = oracle.run(layout, offs, w, x, v["kf"])
+    gi, wi = layout.interior(got), layout.interior(want)
+    err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+    assert err < 2e-6, (v["name"], err)
