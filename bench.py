@@ -81,7 +81,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "bf16x3", "fp16x2", "fp32x2"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
+                    help="only run N plain device copies of 2^27 doubles (reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import ctypes
+        from convstencil_amd import _lib
+        lib = _lib.load()
+        lib.cs_debug_copy_bench.restype = ctypes.c_int32
+        lib.cs_debug_copy_bench.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+        ms = ctypes.c_double(0)
+        rc = lib.cs_debug_copy_bench(1 << 27, args.copy_calibration, ctypes.byref(ms))
+        print(json.dumps({"copy_doubles": 1 << 27, "copy_bytes_read": 8 << 27, "copy_bytes_written": 8 << 27,
+                          "ms_per_copy": ms.value, "rc": rc}), flush=True)
+        sys.exit(rc)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -99,7 +113,8 @@ def main():
     w = cs.fill_weights_host(len(offs), args.seed)
     dense = dense_weights(ndim, r, offs, w)
 
-    plan = Plan(L, r, dense, fusion_depth=args.fusion, variant=args.variant, use_graph=not args.no_graph)
+    plan = Plan(L, r, dense, fusion_depth=args.fusion, variant=args.variant, use_graph=not args.no_graph,
+                precision=args.precision)
     plan.fill_random(args.seed)
     desc = plan.describe()
     # warmup + timed region: HIP events on the plan stream around exactly K runs of T steps

@@ -1,0 +1,381 @@
+// cs_batch.cpp -- host side of the persistent batched launch (cs_batch_* in convstencil.h).
+//
+// Turns `steps` time steps of every grid into step blocks of work items (see
+// k2d_batch.hip), uploads the item table once per step count, and launches the
+// reset kernel plus the persistent kernel, optionally captured in a hipGraph.
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cs_batch.h"
+#include "cs_host_util.h"
+#include "cs_rng.h"
+
+using namespace cs::host;
+
+namespace {
+
+struct Seq {
+  cs::BatchItem *items = nullptr;
+  int *block_start = nullptr;
+  unsigned *sync = nullptr;
+  int nblocks = 0, nsync = 0;
+  int64_t nitems = 0;
+  std::vector<int> parity_flip;  // per grid: number of launches' buffer swaps (mod 2)
+  hipGraphExec_t graph = nullptr;
+  std::vector<char> uniform;     // per-grid fusion eligibility the sequence was built for
+};
+
+}  // namespace
+
+struct cs_batch {
+  int count = 0;
+  int prec = 0;  // 0 FP64, 1 BF16x3
+  cs_options opt{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<cs_layout> layouts;
+  std::vector<int> R, shape, kind_main, kind_one, kf;
+  std::vector<double *> buf0, buf1, bfrag;
+  std::vector<int> cur;        // buffer holding each grid's state
+  std::vector<char> uniform;   // both halos equal -> fused blocks are exact
+  cs::BatchGrid *d_grids = nullptr;
+  std::vector<cs::BatchGrid> h_grids;  // host copy; must outlive the asynchronous upload
+  int max_wg = 0;
+  std::map<int64_t, Seq> seqs;  // keyed by steps (rebuilt when the per-grid state changes)
+  int64_t launches = 0;
+};
+
+namespace {
+
+void free_seq(Seq &s) {
+  if (s.graph) (void)hipGraphExecDestroy(s.graph);
+  if (s.items) (void)hipFree(s.items);
+  if (s.block_start) (void)hipFree(s.block_start);
+  if (s.sync) (void)hipFree(s.sync);
+  s = Seq{};
+}
+
+// Maps each grid's buffer roles (0 = holds the state now) to its buffers; copied in
+// stream order, so a previous launch still reading the table is not disturbed.
+cs_status upload_grid_table(cs_batch *b) {
+  std::vector<cs::BatchGrid> &g = b->h_grids;
+  g.assign((size_t)b->count, cs::BatchGrid{});
+  for (int i = 0; i < b->count; ++i) {
+    // BatchItem::src indexes buf[]: buf[0] is the buffer that holds the state now
+    g[(size_t)i].buf[0] = b->cur[(size_t)i] == 0 ? b->buf0[(size_t)i] : b->buf1[(size_t)i];
+    g[(size_t)i].buf[1] = b->cur[(size_t)i] == 0 ? b->buf1[(size_t)i] : b->buf0[(size_t)i];
+    g[(size_t)i].bfrag = b->bfrag[(size_t)i];
+    cs::Geo &geo = g[(size_t)i].geo;
+    const cs_layout &L = b->layouts[(size_t)i];
+    for (int d = 0; d < 3; ++d) {
+      geo.ext[d] = L.ext[d];
+      geo.org[d] = L.org[d];
+      geo.n[d] = L.n[d];
+    }
+    geo.tiles[0] = 1;
+    geo.tiles[1] = (L.n[1] + 63) / 64;
+    geo.tiles[2] = (L.n[2] + 63) / 64;
+  }
+  CS_HIP(hipMemcpyAsync(b->d_grids, g.data(), g.size() * sizeof(cs::BatchGrid), hipMemcpyHostToDevice, b->stream));
+  return CS_OK;
+}
+
+// Builds the step blocks for `steps` steps from the current per-grid state.
+cs_status build_seq(cs_batch *b, int64_t steps, Seq *s) {
+  std::vector<std::vector<cs::BatchItem>> blocks;
+  s->parity_flip.assign((size_t)b->count, 0);
+  for (int g = 0; g < b->count; ++g) {
+    const int kf = b->uniform[(size_t)g] ? b->kf[(size_t)g] : 1;
+    const int64_t ntiles = ((b->layouts[(size_t)g].n[1] + 63) / 64) * ((b->layouts[(size_t)g].n[2] + 63) / 64);
+    int64_t done = 0;
+    int blk = 0;
+    while (done < steps) {
+      const bool full = steps - done >= kf;
+      const int kind = (full && kf > 1) ? b->kind_main[(size_t)g] : b->kind_one[(size_t)g];
+      if ((int)blocks.size() <= blk) blocks.resize((size_t)blk + 1);
+      for (int64_t t = 0; t < ntiles; ++t) blocks[(size_t)blk].push_back(cs::BatchItem{g, (int)t, kind, blk & 1});
+      done += (full && kf > 1) ? kf : 1;
+      ++blk;
+    }
+    s->parity_flip[(size_t)g] = blk & 1;
+  }
+  std::vector<cs::BatchItem> flat;
+  std::vector<int> start;
+  for (auto &v : blocks) {
+    start.push_back((int)flat.size());
+    flat.insert(flat.end(), v.begin(), v.end());
+  }
+  start.push_back((int)flat.size());
+  if (flat.size() > (size_t)0x7fffffff) return CS_ERR_UNSUPPORTED;
+  s->nblocks = (int)blocks.size();
+  s->nitems = (int64_t)flat.size();
+  s->nsync = s->nblocks + 2;
+  if (!flat.empty()) {
+    CS_HIP(hipMalloc(&s->items, flat.size() * sizeof(cs::BatchItem)));
+    CS_HIP(hipMemcpy(s->items, flat.data(), flat.size() * sizeof(cs::BatchItem), hipMemcpyHostToDevice));
+  }
+  CS_HIP(hipMalloc(&s->block_start, start.size() * sizeof(int)));
+  CS_HIP(hipMemcpy(s->block_start, start.data(), start.size() * sizeof(int), hipMemcpyHostToDevice));
+  CS_HIP(hipMalloc(&s->sync, (size_t)s->nsync * sizeof(unsigned)));
+  s->uniform = b->uniform;
+  return CS_OK;
+}
+
+cs_status issue(cs_batch *b, Seq &s) {
+  const unsigned nwg = (unsigned)b->max_wg;
+  CS_HIP(cs::batch_launch(b->prec, b->d_grids, s.items, s.block_start, s.nblocks, s.sync, s.nsync, nwg, b->stream));
+  return CS_OK;
+}
+
+cs_status run(cs_batch *b, int64_t steps) {
+  if (steps < 0) return CS_ERR_INVALID;
+  if (steps == 0) return CS_OK;
+  CS_HIP(hipSetDevice(b->device));
+  auto it = b->seqs.find(steps);
+  if (it != b->seqs.end() && it->second.uniform != b->uniform) {
+    free_seq(it->second);
+    b->seqs.erase(it);
+    it = b->seqs.end();
+  }
+  if (it == b->seqs.end()) {
+    Seq s;
+    cs_status st = build_seq(b, steps, &s);
+    if (st != CS_OK) {
+      free_seq(s);
+      return st;
+    }
+    it = b->seqs.emplace(steps, s).first;
+  }
+  Seq &s = it->second;
+  CS_TRY(upload_grid_table(b));
+  if (b->opt.use_graph) {
+    if (!s.graph) {
+      hipGraph_t graph = nullptr;
+      CS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      cs_status st = issue(b, s);
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (st != CS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      CS_HIP(e);
+      hipError_t ei = hipGraphInstantiate(&s.graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      CS_HIP(ei);
+    }
+    CS_HIP(hipGraphLaunch(s.graph, b->stream));
+  } else {
+    CS_TRY(issue(b, s));
+  }
+  // the host table must stay unchanged until the copy above has run
+  CS_HIP(hipStreamSynchronize(b->stream));
+  b->launches += 1;
+  for (int g = 0; g < b->count; ++g) b->cur[(size_t)g] ^= s.parity_flip[(size_t)g];
+  return CS_OK;
+}
+
+cs_status check_error_flag(cs_batch *b) {
+  for (auto &kv : b->seqs) {
+    unsigned flag = 0;
+    CS_HIP(hipMemcpy(&flag, kv.second.sync + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
+    if (flag) {
+      fprintf(stderr, "convstencil: batch barrier did not complete within its spin limit\n");
+      return CS_ERR_DEVICE;
+    }
+  }
+  return CS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layouts, const cs_stencil *stencils,
+                          const cs_options *opt) {
+  if (!out || count <= 0 || !layouts || !stencils) return CS_ERR_INVALID;
+  *out = nullptr;
+  const int prec = opt ? opt->precision : CS_PREC_FP64;
+  if (prec != CS_PREC_FP64 && prec != CS_PREC_BF16X3) return CS_ERR_UNSUPPORTED;
+  for (int i = 0; i < count; ++i) {
+    const cs_layout &L = layouts[i];
+    const cs_stencil &st = stencils[i];
+    if (L.ndim != 2 || st.ndim != 2 || !st.weights || (st.radius != 1 && st.radius != 3)) return CS_ERR_UNSUPPORTED;
+    if (!layout_valid(&L, st.radius)) return CS_ERR_INVALID;
+  }
+  int dev = 0;
+  CS_TRY(check_device(&dev));
+  cs_batch *b = new cs_batch();
+  b->count = count;
+  b->prec = prec == CS_PREC_FP64 ? 0 : 1;
+  if (opt) b->opt = *opt;
+  b->device = dev;
+  b->layouts.assign(layouts, layouts + count);
+  cs_status s = CS_OK;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) s = CS_ERR_DEVICE;
+  for (int i = 0; i < count && s == CS_OK; ++i) {
+    const cs_stencil &st = stencils[i];
+    const int R = st.radius;
+    const int D = 2 * R + 1;
+    std::vector<double> w(st.weights, st.weights + D * D);
+    const int shape = b->prec == 0 ? classify_2d(R, w.data()) : cs::SHAPE_BOX;
+    const int kf = cs::batch_main_kf(b->prec, R);
+    const int km = cs::batch_kind(b->prec, R, shape, kf), k1 = cs::batch_kind(b->prec, R, shape, 1);
+    if (km < 0 || k1 < 0) {
+      s = CS_ERR_UNSUPPORTED;
+      break;
+    }
+    b->R.push_back(R);
+    b->shape.push_back(shape);
+    b->kf.push_back(kf);
+    b->kind_main.push_back(km);
+    b->kind_one.push_back(k1);
+    // B fragments
+    double *d = nullptr;
+    if (b->prec == 0) {
+      const int nks = cs::bfrag_ksteps_2d_f64(4, R, shape);
+      std::vector<double> h((size_t)nks * 64);
+      cs::build_bfrag_2d_f64(4, R, shape, w.data(), h.data());
+      if (hipMalloc(&d, h.size() * sizeof(double)) != hipSuccess ||
+          hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        s = CS_ERR_ALLOC;
+    } else {
+      std::vector<unsigned char> h(1 << 20);
+      const size_t bytes = cs::build_bfrag_split(CS_PREC_BF16X3, R, w.data(), h.data(), h.size());
+      if (bytes == 0 || hipMalloc(&d, bytes) != hipSuccess ||
+          hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        s = CS_ERR_ALLOC;
+    }
+    b->bfrag.push_back(d);
+    double *p0 = nullptr, *p1 = nullptr;
+    const size_t bytes = (size_t)cell_count(&layouts[i]) * sizeof(double);
+    if (s == CS_OK && (hipMalloc(&p0, bytes) != hipSuccess || hipMalloc(&p1, bytes) != hipSuccess)) {
+      (void)hipGetLastError();
+      s = CS_ERR_ALLOC;
+    }
+    b->buf0.push_back(p0);
+    b->buf1.push_back(p1);
+    b->cur.push_back(0);
+    b->uniform.push_back(1);
+  }
+  if (s == CS_OK && hipMalloc(&b->d_grids, (size_t)count * sizeof(cs::BatchGrid)) != hipSuccess) s = CS_ERR_ALLOC;
+  if (s == CS_OK && cs::batch_prepare(b->prec, &b->max_wg) != hipSuccess) s = CS_ERR_DEVICE;
+  if (s == CS_OK && b->max_wg <= 0) s = CS_ERR_UNSUPPORTED;
+  if (s != CS_OK) {
+    cs_batch_destroy(b);
+    return s;
+  }
+  *out = b;
+  return CS_OK;
+}
+
+cs_status cs_batch_destroy(cs_batch *b) {
+  if (!b) return CS_ERR_INVALID;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto &kv : b->seqs) free_seq(kv.second);
+  for (double *p : b->buf0)
+    if (p) (void)hipFree(p);
+  for (double *p : b->buf1)
+    if (p) (void)hipFree(p);
+  for (double *p : b->bfrag)
+    if (p) (void)hipFree(p);
+  if (b->d_grids) (void)hipFree(b->d_grids);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return CS_OK;
+}
+
+cs_status cs_batch_upload(cs_batch *b, int32_t grid, const double *host_in) {
+  if (!b || !host_in || grid < 0 || grid >= b->count) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(b->device));
+  const cs_layout &L = b->layouts[(size_t)grid];
+  const size_t n = (size_t)cell_count(&L);
+  CS_HIP(hipMemcpy(b->buf0[(size_t)grid], host_in, n * sizeof(double), hipMemcpyHostToDevice));
+  CS_HIP(cs::launch_zero(b->buf1[(size_t)grid], n, b->stream));
+  CS_HIP(hipStreamSynchronize(b->stream));
+  b->cur[(size_t)grid] = 0;
+  b->uniform[(size_t)grid] = halo_is_zero(&L, host_in) ? 1 : 0;
+  return CS_OK;
+}
+
+cs_status cs_batch_fill_random(cs_batch *b, const uint64_t *seeds) {
+  if (!b || !seeds) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(b->device));
+  for (int i = 0; i < b->count; ++i) {
+    const cs_layout &L = b->layouts[(size_t)i];
+    cs::Geo g{};
+    for (int d = 0; d < 3; ++d) {
+      g.ext[d] = L.ext[d];
+      g.org[d] = L.org[d];
+      g.n[d] = L.n[d];
+    }
+    CS_HIP(cs::launch_fill_random(b->buf0[(size_t)i], g, cs::stream_seed(seeds[i], 0), b->stream));
+    CS_HIP(cs::launch_zero(b->buf1[(size_t)i], (size_t)cell_count(&L), b->stream));
+    b->cur[(size_t)i] = 0;
+    b->uniform[(size_t)i] = 1;
+  }
+  CS_HIP(hipStreamSynchronize(b->stream));
+  return CS_OK;
+}
+
+cs_status cs_batch_run(cs_batch *b, int64_t steps, cs_report *rep) {
+  if (!b) return CS_ERR_INVALID;
+  const int64_t l0 = b->launches;
+  CS_TRY(run(b, steps));
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->launches = b->launches - l0;
+  }
+  return CS_OK;
+}
+
+cs_status cs_batch_time(cs_batch *b, int64_t steps, int32_t warmup, int32_t reps, cs_report *rep) {
+  if (!b || reps <= 0 || warmup < 0 || steps <= 0) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(b->device));
+  for (int i = 0; i < warmup; ++i) CS_TRY(run(b, steps));
+  CS_HIP(hipStreamSynchronize(b->stream));
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  const int64_t l0 = b->launches;
+  CS_HIP(hipEventRecord(e0, b->stream));
+  for (int i = 0; i < reps; ++i) CS_TRY(run(b, steps));
+  CS_HIP(hipEventRecord(e1, b->stream));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  CS_HIP(hipEventDestroy(e0));
+  CS_HIP(hipEventDestroy(e1));
+  CS_TRY(check_error_flag(b));
+  if (rep) {
+    memset(rep, 0, sizeof(*rep));
+    rep->elapsed_ms = ms / reps;
+    double pts = 0;
+    for (const cs_layout &L : b->layouts) pts += (double)L.n[1] * (double)L.n[2];
+    rep->gstencil_per_s = pts * (double)steps / (rep->elapsed_ms * 1e-3) / 1e9;
+    rep->launches = (b->launches - l0) / reps;
+  }
+  return CS_OK;
+}
+
+cs_status cs_batch_download(cs_batch *b, int32_t grid, double *host_out) {
+  if (!b || !host_out || grid < 0 || grid >= b->count) return CS_ERR_INVALID;
+  CS_HIP(hipSetDevice(b->device));
+  CS_HIP(hipStreamSynchronize(b->stream));
+  CS_TRY(check_error_flag(b));
+  const double *src = b->cur[(size_t)grid] == 0 ? b->buf0[(size_t)grid] : b->buf1[(size_t)grid];
+  CS_HIP(hipMemcpy(host_out, src, (size_t)cell_count(&b->layouts[(size_t)grid]) * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return CS_OK;
+}
+
+cs_status cs_batch_describe(cs_batch *b, char *buf, int32_t len) {
+  if (!b || !buf || len <= 0) return CS_ERR_INVALID;
+  int64_t tiles = 0;
+  for (const cs_layout &L : b->layouts) tiles += ((L.n[1] + 63) / 64) * ((L.n[2] + 63) / 64);
+  snprintf(buf, (size_t)len, "batch grids=%d precision=%s tiles=%lld workgroups=%d lds=%zu", b->count,
+           b->prec == 0 ? "fp64" : "bf16x3", (long long)tiles, b->max_wg, cs::batch_lds_bytes(b->prec));
+  return CS_OK;
+}
+
+}  // extern "C"

@@ -28,7 +28,7 @@ int32_t cs_debug_variant_info(int32_t ndim, int32_t i, int64_t *info, char *name
   const cs::Variant *v = family(ndim, &n);
   if (i < 0 || i >= n || !info) return -1;
   const cs::Variant &x = v[i];
-  const int64_t vals[11] = {x.id, x.mf, x.R, x.shape, x.kf, x.by, x.bx, x.nw, (int64_t)x.lds_bytes, x.px, x.st};
+  const int64_t vals[12] = {x.id, x.mf, x.R,  x.shape, x.kf, x.by, x.bx, x.nw, (int64_t)x.lds_bytes, x.px, x.st, x.cs};
   memcpy(info, vals, sizeof(vals));
   if (name && len > 0) {
     strncpy(name, x.name, (size_t)len - 1);
@@ -47,6 +47,29 @@ int32_t cs_debug_ktable(int32_t mf, int32_t radius, int32_t shape, int32_t *dy, 
     live[i] = t.live[i];
   }
   return t.n;
+}
+
+int32_t cs_debug_dtable(int32_t radius, int32_t shape, int32_t *hdr, int32_t *ty, int32_t *tx, int32_t *live,
+                        int32_t cap) {
+  const cs::DTable t = cs::make_dtable(radius, shape);
+  const int n = 4 * (t.np + t.ns);
+  if (n > cap || !hdr) return -1;
+  hdr[0] = t.np;
+  hdr[1] = t.ns;
+  hdr[2] = t.pp;
+  for (int ps = 0; ps < t.np; ++ps)
+    for (int q = 0; q < 4; ++q) {
+      ty[4 * ps + q] = t.pty[ps][q];
+      tx[4 * ps + q] = t.ptx[ps][q];
+      live[4 * ps + q] = t.plive[ps][q];
+    }
+  for (int ss = 0; ss < t.ns; ++ss)
+    for (int q = 0; q < 4; ++q) {
+      ty[4 * (t.np + ss) + q] = t.sty[ss][q];
+      tx[4 * (t.np + ss) + q] = t.stx[ss][q];
+      live[4 * (t.np + ss) + q] = t.slive[ss][q];
+    }
+  return n;
 }
 
 int32_t cs_debug_bfrag_2d(int32_t mf, int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap) {
@@ -94,6 +117,34 @@ void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49) {
 
 int64_t cs_debug_bfrag_split(int32_t precision, int32_t radius, const double *w, void *out, int64_t cap) {
   return (int64_t)cs::build_bfrag_split(precision, radius, w, out, (size_t)cap);
+}
+
+int32_t cs_debug_copy_bench(int64_t count, int32_t reps, double *ms_per_copy) {
+  // Plain device-to-device copy of `count` doubles (one element per thread), `reps`
+  // times: the reference point for HBM byte counters (FETCH_SIZE / WRITE_SIZE).
+  double *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, (size_t)count * sizeof(double)) != hipSuccess) return 3;
+  if (hipMalloc(&b, (size_t)count * sizeof(double)) != hipSuccess) {
+    (void)hipFree(a);
+    return 3;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipError_t e = cs::launch_zero(a, (size_t)count, nullptr);
+  if (e == hipSuccess) e = cs::launch_copy(a, b, (size_t)count, nullptr);
+  (void)hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = cs::launch_copy(a, b, (size_t)count, nullptr);
+  (void)hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (ms_per_copy) *ms_per_copy = reps > 0 ? ms / reps : 0.0;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return e == hipSuccess ? 0 : 4;
 }
 
 }  // extern "C"

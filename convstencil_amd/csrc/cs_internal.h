@@ -31,6 +31,7 @@ struct Variant {
   LaunchFn launch;  // one launch = kf fused steps from `in` to `out`
   hipError_t (*prepare)();  // one-time function attributes (dynamic LDS above 64 KiB)
   const char *name;
+  int cs = 0;  // 2-D 4x4x4: LDS storage column of tile column 0
 };
 
 const Variant *variants_1d_f64(int *count);
@@ -51,5 +52,6 @@ int bfrag_ksteps_3d_f64(int R, int shape);
 // Device-side synthetic input fill (same bits as the host generator).
 hipError_t launch_fill_random(double *a, const Geo &g, uint64_t stream_seed, hipStream_t s);
 hipError_t launch_zero(double *a, size_t count, hipStream_t s);
+hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t s);
 
 }  // namespace cs

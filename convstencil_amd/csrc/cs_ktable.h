@@ -173,3 +173,161 @@ constexpr KTable3D make_ktable3d(int R, int shape) {
 }
 
 }  // namespace cs
+
+namespace cs {
+
+// ---- dilated 4x4x4 table (2-D FP64, 8 x 8 output blocks) ----
+//
+// One v_mfma_f64_4x4x4 (4 blocks b, D_b[m][n] = sum_k A_b[m][k] B_b[k][n]) computes
+// an 8 x 8 output block: output (b, m, n) is cell (y + 2b + (n >> 1), x + 2m + (n & 1)),
+// i.e. the point (y + 2b, x + 2m) moved by v_n, v_n in {0, 1}^2.  With K running over
+// the dilated footprint T = {d + v : d a tap, v in {0, 1}^2} (tap d, t = d + v):
+//   A_b[m][t] = tile[y + 2b + t.y][x + 2m + t.x],   B[t][n] = W[t - v_n] (0 off the stencil),
+// B being the same for the four blocks.  Lane l holds A_b[m][k] with k = l >> 4,
+// b = (l >> 2) & 3, m = l & 3, so the 16 lanes of one k read four 64-byte row segments
+// two rows apart (no LDS bank conflict when the pitch is 4 mod 16 doubles), and
+// B[k][n] with n = l & 3; D_b[m][n] is in lane 16 m + 4 b + n.
+//
+// T has (2R + 2)^2 entries for a box (16 for box2d1r: 4 k-steps per 64 outputs,
+// 16 executed MACs per point-step against 9 useful), 8R + 4 for a star.  Entries
+// are grouped into 16-byte pairs -- t and t + (0, 1), read with one ds_read_b128 and
+// fed to two k-steps -- which must start at an even LDS address: every pair starts
+// at a column t.x of parity pp, and blocks start at columns x with x + CS + pp even
+// (CS: the tile's storage column shift).  A row of T whose ends have the wrong parity
+// leaves single entries (8-byte reads), or a pair padded with a zero-weight column;
+// the split/pad choice minimises k-steps, then LDS reads.  Pair steps (two k-steps
+// each) come first, single k-steps last, four entries (one per k lane group) per
+// step; unused slots repeat a live address with zero weight (live = 0).
+struct DTable {
+  int np;              // pair steps
+  int ns;              // single k-steps
+  int pp;              // parity of every pair's first column t.x
+  int pty[16][4], ptx[16][4], plive[16][4];
+  int sty[16][4], stx[16][4], slive[16][4];
+  int tmin, tmax;      // rows t.y read, relative to the block row 2b
+  int cmin, cmax;      // columns t.x read (a pair's second column included), relative to 2m
+};
+
+// Columns [lo, hi] of row ty of the dilated footprint T (lo > hi if the row is empty).
+constexpr void dilated_row(int R, int shape, int ty, int &lo, int &hi) {
+  lo = 1 << 20;
+  hi = -(1 << 20);
+  for (int vy = 0; vy <= 1; ++vy) {
+    const int dy = ty - vy;
+    if (dy < -R || dy > R) continue;
+    int a = 0, b = -1;
+    row_span(R, shape, dy, a, b);
+    if (a > b) continue;
+    lo = a < lo ? a : lo;
+    hi = b + 1 > hi ? b + 1 : hi;
+  }
+}
+
+constexpr int ceil4(int v) { return (v + 3) / 4; }
+
+constexpr DTable make_dtable(int R, int shape) {
+  DTable best{};
+  int best_ks = 1 << 30, best_rd = 1 << 30;
+  for (int pp = 0; pp < 2; ++pp) {
+    int pty[64] = {}, ptx[64] = {}, np0 = 0;
+    int sty[64] = {}, stx[64] = {}, side[64] = {}, ns0 = 0;
+    for (int ty = -R; ty <= R + 1; ++ty) {
+      int lo = 0, hi = -1;
+      dilated_row(R, shape, ty, lo, hi);
+      if (lo > hi) continue;
+      int a = lo;
+      if ((((a % 2) + 2) % 2) != pp) {  // left end of the wrong parity
+        sty[ns0] = ty;
+        stx[ns0] = a;
+        side[ns0] = -1;
+        ++ns0;
+        ++a;
+      }
+      for (; a + 1 <= hi; a += 2) {
+        pty[np0] = ty;
+        ptx[np0] = a;
+        ++np0;
+      }
+      if (a == hi) {  // right end left over
+        sty[ns0] = ty;
+        stx[ns0] = a;
+        side[ns0] = 1;
+        ++ns0;
+      }
+    }
+    // split the last j pairs into singles; pad the first k singles into pairs
+    for (int j = 0; j <= np0; ++j)
+      for (int k = 0; k <= ns0; ++k) {
+        const int P = np0 - j + k, S = ns0 - k + 2 * j;
+        const int ks = 2 * ceil4(P) + ceil4(S), rd = ceil4(P) + ceil4(S);
+        if (ks > best_ks || (ks == best_ks && rd >= best_rd)) continue;
+        if (P > 64 || S > 64) continue;
+        best_ks = ks;
+        best_rd = rd;
+        DTable t{};
+        t.pp = pp;
+        int fy[64] = {}, fx[64] = {}, gy[64] = {}, gx[64] = {};
+        int nP = 0, nS = 0;
+        for (int i = 0; i < np0 - j; ++i) {
+          fy[nP] = pty[i];
+          fx[nP] = ptx[i];
+          ++nP;
+        }
+        for (int i = 0; i < k; ++i) {  // a single at the left end pairs with the column before it
+          fy[nP] = sty[i];
+          fx[nP] = side[i] < 0 ? stx[i] - 1 : stx[i];
+          ++nP;
+        }
+        for (int i = k; i < ns0; ++i) {
+          gy[nS] = sty[i];
+          gx[nS] = stx[i];
+          ++nS;
+        }
+        for (int i = np0 - j; i < np0; ++i)
+          for (int h = 0; h < 2; ++h) {
+            gy[nS] = pty[i];
+            gx[nS] = ptx[i] + h;
+            ++nS;
+          }
+        t.np = ceil4(nP);
+        t.ns = ceil4(nS);
+        for (int e = 0; e < 4 * t.np; ++e) {
+          const bool live = e < nP;
+          const int src = live ? e : 4 * (e / 4);
+          t.pty[e / 4][e % 4] = fy[src];
+          t.ptx[e / 4][e % 4] = fx[src];
+          t.plive[e / 4][e % 4] = live ? 1 : 0;
+        }
+        for (int e = 0; e < 4 * t.ns; ++e) {
+          const bool live = e < nS;
+          const int src = live ? e : 4 * (e / 4);
+          t.sty[e / 4][e % 4] = gy[src];
+          t.stx[e / 4][e % 4] = gx[src];
+          t.slive[e / 4][e % 4] = live ? 1 : 0;
+        }
+        t.tmin = 1 << 20;
+        t.tmax = -(1 << 20);
+        t.cmin = 1 << 20;
+        t.cmax = -(1 << 20);
+        for (int e = 0; e < nP; ++e) {
+          t.tmin = fy[e] < t.tmin ? fy[e] : t.tmin;
+          t.tmax = fy[e] > t.tmax ? fy[e] : t.tmax;
+          t.cmin = fx[e] < t.cmin ? fx[e] : t.cmin;
+          t.cmax = fx[e] + 1 > t.cmax ? fx[e] + 1 : t.cmax;
+        }
+        for (int e = 0; e < nS; ++e) {
+          t.tmin = gy[e] < t.tmin ? gy[e] : t.tmin;
+          t.tmax = gy[e] > t.tmax ? gy[e] : t.tmax;
+          t.cmin = gx[e] < t.cmin ? gx[e] : t.cmin;
+          t.cmax = gx[e] > t.cmax ? gx[e] : t.cmax;
+        }
+        best = t;
+      }
+  }
+  return best;
+}
+
+// Number of k-steps of a dilated table.
+constexpr int dtable_ksteps(const DTable &t) { return 2 * t.np + t.ns; }
+
+}  // namespace cs

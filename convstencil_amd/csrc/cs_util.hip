@@ -21,6 +21,17 @@ __global__ void zero_kernel(double *a, size_t count) {
   for (; i < count; i += stride) a[i] = 0.0;
 }
 
+__global__ void copy_kernel(const double *__restrict__ src, double *__restrict__ dst, size_t count) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const double *src, double *dst, size_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, src, dst, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero(double *a, size_t count, hipStream_t s) {
   if (count == 0) return hipSuccess;
   size_t blocks = (count + 255) / 256;

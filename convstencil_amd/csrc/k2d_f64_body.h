@@ -1,0 +1,286 @@
+// k2d_f64_body.h -- plain FP64 2-D ConvStencil tile body for gfx950 (FP64 MFMA).
+//
+// Replaces kernel2d (reference src/2d/gpu.cu:30-70) and its launchers
+// gpu_box_2d1r / gpu_box_2d3r (src/2d/gpu.cu:245-425).  What changes for CDNA4:
+//   * the A operand is the input tile itself, read straight from LDS (see
+//     cs_ktable.h), so the reference's lookup-table scatter into two stencil2row
+//     copies is gone and each input element is stored in LDS once;
+//   * B is a banded Toeplitz slice of the weights, held in registers for the
+//     whole kernel, one f64 per lane per k-step;
+//   * two MFMA shapes: v_mfma_f64_4x4x4 on 8 x 8 output blocks with the dilated
+//     table (16-byte LDS reads, fewest wasted flops; the default) and
+//     v_mfma_f64_16x16x4 on 16 x 16 blocks (for comparison);
+//   * the time loop is fused inside the kernel: a workgroup loads a tile with a
+//     halo of KF*R cells, advances it KF steps in LDS, keeping the domain's halo
+//     cells fixed after every step, and writes only the final BY x BX block.  This
+//     equals KF separate steps exactly in exact arithmetic -- unlike the
+//     reference's pre-composed K^3 weights, which differ from stepping in the outer
+//     (KF-1)*R ring of the domain;
+//   * every fused step is its own compile-time instance (region, block grid and
+//     blocks per wave are constants), and each wave computes all of its blocks of a
+//     step into registers before any result is stored: the blocks' LDS reads and
+//     MFMA chains are independent, so they overlap without branches between them;
+//   * two staging modes: ST = 0 ping-pongs between two LDS buffers; ST = 1 keeps one
+//     buffer and writes the registers back between two barriers (half the LDS, so
+//     more workgroups share a CU and one's HBM loads overlap the others' MFMAs);
+//   * workgroups are mapped to tiles so that each XCD (own L2) sweeps a contiguous
+//     band of tile rows, keeping vertical halo re-reads inside one L2.
+#pragma once
+#include "cs_internal.h"
+
+namespace cs {
+
+template <int MF_, int R_, int SH_, int KF_, int BY_, int BX_, int NW_, int ST_>
+struct Cfg2D {
+  static constexpr int MF = MF_, R = R_, SH = SH_, KF = KF_, BY = BY_, BX = BX_, NW = NW_, ST = ST_;
+  static constexpr int EY = BY + 2 * KF * R;
+  static constexpr int EX = BX + 2 * KF * R;
+  static constexpr int BH = MF == 16 ? 16 : 8;  // output block rows
+  static constexpr int BW = MF == 16 ? 16 : 8;  // output block columns
+  static constexpr KTable KT = make_ktable(R, SH);  // 16x16x4
+  static constexpr DTable DT = make_dtable(R, SH);  // 4x4x4
+  static constexpr int NKS = MF == 16 ? KT.n : dtable_ksteps(DT);
+  static constexpr int NOFF = MF == 16 ? KT.n : DT.np + DT.ns;  // LDS reads per block
+  // 4x4x4: tile column c is stored at column c + CS; every block starts at a column x
+  // with x + CS + pp even, so that all 16-byte pair reads are aligned.  The first block
+  // of a step may start one column left of the region.
+  static constexpr int CS = MF == 16 ? 0 : ((1 - R - DT.cmin) > 0 ? (1 - R - DT.cmin) : 0);
+  static constexpr int PP = MF == 16 ? 0 : DT.pp;
+
+  // region of fused step s: rows [LO, LO + H), columns [LO, HI)
+  static constexpr int H(int s) { return BY + 2 * R * (KF - 1 - s); }
+  static constexpr int W(int s) { return BX + 2 * R * (KF - 1 - s); }
+  static constexpr int LO(int s) { return R * (s + 1); }
+  static constexpr int HI(int s) { return LO(s) + W(s); }
+  static constexpr int LOX(int s) { return MF == 16 ? LO(s) : LO(s) - ((LO(s) + CS + PP) & 1); }
+  static constexpr int LASTX(int s) {
+    return MF == 16 ? HI(s) - 16 : (HI(s) - 8) + (((HI(s) - 8) + CS + PP) & 1);
+  }
+  static constexpr int NBY(int s) { return (H(s) + BH - 1) / BH; }
+  static constexpr int NBX(int s) { return (LASTX(s) - LOX(s) + BW - 1) / BW + 1; }
+  static constexpr int NB(int s) { return NBY(s) * NBX(s); }
+  static constexpr int PERS(int s) { return (NB(s) + NW - 1) / NW; }
+
+  // columns and rows read relative to a block's (y, x)
+  static constexpr int RD_C0 = MF == 16 ? 0 : DT.cmin;
+  static constexpr int RD_C1 = MF == 16 ? 15 + ktable_overrun(KT) : 6 + DT.cmax;
+  static constexpr int RD_R0 = MF == 16 ? -R : DT.tmin;
+  static constexpr int RD_R1 = MF == 16 ? 15 + R : 6 + DT.tmax;
+  static constexpr int max_col() {
+    int m = 0;
+    for (int s = 0; s < KF; ++s) m = LASTX(s) + RD_C1 > m ? LASTX(s) + RD_C1 : m;
+    return m;
+  }
+  static constexpr int min_col() {
+    int m = 1 << 20;
+    for (int s = 0; s < KF; ++s) m = LOX(s) + RD_C0 < m ? LOX(s) + RD_C0 : m;
+    return m;
+  }
+  static constexpr int PX0 = CS + (max_col() + 1 > EX ? max_col() + 1 : EX);
+  // 16x16x4: odd pitch; 4x4x4: pitch = 4 mod 8, so the four 64-byte row segments that
+  // one k lane group reads (rows 2 apart) fall in distinct quarters of the banks.
+  static constexpr int PX = MF == 16 ? (PX0 | 1) : PX0 + ((4 - PX0 % 8) + 8) % 8;
+  static constexpr int NBUF = (KF > 1 && ST == 0) ? 2 : 1;
+  static constexpr int BUF = EY * PX;
+  static constexpr size_t LDS = (size_t)NBUF * BUF * sizeof(double);
+  static_assert(BY % 16 == 0 && BX % 16 == 0, "tile must be a multiple of 16");
+  static_assert(min_col() + CS >= 0, "block reads left of the LDS row");
+  static_assert(MF == 16 || (PX % 2 == 0 && (CS + PP + LOX(0)) % 2 == 0), "pair reads must be aligned");
+  static_assert(LO(0) + RD_R0 >= 0 && LO(0) + H(0) - BH + RD_R1 < EY, "block reads outside the tile rows");
+  static_assert(LDS <= 160 * 1024, "tile does not fit gfx950 LDS");
+};
+
+__device__ __forceinline__ int64_t xcd_tile(int64_t wg, int64_t ntiles) {
+  // Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
+  // contiguous tile range [x * per, (x + 1) * per).
+  const int64_t per = (ntiles + 7) / 8;
+  return (wg % 8) * per + wg / 8;
+}
+
+// MFMA result of one block: 4 f64 per lane (16x16x4) or 1 (4x4x4).
+template <int MF>
+struct Acc;
+template <>
+struct Acc<16> {
+  d4 v;
+};
+template <>
+struct Acc<4> {
+  double v;
+};
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+// Per-workgroup context of one tile.
+struct TileCtx {
+  int lane, wave;
+  int orow, ocol;  // 4x4x4: the output cell of this lane relative to the block
+  bool full;       // the whole tile (with halo) lies inside the domain interior
+  int64_t Y0, X0, ny, nx, ay0, ax0, pitch;
+};
+
+// One block's MFMA chain: A from LDS at p + off[], B from registers.
+template <class C>
+__device__ __forceinline__ Acc<C::MF> block_mfma(const double *p, const int *off, const double *b) {
+  Acc<C::MF> a;
+  if constexpr (C::MF == 16) {
+    a.v = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < C::NKS; ++ks) a.v = __builtin_amdgcn_mfma_f64_16x16x4f64(p[off[ks]], b[ks], a.v, 0, 0, 0);
+  } else {
+    a.v = 0.0;
+#pragma unroll
+    for (int ps = 0; ps < C::DT.np; ++ps) {  // one 16-byte read feeds two k-steps
+      const dbl2 pr = *reinterpret_cast<const dbl2 *>(p + off[ps]);
+      a.v = __builtin_amdgcn_mfma_f64_4x4x4f64(pr.x, b[2 * ps], a.v, 0, 0, 0);
+      a.v = __builtin_amdgcn_mfma_f64_4x4x4f64(pr.y, b[2 * ps + 1], a.v, 0, 0, 0);
+    }
+#pragma unroll
+    for (int ss = 0; ss < C::DT.ns; ++ss)
+      a.v = __builtin_amdgcn_mfma_f64_4x4x4f64(p[off[C::DT.np + ss]], b[2 * C::DT.np + ss], a.v, 0, 0, 0);
+  }
+  return a;
+}
+
+// Block `blk` of step S -> its first output cell (y, x) in tile coordinates.
+template <class C, int S>
+__device__ __forceinline__ void block_origin(int blk, int &y, int &x) {
+  constexpr int NBX = C::NBX(S);
+  const int ba = blk / NBX, bb = blk - ba * NBX;
+  y = C::LO(S) + min(C::BH * ba, C::H(S) - C::BH);
+  x = min(C::LOX(S) + C::BW * bb, C::LASTX(S));
+}
+
+// Stores a block's results to LDS (tile coordinates) or, in the last step, to the
+// output grid.  Cells outside the domain interior are skipped (they keep their halo
+// value), and so are columns outside the step's region [lo, hi).
+template <class C, bool GLOBAL>
+__device__ __forceinline__ void put_block(const Acc<C::MF> &a, int y, int x, int lo, int hi, const TileCtx &t,
+                                          double *lds_dst, double *out) {
+  if constexpr (C::MF == 16) {
+    const int tx = x + (t.lane & 15);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int ty = y + 4 * v + (t.lane >> 4);
+      if (t.full || (t.Y0 + ty >= 0 && t.Y0 + ty < t.ny && t.X0 + tx >= 0 && t.X0 + tx < t.nx)) {
+        if constexpr (GLOBAL)
+          out[(t.ay0 + ty) * t.pitch + t.ax0 + tx] = a.v[v];
+        else
+          lds_dst[ty * C::PX + tx] = a.v[v];
+      }
+    }
+  } else {
+    const int ty = y + t.orow, tx = x + t.ocol;
+    if (tx >= lo && tx < hi && (t.full || (t.Y0 + ty >= 0 && t.Y0 + ty < t.ny && t.X0 + tx >= 0 && t.X0 + tx < t.nx))) {
+      if constexpr (GLOBAL)
+        out[(t.ay0 + ty) * t.pitch + t.ax0 + tx] = a.v;
+      else
+        lds_dst[ty * C::PX + tx + C::CS] = a.v;
+    }
+  }
+}
+
+// Fused step S: every wave computes its blocks (wave, wave + NW, ...) into registers,
+// then stores them.  In the last round a wave without a block of its own recomputes
+// the step's last block and stores nothing.
+template <class C, int S>
+__device__ __forceinline__ void fused_step(const TileCtx &t, const double *b, const int *off, double *lds,
+                                           double *out) {
+  constexpr bool LAST = S == C::KF - 1;
+  constexpr int NB = C::NB(S), PERS = C::PERS(S), NW = C::NW;
+  const double *src = lds + (C::NBUF == 2 ? (S & 1) * C::BUF : 0);
+  double *dst = lds + (C::NBUF == 2 ? ((S + 1) & 1) * C::BUF : 0);
+  Acc<C::MF> acc[PERS];
+#pragma unroll
+  for (int j = 0; j < PERS; ++j) {
+    int y, x;
+    block_origin<C, S>(min(j * NW + t.wave, NB - 1), y, x);
+    acc[j] = block_mfma<C>(src + y * C::PX + x, off, b);
+  }
+  if constexpr (!LAST && C::NBUF == 1) __syncthreads();  // every wave has read this step's input
+#pragma unroll
+  for (int j = 0; j < PERS; ++j) {
+    const int blk = j * NW + t.wave;
+    if ((NB % NW == 0 || j < PERS - 1) || blk < NB) {
+      int y, x;
+      block_origin<C, S>(blk, y, x);
+      put_block<C, LAST>(acc[j], y, x, C::LO(S), C::HI(S), t, dst, out);
+    }
+  }
+  if constexpr (!LAST) __syncthreads();  // this step's output is complete before it is read
+}
+
+template <class C, int S>
+__device__ __forceinline__ void fused_steps(const TileCtx &t, const double *b, const int *off, double *lds,
+                                            double *out) {
+  if constexpr (S < C::KF) {
+    fused_step<C, S>(t, b, off, lds, out);
+    fused_steps<C, S + 1>(t, b, off, lds, out);
+  }
+}
+
+// One workgroup advances tile `tile` of grid g by KF steps (lds: the kernel's dynamic LDS).
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+__device__ __forceinline__ void tile_f64(const double *__restrict__ in, double *__restrict__ out,
+                                         const double *__restrict__ bfrag, const Geo &g, int64_t tile,
+                                         double *__restrict__ lds) {
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
+  constexpr int NKS = C::NKS;
+  constexpr int PX = C::PX;
+  constexpr int CS = C::CS;
+
+  const int tid = threadIdx.x;
+  TileCtx t;
+  t.lane = tid & 63;
+  t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  t.pitch = g.ext[2];
+  t.ny = g.n[1];
+  t.nx = g.n[2];
+  // interior coordinates of tile cell (0, 0)
+  t.Y0 = (tile / g.tiles[2]) * BY - KF * R;
+  t.X0 = (tile % g.tiles[2]) * BX - KF * R;
+  t.ay0 = g.org[1] + t.Y0;
+  t.ax0 = g.org[2] + t.X0;
+  t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
+  const int lane = t.lane;
+
+  double b[NKS];
+#pragma unroll
+  for (int ks = 0; ks < NKS; ++ks) b[ks] = bfrag[ks * 64 + lane];
+
+  // Per-lane LDS offsets of the A operand relative to the block's (y, x) cell.
+  int off[C::NOFF];
+  if constexpr (MF == 16) {
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) off[ks] = ((lane & 15) + C::KT.dy[ks]) * PX + C::KT.c0[ks] + (lane >> 4);
+    t.orow = 0;
+    t.ocol = 0;
+  } else {
+    const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
+#pragma unroll
+    for (int ps = 0; ps < C::DT.np; ++ps) off[ps] = (2 * b4 + C::DT.pty[ps][q]) * PX + 2 * m + C::DT.ptx[ps][q] + CS;
+#pragma unroll
+    for (int ss = 0; ss < C::DT.ns; ++ss)
+      off[C::DT.np + ss] = (2 * b4 + C::DT.sty[ss][q]) * PX + 2 * m + C::DT.stx[ss][q] + CS;
+    // D_b[m][n] in lane 16 m + 4 b + n is cell (2 b + (n >> 1), 2 m + (n & 1))
+    t.orow = 2 * b4 + ((lane & 3) >> 1);
+    t.ocol = 2 * (lane >> 4) + (lane & 1);
+  }
+
+  // ---- load the tile (and its halo); cells outside the array and pad columns are 0 ----
+  const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
+  for (int idx = tid; idx < C::BUF; idx += NW * 64) {
+    const int r = idx / PX, c = idx - r * PX - CS;  // storage column idx - r*PX holds tile column c
+    const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
+    double v = 0.0;
+    if (c >= 0 && c < C::EX && (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch)))
+      v = in[ay * t.pitch + ax];
+    lds[idx] = v;
+    if constexpr (C::NBUF == 2) lds[C::BUF + idx] = v;
+  }
+  __syncthreads();
+
+  fused_steps<C, 0>(t, b, off, lds, out);
+}
+
+}  // namespace cs

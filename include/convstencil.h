@@ -122,6 +122,24 @@ cs_status cs_plan_device_buffers(cs_plan *plan, double **current, double **other
 /* Human-readable description of the kernel choice (variant, tile, fusion depth). */
 cs_status cs_plan_describe(cs_plan *plan, char *buf, int32_t len);
 
+/* ---- batches: many independent 2-D grids advanced by one persistent launch ----
+ * No counterpart in the reference (one grid per process).  Every grid has its own
+ * layout and stencil; supported stencils are the radius-1 box (box2d1r) and the
+ * radius-3 star (star2d3r) in FP64, and radius 1 or 3 in CS_PREC_BF16X3.  cs_batch_run advances every grid by
+ * `steps` steps with a single persistent kernel launch (plus a small reset kernel),
+ * optionally captured in a hipGraph (opt->use_graph). */
+typedef struct cs_batch cs_batch;
+cs_status cs_batch_create(cs_batch **batch, int32_t count, const cs_layout *layouts, const cs_stencil *stencils,
+                          const cs_options *opt);
+cs_status cs_batch_destroy(cs_batch *batch);
+cs_status cs_batch_upload(cs_batch *batch, int32_t grid, const double *host_in);
+/* Seeds grid i with seeds[i] (as cs_plan_fill_random). */
+cs_status cs_batch_fill_random(cs_batch *batch, const uint64_t *seeds);
+cs_status cs_batch_run(cs_batch *batch, int64_t steps, cs_report *report);
+cs_status cs_batch_time(cs_batch *batch, int64_t steps, int32_t warmup, int32_t reps, cs_report *report);
+cs_status cs_batch_download(cs_batch *batch, int32_t grid, double *host_out);
+cs_status cs_batch_describe(cs_batch *batch, char *buf, int32_t len);
+
 /* ---- one-shot host-to-host run (what the reference launchers do, minus printing) ---- */
 cs_status cs_run(const cs_layout *layout, const cs_stencil *stencil, const double *host_in, double *host_out,
                  int64_t steps, const cs_options *opt, cs_report *report);

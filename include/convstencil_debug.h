@@ -18,14 +18,21 @@ extern "C" {
 /* Number of compiled kernel variants of a family: 1, 2, 3 = FP64 kernels for 1-, 2-, 3-D grids;
  * 4 = 2-D split-precision kernels (there info[1] holds the cs_precision). */
 int32_t cs_debug_variant_count(int32_t ndim);
-/* info[0..10] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
+/* info[0..11] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
  * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
  * (doubles), staging of fused steps (0 = two LDS buffers, 1 = one buffer + registers; for 3-D
- * variants: the z-chunk length).  Returns 0, or -1 if i is out of range.  name receives the variant's name. */
+ * variants: the z-chunk length), LDS storage column of tile column 0 (2-D 4x4x4 variants, else 0).
+ * Returns 0, or -1 if i is out of range.  name receives the variant's name. */
 int32_t cs_debug_variant_info(int32_t ndim, int32_t i, int64_t *info, char *name, int32_t len);
-/* k-step table of an MFMA shape: dy[], c0[], live[] (cap entries each); returns the table length
- * (k-steps for 16x16x4, flattened entries for 4x4x4), or -1 if cap is too small. */
+/* k-step table of the 16x16x4 kernels (mf = 16) or the flattened 4x4x4 table of the 1-D kernels'
+ * layout (mf = 4): dy[], c0[], live[] (cap entries each); returns the table length, or -1. */
 int32_t cs_debug_ktable(int32_t mf, int32_t radius, int32_t shape, int32_t *dy, int32_t *c0, int32_t *live,
+                        int32_t cap);
+/* Dilated 4x4x4 table of the 2-D FP64 kernels (8 x 8 output blocks): hdr = {pair steps np, single
+ * steps ns, pair column parity}; entry e = 4 * step + q (pair steps first, then single steps) is the
+ * footprint cell (ty[e], tx[e]) read by lane group q (a pair also reads tx + 1), live[e] = 0 for a
+ * zero-weight slot.  Returns 4 * (np + ns), or -1 if cap is too small. */
+int32_t cs_debug_dtable(int32_t radius, int32_t shape, int32_t *hdr, int32_t *ty, int32_t *tx, int32_t *live,
                         int32_t cap);
 /* B fragments bfrag[ks * 64 + lane] for dense weights w[(2r+1)^2]; returns the k-step count,
  * or -1 if cap (doubles) is too small. */
@@ -48,6 +55,10 @@ void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49);
 /* Split-precision B fragments as raw slice elements (bf16 / fp32 / fp16 by precision), layout
  * [weight slice][instruction][lane][element]; returns the byte count, or 0 if cap is too small. */
 int64_t cs_debug_bfrag_split(int32_t precision, int32_t radius, const double *w, void *out, int64_t cap);
+
+/* Times `reps` plain device copies of `count` doubles (needs a GPU); the kernel is the
+ * reference for HBM byte counters.  Returns 0 on success. */
+int32_t cs_debug_copy_bench(int64_t count, int32_t reps, double *ms_per_copy);
 
 /* Runs the FP64 MFMA lane-layout probe on the current device (needs a GPU).  which = 0:
  * v_mfma_f64_16x16x4, host_out[64 * 64 * 4] = D[lane][item] for A one-hot at lane p (p-major);

@@ -7,7 +7,9 @@ block coverage, halo masking and MFMA lane algebra in numpy, with:
   * the MFMA operand/result lane layouts measured on gfx950 (see the layout test
     in tests/test_gpu_mfma_layout.py):
       v_mfma_f64_16x16x4: A[l%16][l/16], B[l/16][l%16], D[4v + l/16][l%16]
-      v_mfma_f64_4x4x4:   block (l>>2)&3, A[l&3][l>>4], B[l>>4][l&3], D[l>>4][l&3];
+      v_mfma_f64_4x4x4:   block (l>>2)&3, A[l&3][l>>4], B[l>>4][l&3], D[l>>4][l&3]
+                          (2-D: block b's D[m][n] is cell (2b + (n>>1), 2m + (n&1)) of
+                          an 8 x 8 output block, the dilated-table mapping);
   * an assertion on every LDS and global index (in bounds of the buffers the
     kernel allocates), so an out-of-bounds access is found here, not on a GPU;
   * the result compared with the CPU oracle (1e-13 relative).
@@ -43,10 +45,10 @@ def variants(ndim=2):
     lib = _dbg()
     out = []
     for i in range(lib.cs_debug_variant_count(ndim)):
-        info = (ctypes.c_int64 * 11)()
+        info = (ctypes.c_int64 * 12)()
         name = ctypes.create_string_buffer(128)
         assert lib.cs_debug_variant_info(ndim, i, info, name, 128) == 0
-        keys = ["id", "mf", "R", "shape", "kf", "by", "bx", "nw", "lds", "px", "st"]
+        keys = ["id", "mf", "R", "shape", "kf", "by", "bx", "nw", "lds", "px", "st", "cs"]
         d = dict(zip(keys, [int(v) for v in info]))
         d["name"] = name.value.decode()
         out.append(d)
@@ -73,23 +75,35 @@ def bfrag(mf, R, shape, w):
     return out[:nks * 64].reshape(nks, 64)
 
 
+def dtable(R, shape):
+    """Dilated 4x4x4 table: (np, ns, pp, ty[e], tx[e], live[e]) with e = 4 * step + q."""
+    lib = _dbg()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.cs_debug_dtable.restype = ctypes.c_int32
+    lib.cs_debug_dtable.argtypes = [ctypes.c_int32, ctypes.c_int32, ip, ip, ip, ip, ctypes.c_int32]
+    cap = 256
+    hdr = (ctypes.c_int32 * 3)()
+    ty, tx, live = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+    n = lib.cs_debug_dtable(R, shape, hdr, ty, tx, live, cap)
+    assert n > 0 and n == 4 * (hdr[0] + hdr[1])
+    return hdr[0], hdr[1], hdr[2], np.array(ty[:n]).reshape(-1, 4), np.array(tx[:n]).reshape(-1, 4), \
+        np.array(live[:n]).reshape(-1, 4)
+
+
 def lane_offsets(v, tab):
-    """Per-lane LDS offsets off[ks][lane] relative to the block's (y, x) cell."""
-    dy, c0, live = tab
+    """Per-lane LDS offsets off[read][lane] relative to the block's (y, x) cell."""
     px = v["px"]
     lanes = np.arange(64)
     if v["mf"] == 16:
+        dy, c0, live = tab
         return np.array([((lanes & 15) + dy[ks]) * px + c0[ks] + (lanes >> 4) for ks in range(len(dy))])
-    nks = len(dy) // 4
-    k, b4, i = lanes >> 4, (lanes >> 2) & 3, lanes & 3
-    f = 4 * np.arange(nks)[:, None] + k[None, :]
-    dyf = np.array(dy)[f]
-    cf = np.array(c0)[f]
-    return (i[None, :] + dyf) * px + 4 * b4[None, :] + cf
+    npair, ns, pp, ty, tx, live = tab
+    q, b4, m = lanes >> 4, (lanes >> 2) & 3, lanes & 3
+    return np.array([(2 * b4 + ty[e][q]) * px + 2 * m + tx[e][q] + v["cs"] for e in range(npair + ns)])
 
 
 def mfma_block(v, a_lanes, bf):
-    """a_lanes[ks][lane], bf[ks][lane] -> output value per (lane, v) as dict of cell offsets."""
+    """a_lanes[ks][lane], bf[ks][lane] -> (rows, cols, vals) of the block's outputs per lane."""
     lanes = np.arange(64)
     if v["mf"] == 16:
         acc = np.zeros((16, 16))
@@ -104,7 +118,7 @@ def mfma_block(v, a_lanes, bf):
         cols = np.array([[l % 16] * 4 for l in lanes])
         vals = acc[rows, cols]
         return rows, cols, vals
-    acc = np.zeros((4, 4, 4))  # block, i, j
+    acc = np.zeros((4, 4, 4))  # block, m, n
     b4 = (lanes >> 2) & 3
     for ks in range(a_lanes.shape[0]):
         A = np.zeros((4, 4, 4))
@@ -112,19 +126,40 @@ def mfma_block(v, a_lanes, bf):
         A[b4, lanes & 3, lanes >> 4] = a_lanes[ks]
         B[b4, lanes >> 4, lanes & 3] = bf[ks]
         acc += A @ B
-    # lane l holds D_b[l>>4][l&3] -> cell (y + (l>>4), x + 4b + (l&3)) = (y + (l>>4), x + (l & 15))
-    rows = (lanes >> 4)[:, None]
-    cols = (lanes & 15)[:, None]
-    vals = acc[b4, lanes >> 4, lanes & 3][:, None]
+    # lane l = 16 m + 4 b + n holds D_b[m][n] = cell (2b + (n >> 1), 2m + (n & 1))
+    n = lanes & 3
+    rows = (2 * b4 + (n >> 1))[:, None]
+    cols = (2 * (lanes >> 4) + (n & 1))[:, None]
+    vals = acc[b4, lanes >> 4, n][:, None]
     return rows, cols, vals
+
+
+def a_operands(v, tab, lds, idx):
+    """The A operand of every k-step, as the kernel feeds it: pair reads give two k-steps."""
+    if v["mf"] == 16:
+        return lds[idx]
+    npair, ns = tab[0], tab[1]
+    out = []
+    for e in range(npair):  # 16-byte read at idx[e] (aligned): .x then .y
+        assert np.all(idx[e] % 2 == 0), (v["name"], "misaligned pair read")
+        out.append(lds[idx[e]])
+        out.append(lds[idx[e] + 1])
+    for e in range(npair, npair + ns):
+        out.append(lds[idx[e]])
+    return np.array(out)
 
 
 def emulate(v, layout, dense, x):
     """Replays one launch of variant v (v['kf'] fused steps) on padded array x."""
     R, kf, by, bx, px = v["R"], v["kf"], v["by"], v["bx"], v["px"]
-    tab = ktable(v["mf"], R, v["shape"])
+    tab = ktable(v["mf"], R, v["shape"]) if v["mf"] == 16 else dtable(R, v["shape"])
     bf = bfrag(v["mf"], R, v["shape"], dense)
     offs = lane_offsets(v, tab)
+    CS = v["cs"]
+    if v["mf"] == 4:
+        assert px % 8 == 4
+        pp = tab[2]
+        assert bf.shape[0] == 2 * tab[0] + tab[1]
     EY, EX = by + 2 * kf * R, bx + 2 * kf * R
     BUF = EY * px
     nbuf = 2 if (kf > 1 and v["st"] == 0) else 1
@@ -138,7 +173,7 @@ def emulate(v, layout, dense, x):
     per = -(-ntiles // 8)
     grid = per * 8
     seen = set()
-    BH = 16 if v["mf"] == 16 else 4
+    BH = BW = 16 if v["mf"] == 16 else 8
     for wg in range(grid):
         tile = (wg % 8) * per + wg // 8
         if tile >= ntiles:
@@ -151,9 +186,10 @@ def emulate(v, layout, dense, x):
         lds = np.full(nbuf * BUF, np.nan)
         for idx in range(BUF):
             r, c = divmod(idx, px)
+            c -= CS
             ay, ax = ay0 + r, ax0 + c
             val = 0.0
-            if c < EX and 0 <= ay < ext[1] and 0 <= ax < pitch:
+            if 0 <= c < EX and 0 <= ay < ext[1] and 0 <= ax < pitch:
                 gi = ay * pitch + ax
                 assert 0 <= gi < layout.cells
                 val = x[gi]
@@ -167,24 +203,31 @@ def emulate(v, layout, dense, x):
             dst = ((s + 1) & 1) * BUF if nbuf == 2 else 0
             H, W = by + 2 * R * (kf - 1 - s), bx + 2 * R * (kf - 1 - s)
             base = R * (s + 1)
-            nby, nbx = -(-H // BH), -(-W // 16)
+            if v["mf"] == 16:
+                lox, lastx = base, base + W - 16
+            else:  # block starts with x + CS + pp even, the first possibly one column early
+                lox = base - ((base + CS + pp) & 1)
+                lastx = (base + W - 8) + (((base + W - 8) + CS + pp) & 1)
+            nby, nbx = -(-H // BH), -(-(lastx - lox) // BW) + 1
             new_vals = []
             for blk in range(nby * nbx):
                 ba, bb = divmod(blk, nbx)
                 y = base + min(BH * ba, H - BH)
-                xx = base + min(16 * bb, W - 16)
+                xx = min(lox + BW * bb, lastx)
                 idx = src + y * px + xx + offs
-                assert idx.min() >= src and idx.max() < src + BUF, (v["name"], s, y, xx)
-                a = lds[idx]
+                top = idx.max() + (1 if v["mf"] == 4 else 0)  # a pair's second half
+                assert idx.min() >= src and top < src + BUF, (v["name"], s, y, xx)
+                # every LDS row read stays inside the row (no wrap into the next row)
+                a = a_operands(v, tab, lds, idx)
                 assert np.isfinite(a).all(), (v["name"], "reads an unwritten LDS cell")
                 rows, cols, vals = mfma_block(v, a, bf)
                 for rr, cc, vv in zip(rows.reshape(-1), cols.reshape(-1), vals.reshape(-1)):
                     ty, tx = y + rr, xx + cc
                     inside = full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
-                    if not inside:
+                    if not inside or tx < base or tx >= base + W:
                         continue
                     if s < kf - 1:
-                        li = dst + ty * px + tx
+                        li = dst + ty * px + tx + CS
                         assert dst <= li < dst + BUF
                         new_vals.append((li, vv))
                     else:
@@ -228,6 +271,30 @@ def test_variant_emulation(v):
         gi, wi = layout.interior(got), layout.interior(want)
         err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
         assert err < 1e-13, (v["name"], err)
+
+
+def test_dtables_cover_every_tap():
+    # every (tap, output cell) pair of an 8 x 8 block appears exactly once among the live
+    # k-slots of the dilated table: slot t serves output v_n through tap t - v_n
+    for R in (1, 2, 3):
+        for shape in (0, 1):
+            npair, ns, pp, ty, tx, live = dtable(R, shape)
+            slots = []
+            for e in range(4 * npair):
+                if live[e // 4][e % 4]:
+                    assert (tx[e // 4][e % 4] - pp) % 2 == 0  # aligned pair start
+                    slots += [(ty[e // 4][e % 4], tx[e // 4][e % 4]), (ty[e // 4][e % 4], tx[e // 4][e % 4] + 1)]
+            for e in range(4 * npair, 4 * (npair + ns)):
+                if live[e // 4][e % 4]:
+                    slots.append((ty[e // 4][e % 4], tx[e // 4][e % 4]))
+            taps = {(a, b) for a in range(-R, R + 1) for b in range(-R, R + 1) if shape == 0 or a == 0 or b == 0}
+            for (tdy, tdx) in taps:
+                for n in range(4):
+                    vy, vx = n >> 1, n & 1
+                    assert slots.count((tdy + vy, tdx + vx)) == 1, (R, shape, tdy, tdx, n)
+            # k-slot counts: the footprint size, rounded up to whole k-steps
+            foot = {(a + vy, b + vx) for (a, b) in taps for vy in (0, 1) for vx in (0, 1)}
+            assert 4 * (2 * npair + ns) < len(foot) + 8
 
 
 def test_ktables_cover_every_tap():
