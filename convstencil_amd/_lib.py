@@ -54,6 +54,15 @@ SIGNATURES = {
     "cs_plan_describe": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
     "cs_run": (ctypes.c_int, [ctypes.POINTER(cs_layout), ctypes.POINTER(cs_stencil), c_double_p, c_double_p,
                               ctypes.c_int64, ctypes.POINTER(cs_options), ctypes.POINTER(cs_report)]),
+    "cs_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32, ctypes.POINTER(cs_layout),
+                                       ctypes.POINTER(cs_stencil), ctypes.POINTER(cs_options)]),
+    "cs_batch_destroy": (ctypes.c_int, [_P]),
+    "cs_batch_upload": (ctypes.c_int, [_P, ctypes.c_int32, c_double_p]),
+    "cs_batch_fill_random": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "cs_batch_run": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.POINTER(cs_report)]),
+    "cs_batch_time": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(cs_report)]),
+    "cs_batch_download": (ctypes.c_int, [_P, ctypes.c_int32, c_double_p]),
+    "cs_batch_describe": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
     "cs_fill_random_host": (None, [c_double_p, ctypes.POINTER(cs_layout), ctypes.c_uint64]),
     "cs_fill_weights_host": (None, [c_double_p, ctypes.c_int32, ctypes.c_uint64]),
     "gpu_1d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int]),

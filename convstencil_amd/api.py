@@ -124,6 +124,71 @@ class Plan:
         return buf.value.decode()
 
 
+class Batch:
+    """Many independent 2-D grids advanced together by one persistent launch per run
+    (cs_batch_*): each grid has its own layout, radius and weights."""
+
+    def __init__(self, layouts, radii, weights_dense, precision="fp64", use_graph=False):
+        lib = _lib.load()
+        n = len(layouts)
+        if n == 0 or len(radii) != n or len(weights_dense) != n:
+            raise ValueError("need one radius and one weight array per layout")
+        self.layouts = list(layouts)
+        self._w = [np.ascontiguousarray(w, dtype=np.float64).reshape(-1) for w in weights_dense]
+        Ls = (_lib.cs_layout * n)(*[L.to_c() for L in self.layouts])
+        Ss = (_lib.cs_stencil * n)(*[_lib.cs_stencil(2, int(r), _dptr(w)) for r, w in zip(radii, self._w)])
+        self._opt = _options(precision=precision, use_graph=use_graph)
+        h = ctypes.c_void_p()
+        check(lib.cs_batch_create(ctypes.byref(h), n, Ls, Ss, ctypes.byref(self._opt)), "cs_batch_create")
+        self._h = h
+        self._lib = lib
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.cs_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, i, host):
+        a = _contig(host, self.layouts[i].cells)
+        check(self._lib.cs_batch_upload(self._h, int(i), _dptr(a)), "cs_batch_upload")
+
+    def fill_random(self, seeds):
+        s = (ctypes.c_uint64 * len(self.layouts))(*[int(v) for v in seeds])
+        check(self._lib.cs_batch_fill_random(self._h, s), "cs_batch_fill_random")
+
+    def run(self, steps):
+        r = _lib.cs_report()
+        check(self._lib.cs_batch_run(self._h, int(steps), ctypes.byref(r)), "cs_batch_run")
+        return Report(r)
+
+    def time(self, steps, warmup=1, reps=3):
+        r = _lib.cs_report()
+        check(self._lib.cs_batch_time(self._h, int(steps), int(warmup), int(reps), ctypes.byref(r)), "cs_batch_time")
+        return Report(r)
+
+    def download(self, i):
+        out = np.empty(self.layouts[i].cells, dtype=np.float64)
+        check(self._lib.cs_batch_download(self._h, int(i), _dptr(out)), "cs_batch_download")
+        return out
+
+    def describe(self):
+        buf = ctypes.create_string_buffer(512)
+        check(self._lib.cs_batch_describe(self._h, buf, 512), "cs_batch_describe")
+        return buf.value.decode()
+
+
 def run(layout: Layout, radius, weights_dense, host_in, steps, **opts):
     """One-shot host-to-host run; returns (padded output, Report)."""
     lib = _lib.load()

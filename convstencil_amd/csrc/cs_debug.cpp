@@ -119,6 +119,24 @@ int64_t cs_debug_bfrag_split(int32_t precision, int32_t radius, const double *w,
   return (int64_t)cs::build_bfrag_split(precision, radius, w, out, (size_t)cap);
 }
 
+int64_t cs_debug_bfrag_split2(int32_t precision, int32_t radius, int32_t shape, const double *w, void *out,
+                              int64_t cap) {
+  return (int64_t)cs::build_bfrag_split2(precision, radius, shape, w, out, (size_t)cap);
+}
+
+int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty, int32_t *tx, int32_t *live,
+                        int32_t cap) {
+  const cs::CTable t = cs::make_ctable(radius, shape);
+  if (t.n > cap || !c0) return -1;
+  *c0 = t.c0;
+  for (int i = 0; i < t.n; ++i) {
+    ty[i] = t.ty[i];
+    tx[i] = t.tx[i];
+    live[i] = t.live[i];
+  }
+  return t.n;
+}
+
 int32_t cs_debug_copy_bench(int64_t count, int32_t reps, double *ms_per_copy) {
   // Plain device-to-device copy of `count` doubles (one element per thread), `reps`
   // times: the reference point for HBM byte counters (FETCH_SIZE / WRITE_SIZE).

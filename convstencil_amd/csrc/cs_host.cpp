@@ -78,23 +78,27 @@ const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   return nullptr;
 }
 
-const Variant *find_variant(int ndim, int mf, int R, int shape, int kf, int prefer_id, int prec = CS_PREC_FP64) {
+// First variant of the family with this (MFMA shape or precision, radius, shape class,
+// fusion depth) and, if st >= 0, this Variant::st; or the variant with id prefer_id.
+const Variant *find_variant(int ndim, int mf, int R, int shape, int kf, int prefer_id, int prec = CS_PREC_FP64,
+                            int st = -1) {
   int cnt = 0;
   const Variant *vs = family(ndim, &cnt, prec);
   const Variant *best = nullptr;
   for (int i = 0; i < cnt; ++i) {
     const Variant &v = vs[i];
     if (prefer_id > 0 && v.id == prefer_id && v.R == R && v.shape == shape) return &v;
-    if (v.mf == mf && v.R == R && v.shape == shape && v.kf == kf && !best) best = &v;
+    if (v.mf == mf && v.R == R && v.shape == shape && v.kf == kf && (st < 0 || v.st == st) && !best) best = &v;
   }
   return best;
 }
 
-int max_kf(int ndim, int mf, int R, int shape, int prec = CS_PREC_FP64) {
+int max_kf(int ndim, int mf, int R, int shape, int prec = CS_PREC_FP64, int st = -1) {
   int cnt = 0, m = 0;
   const Variant *vs = family(ndim, &cnt, prec);
   for (int i = 0; i < cnt; ++i)
-    if (vs[i].mf == mf && vs[i].R == R && vs[i].shape == shape) m = std::max(m, vs[i].kf);
+    if (vs[i].mf == mf && vs[i].R == R && vs[i].shape == shape && (st < 0 || vs[i].st == st))
+      m = std::max(m, vs[i].kf);
   return m;
 }
 
@@ -104,30 +108,44 @@ cs_status select_variants(cs_plan *p) {
   if (nd < 1 || nd > 3) return CS_ERR_UNSUPPORTED;
   if (prec != CS_PREC_FP64 && (prec < CS_PREC_BF16X3 || prec > CS_PREC_FP16X2)) return CS_ERR_INVALID;
   if (prec != CS_PREC_FP64 && nd != 2) return CS_ERR_UNSUPPORTED;
-  // split-precision variants use the full weight square (Variant::mf holds the precision)
-  if (prec != CS_PREC_FP64) p->shape = cs::SHAPE_BOX;
+  // Split precision (Variant::mf holds the precision): the dilated form (st 2) for the
+  // stencil's shape class, else its box variants (zero weights off the stencil), else the
+  // Toeplitz form (st 0, box only).  The remainder launches come from the same form.
   const int mf = prec != CS_PREC_FP64 ? prec : 4;
   const Variant *v = nullptr;
   if (p->opt.tile_variant > 0) {
     v = find_variant(nd, 0, p->R, p->shape, 0, p->opt.tile_variant, prec);
+    if (!v && prec != CS_PREC_FP64) v = find_variant(nd, 0, p->R, cs::SHAPE_BOX, 0, p->opt.tile_variant, prec);
     if (!v) return CS_ERR_INVALID;
-  } else {
+  } else if (prec == CS_PREC_FP64) {
     const int top = max_kf(nd, mf, p->R, p->shape, prec);
     int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
     for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, p->shape, kf, 0, prec);
+  } else {
+    const int tries[3][2] = {{p->shape, 2}, {cs::SHAPE_BOX, 2}, {cs::SHAPE_BOX, 0}};
+    for (const auto &tr : tries) {
+      const int top = max_kf(nd, mf, p->R, tr[0], prec, tr[1]);
+      if (top == 0) continue;
+      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
+      for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, tr[0], kf, 0, prec, tr[1]);
+      if (v) break;
+    }
   }
   if (!v) return CS_ERR_UNSUPPORTED;
+  p->shape = v->shape;
+  const int fam = prec != CS_PREC_FP64 ? v->st : -1;
   p->vmain = v;
-  p->vone = find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec);
+  p->vone = find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec, fam);
   if (!p->vone) return CS_ERR_UNSUPPORTED;
   p->vrem.assign(v->kf, nullptr);
-  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0, prec);
+  for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0, prec, fam);
   for (const Variant *u : {p->vmain, p->vone}) CS_HIP(u->prepare());
   for (const Variant *u : p->vrem)
     if (u) CS_HIP(u->prepare());
   if (prec != CS_PREC_FP64) {
     std::vector<unsigned char> hb(1 << 20);
-    const size_t bytes = cs::build_bfrag_split(prec, p->R, p->w.data(), hb.data(), hb.size());
+    const size_t bytes = v->st == 2 ? cs::build_bfrag_split2(prec, p->R, p->shape, p->w.data(), hb.data(), hb.size())
+                                    : cs::build_bfrag_split(prec, p->R, p->w.data(), hb.data(), hb.size());
     if (bytes == 0) return CS_ERR_UNSUPPORTED;
     if (p->bfrag) (void)hipFree(p->bfrag);
     p->bfrag = nullptr;

@@ -27,7 +27,8 @@ struct Variant {
   int R, shape, kf, by, bx, nw;
   size_t lds_bytes;
   int px;  // LDS row pitch in doubles
-  int st;  // fused-step staging: 0 = two LDS buffers, 1 = one buffer + registers
+  int st;  // fused-step staging: 0 = two LDS buffers, 1 = one buffer + registers;
+           // split-precision: 0 = Toeplitz form, 2 = dilated form
   LaunchFn launch;  // one launch = kf fused steps from `in` to `out`
   hipError_t (*prepare)();  // one-time function attributes (dynamic LDS above 64 KiB)
   const char *name;
@@ -47,6 +48,8 @@ int bfrag_ksteps_1d_f64(int R);
 void build_bfrag_3d_f64(int R, int shape, const double *w_dense, double *bfrag);
 // Split-precision B fragments (prec: 1 BF16x3, 2 FP32x2, 3 FP16x2); returns bytes written, 0 if cap is short.
 size_t build_bfrag_split(int prec, int R, const double *w_dense, void *out, size_t cap);
+// A fragments of the dilated split form (Variant::st == 2; prec 1 BF16x3 or 3 FP16x2).
+size_t build_bfrag_split2(int prec, int R, int shape, const double *w_dense, void *out, size_t cap);
 int bfrag_ksteps_3d_f64(int R, int shape);
 
 // Device-side synthetic input fill (same bits as the host generator).

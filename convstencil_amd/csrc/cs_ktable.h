@@ -331,3 +331,77 @@ constexpr DTable make_dtable(int R, int shape) {
 constexpr int dtable_ksteps(const DTable &t) { return 2 * t.np + t.ns; }
 
 }  // namespace cs
+
+namespace cs {
+
+// ---- chunk table of the split-precision 2-D kernels (v_mfma_f32_16x16x32, 16 x 16 blocks) ----
+//
+// D[m][n] = sum_k A[m][k] B[k][n] with the weights as A and the tile as B.  Output
+// (m, n) of a block at (y, x) is the cell (y, x) + u_n + v_m with anchors
+// u_n = (2 (n & 7), 8 (n >> 3)) and offsets v_m = (m >> 3, m & 7), a bijection onto the
+// 16 x 16 block.  K runs over the dilated footprint T = {d + v : d a tap, v in V},
+//   A[m][t] = W[t - v_m] (0 off the stencil),   B[t][n] = tile[(y, x) + u_n + t],
+// grouped into chunks of 8 consecutive columns t = (ty, tx0 .. tx0 + 7): lane l holds
+// B[k][n] for n = l & 15 and the 8 k of chunk l >> 4 of the instruction -- one 16-byte
+// LDS read -- and D[m][n] for m = 4 (l >> 4) + 0..3, i.e. 4 adjacent columns of one row.
+// Chunks start at columns tx0 = c0 (mod 8), c0 in {0, 4} (fewest chunks), so that both
+// the 16-byte reads and the 8-byte writes of the 4 adjacent results are aligned when
+// block columns satisfy x + CS + c0 = 0 (mod 8).  box2d1r: 8 chunks (2 MFMAs per slice
+// product per 256 outputs), box2d3r: 16 (4 MFMAs), star2d3r: 12 (3 MFMAs).
+struct CTable {
+  int n;             // chunks, a multiple of 4 (4 per MFMA)
+  int c0;            // chunk start residue mod 8
+  int ty[64], tx[64], live[64];
+  int tmin, tmax;    // rows read relative to a block row anchor 2 (n & 7)
+  int cmin, cmax;    // first and last chunk start column
+};
+
+constexpr CTable make_ctable(int R, int shape) {
+  CTable best{};
+  best.n = 1 << 20;
+  for (int c0 = 0; c0 <= 4; c0 += 4) {
+    CTable t{};
+    t.c0 = c0;
+    int n = 0;
+    for (int ty = -R; ty <= R + 1; ++ty) {
+      int lo = 1 << 20, hi = -(1 << 20);
+      for (int vy = 0; vy <= 1; ++vy) {
+        const int dy = ty - vy;
+        if (dy < -R || dy > R) continue;
+        int a = 0, b = -1;
+        row_span(R, shape, dy, a, b);
+        if (a > b) continue;
+        lo = a < lo ? a : lo;
+        hi = b + 7 > hi ? b + 7 : hi;
+      }
+      if (lo > hi) continue;
+      for (int a = lo - (((lo - c0) % 8) + 8) % 8; a <= hi; a += 8) {
+        t.ty[n] = ty;
+        t.tx[n] = a;
+        t.live[n] = 1;
+        ++n;
+      }
+    }
+    const int n4 = (n + 3) / 4 * 4;
+    for (int e = n; e < n4; ++e) {  // padding chunks repeat the first chunk with zero weight
+      t.ty[e] = t.ty[0];
+      t.tx[e] = t.tx[0];
+      t.live[e] = 0;
+    }
+    t.n = n4;
+    t.tmin = 1 << 20;
+    t.tmax = -(1 << 20);
+    t.cmin = 1 << 20;
+    t.cmax = -(1 << 20);
+    for (int e = 0; e < n4; ++e) {
+      t.tmin = t.ty[e] < t.tmin ? t.ty[e] : t.tmin;
+      t.tmax = t.ty[e] > t.tmax ? t.ty[e] : t.tmax;
+      t.cmin = t.tx[e] < t.cmin ? t.tx[e] : t.cmin;
+      t.cmax = t.tx[e] > t.cmax ? t.tx[e] : t.cmax;
+    }
+    if (t.n < best.n) best = t;
+  }
+  return best;
+}
+
+}  // namespace cs

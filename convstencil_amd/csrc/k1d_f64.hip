@@ -31,7 +31,8 @@ struct Cfg1D {
   static constexpr int E = L + 2 * KF * R;
   static constexpr int P = E + 1;  // padded buffer length
   static constexpr int NBUF = KF > 1 ? 2 : 1;
-  static constexpr size_t LDS = (size_t)NBUF * P * sizeof(double);
+  static constexpr int SPARE = NBUF * P;  // one scratch cell after the buffers
+  static constexpr size_t LDS = (size_t)(NBUF * P + 2) * sizeof(double);
   static_assert(L % 64 == 0, "tile must be a multiple of 64");
   static_assert(LDS <= 160 * 1024, "tile does not fit gfx950 LDS");
 };
@@ -60,12 +61,26 @@ __global__ __launch_bounds__(NW * 64) void k1d_f64(const double *__restrict__ in
     off[ks] = 16 * i + 4 * b4 + T::c(f);
   }
 
-  for (int idx = tid; idx < C::P; idx += NW * 64) {
-    const int64_t ax = ax0 + idx;
-    double v = 0.0;
-    if (idx < C::E && ax >= 0 && ax < ext) v = in[ax];
-    lds[idx] = v;
-    if constexpr (KF > 1) lds[C::P + idx] = v;
+  // all loads in flight before the first LDS store; a cell that is not read loads in[0]
+  // (always valid) and is replaced by 0, threads past the buffer store to the spare cell
+  {
+    constexpr int NT = NW * 64, NIT = (C::P + NT - 1) / NT;
+    double v[NIT];
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int idx = tid + i * NT;
+      const int64_t ax = ax0 + idx;
+      const bool ok = idx < C::E && ax >= 0 && ax < ext;
+      const double x = *(ok ? in + ax : in);
+      v[i] = ok ? x : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NIT; ++i) {
+      const int idx = tid + i * NT;
+      const bool in_buf = i < NIT - 1 || idx < C::P;
+      lds[in_buf ? idx : C::SPARE] = v[i];
+      if constexpr (KF > 1) lds[in_buf ? C::P + idx : C::SPARE] = v[i];
+    }
   }
   __syncthreads();
 

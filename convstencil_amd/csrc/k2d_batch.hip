@@ -55,25 +55,25 @@ __global__ void batch_reset(unsigned *sync, int n) {
 //   FP64:  0 box r1 kf4, 1 box r1 kf1, 2 star r3 kf2, 3 star r3 kf1
 //   split: 0 bf16x3 r1 kf4, 1 bf16x3 r1 kf1, 2 bf16x3 r3 kf2, 3 bf16x3 r3 kf1
 template <int PREC>
-__device__ __forceinline__ void run_item(const BatchGrid &G, const BatchItem &m, unsigned char *smem) {
+__device__ __forceinline__ void run_item(const BatchGrid &G, const BatchItem &m, unsigned char *smem, int tid) {
   const double *in = G.buf[m.src];
   double *out = G.buf[m.src ^ 1];
   double *lds = reinterpret_cast<double *>(smem);
   if constexpr (PREC == 0) {
     switch (m.kind) {
-      case 0: tile_f64<4, 1, SHAPE_BOX, 4, 64, 64, 8, 1>(in, out, G.bfrag, G.geo, m.tile, lds); break;
-      case 1: tile_f64<4, 1, SHAPE_BOX, 1, 64, 64, 8, 1>(in, out, G.bfrag, G.geo, m.tile, lds); break;
-      case 2: tile_f64<4, 3, SHAPE_STAR, 2, 64, 64, 8, 1>(in, out, G.bfrag, G.geo, m.tile, lds); break;
-      case 3: tile_f64<4, 3, SHAPE_STAR, 1, 64, 64, 8, 1>(in, out, G.bfrag, G.geo, m.tile, lds); break;
+      case 0: tile_f64<4, 1, SHAPE_BOX, 4, 64, 64, 8, 1, 64>(in, out, G.bfrag, G.geo, m.tile, lds, tid); break;
+      case 1: tile_f64<4, 1, SHAPE_BOX, 1, 64, 64, 8, 1, 64>(in, out, G.bfrag, G.geo, m.tile, lds, tid); break;
+      case 2: tile_f64<4, 3, SHAPE_STAR, 2, 64, 64, 8, 1, 64>(in, out, G.bfrag, G.geo, m.tile, lds, tid); break;
+      case 3: tile_f64<4, 3, SHAPE_STAR, 1, 64, 64, 8, 1, 64>(in, out, G.bfrag, G.geo, m.tile, lds, tid); break;
       default: break;
     }
   } else {
     const TrBF16::T *bf = reinterpret_cast<const TrBF16::T *>(G.bfrag);
     switch (m.kind) {
-      case 0: tile_split<TrBF16, 1, 4, 64, 64, 8>(in, out, bf, G.geo, m.tile, smem); break;
-      case 1: tile_split<TrBF16, 1, 1, 64, 64, 8>(in, out, bf, G.geo, m.tile, smem); break;
-      case 2: tile_split<TrBF16, 3, 2, 64, 64, 8>(in, out, bf, G.geo, m.tile, smem); break;
-      case 3: tile_split<TrBF16, 3, 1, 64, 64, 8>(in, out, bf, G.geo, m.tile, smem); break;
+      case 0: tile_split<TrBF16, 1, 4, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 1: tile_split<TrBF16, 1, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 2: tile_split<TrBF16, 3, 2, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 3: tile_split<TrBF16, 3, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
       default: break;
     }
   }
@@ -102,7 +102,12 @@ __global__ __launch_bounds__(512, 4) void k2d_batch(const BatchGrid *__restrict_
       m.tile = __builtin_amdgcn_readfirstlane(m.tile);
       m.kind = __builtin_amdgcn_readfirstlane(m.kind);
       m.src = __builtin_amdgcn_readfirstlane(m.src);
-      run_item<PREC>(grids[m.grid], m, smem);
+      // an opaque copy of the thread index: keeps the compiler from hoisting each tile
+      // body's per-lane set-up out of this loop, which would hold the set-up of every
+      // kind in registers at once
+      int tid;
+      asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"((int)threadIdx.x));
+      run_item<PREC>(grids[m.grid], m, smem, tid);
       __syncthreads();  // LDS is reused by the next item
     }
     grid_barrier(barrier, error, (unsigned)(b + 1) * nwg);

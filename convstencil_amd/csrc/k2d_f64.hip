@@ -10,7 +10,7 @@ __global__ __launch_bounds__(NW * 64) void k2d_f64(const double *__restrict__ in
   const int64_t ntiles = g.tiles[1] * g.tiles[2];
   const int64_t tile = xcd_tile(blockIdx.x, ntiles);
   if (tile >= ntiles) return;  // uniform over the workgroup
-  tile_f64<MF, R, SH, KF, BY, BX, NW, ST>(in, out, bfrag, g, tile, lds);
+  tile_f64<MF, R, SH, KF, BY, BX, NW, ST>(in, out, bfrag, g, tile, lds, threadIdx.x);
 }
 
 template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>

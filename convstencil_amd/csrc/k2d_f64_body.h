@@ -82,7 +82,8 @@ struct Cfg2D {
   static constexpr int PX = MF == 16 ? (PX0 | 1) : PX0 + ((4 - PX0 % 8) + 8) % 8;
   static constexpr int NBUF = (KF > 1 && ST == 0) ? 2 : 1;
   static constexpr int BUF = EY * PX;
-  static constexpr size_t LDS = (size_t)NBUF * BUF * sizeof(double);
+  static constexpr int SPARE = NBUF * BUF;  // one scratch cell after the buffers
+  static constexpr size_t LDS = (size_t)(NBUF * BUF + 2) * sizeof(double);
   static_assert(BY % 16 == 0 && BX % 16 == 0, "tile must be a multiple of 16");
   static_assert(min_col() + CS >= 0, "block reads left of the LDS row");
   static_assert(MF == 16 || (PX % 2 == 0 && (CS + PP + LOX(0)) % 2 == 0), "pair reads must be aligned");
@@ -113,7 +114,7 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 // Per-workgroup context of one tile.
 struct TileCtx {
-  int lane, wave;
+  int tid, lane, wave;
   int orow, ocol;  // 4x4x4: the output cell of this lane relative to the block
   bool full;       // the whole tile (with halo) lies inside the domain interior
   int64_t Y0, X0, ny, nx, ay0, ax0, pitch;
@@ -219,18 +220,58 @@ __device__ __forceinline__ void fused_steps(const TileCtx &t, const double *b, c
   }
 }
 
-// One workgroup advances tile `tile` of grid g by KF steps (lds: the kernel's dynamic LDS).
-template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+// Copies the tile with its halo (EY x EX cells from (ay0, ax0)) into LDS, zero for cells
+// outside the array and for pad columns.  Each thread issues its loads in groups of LG
+// (all of them by default) before the group's first LDS store, so a whole group of the
+// tile is in flight at once; a cell that is not read loads the array's first element
+// (always valid) and is replaced by 0.
+template <class C, int LG>
+__device__ __forceinline__ void load_tile(const double *__restrict__ in, const Geo &g, const TileCtx &t,
+                                          double *__restrict__ lds) {
+  constexpr int NT = C::NW * 64;
+  constexpr int NIT = (C::BUF + NT - 1) / NT;
+  constexpr int G = LG < NIT ? LG : NIT;
+  const int tid = t.tid;
+  const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
+  const double *base = in + t.ay0 * t.pitch + t.ax0;
+#pragma unroll
+  for (int i0 = 0; i0 < NIT; i0 += G) {
+    double v[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const int idx = tid + (i0 + j) * NT;
+      const int r = idx / C::PX, c = idx - r * C::PX - C::CS;  // storage column idx - r*PX holds tile column c
+      const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
+      const bool ok = i0 + j < NIT && idx < C::BUF && c >= 0 && c < C::EX &&
+                      (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch));
+      const double x = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
+      v[j] = ok ? x : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const int idx = tid + (i0 + j) * NT;
+      // threads past the buffer store to the spare cell after the buffers (unconditional
+      // stores keep every load of the group ahead of the first wait)
+      const bool in_buf = i0 + j < NIT - 1 || (i0 + j == NIT - 1 && idx < C::BUF);
+      lds[in_buf ? idx : C::SPARE] = v[j];
+      if constexpr (C::NBUF == 2) lds[in_buf ? C::BUF + idx : C::SPARE] = v[j];
+    }
+  }
+}
+
+// One workgroup advances tile `tile` of grid g by KF steps (lds: the kernel's dynamic LDS;
+// LG: tile loads in flight per thread; tid: threadIdx.x).
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST, int LG = 64>
 __device__ __forceinline__ void tile_f64(const double *__restrict__ in, double *__restrict__ out,
                                          const double *__restrict__ bfrag, const Geo &g, int64_t tile,
-                                         double *__restrict__ lds) {
+                                         double *__restrict__ lds, int tid) {
   using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
   constexpr int NKS = C::NKS;
   constexpr int PX = C::PX;
   constexpr int CS = C::CS;
 
-  const int tid = threadIdx.x;
   TileCtx t;
+  t.tid = tid;
   t.lane = tid & 63;
   t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   t.pitch = g.ext[2];
@@ -268,16 +309,7 @@ __device__ __forceinline__ void tile_f64(const double *__restrict__ in, double *
   }
 
   // ---- load the tile (and its halo); cells outside the array and pad columns are 0 ----
-  const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
-  for (int idx = tid; idx < C::BUF; idx += NW * 64) {
-    const int r = idx / PX, c = idx - r * PX - CS;  // storage column idx - r*PX holds tile column c
-    const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
-    double v = 0.0;
-    if (c >= 0 && c < C::EX && (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch)))
-      v = in[ay * t.pitch + ax];
-    lds[idx] = v;
-    if constexpr (C::NBUF == 2) lds[C::BUF + idx] = v;
-  }
+  load_tile<C, LG>(in, g, t, lds);
   __syncthreads();
 
   fused_steps<C, 0>(t, b, off, lds, out);

@@ -1,5 +1,7 @@
-// k2d_split.hip -- split-precision 2-D ConvStencil kernels (tile body: k2d_split_body.h).
-#include "k2d_split_body.h"
+// k2d_split.hip -- split-precision 2-D ConvStencil kernels: the dilated form
+// (k2d_split2_body.h, BF16x3 and FP16x2, preferred) and the Toeplitz form
+// (k2d_split_body.h, also FP32x2).
+#include "k2d_split2_body.h"
 
 namespace cs {
 
@@ -11,7 +13,35 @@ __global__ __launch_bounds__(NW * 64) void k2d_split(const double *__restrict__ 
   const int64_t per = (ntiles + 7) / 8;
   const int64_t tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= ntiles) return;
-  tile_split<Tr, R, KF, BY, BX, NW>(in, out, bfrag, g, tile, smem);
+  tile_split<Tr, R, KF, BY, BX, NW>(in, out, bfrag, g, tile, smem, threadIdx.x);
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+__global__ __launch_bounds__(NW * 64) void k2d_split2(const double *__restrict__ in, double *__restrict__ out,
+                                                      const typename Tr::T *__restrict__ bfrag, Geo g) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t ntiles = g.tiles[1] * g.tiles[2];
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= ntiles) return;
+  tile_split2<Tr, R, SH, KF, BY, BX, NW>(in, out, bfrag, g, tile, smem, threadIdx.x);
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+static hipError_t prepare_ks2() {
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  return hipFuncSetAttribute((const void *)k2d_split2<Tr, R, SH, KF, BY, BX, NW>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+static hipError_t launch_ks2(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s) {
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  const int64_t ntiles = g.tiles[1] * g.tiles[2];
+  const int64_t grid = ((ntiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((k2d_split2<Tr, R, SH, KF, BY, BX, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in,
+                     out, reinterpret_cast<const typename Tr::T *>(bfrag), g);
+  return hipGetLastError();
 }
 
 template <class Tr, int R, int KF, int BY, int BX, int NW>
@@ -39,7 +69,43 @@ static hipError_t launch_ks(const double *in, double *out, const double *bfrag, 
         "k2d_split<" #TR ",r" #R ",kf" #KF "," #BY "x" #BX ",w" #NW ">"                                            \
   }
 
+// Dilated form: Variant::st = 2.  Ids 60000 + R * 1000 + precision * 100 + (star) * 10 + KF.
+#define CS_VS2(ID, PREC, TR, R, SH, KF, BY, BX, NW)                                                     \
+  Variant {                                                                                             \
+    ID, PREC, R, SH, KF, BY, BX, NW, CfgS2<TR, R, SH, KF, BY, BX, NW>::LDS,                             \
+        CfgS2<TR, R, SH, KF, BY, BX, NW>::PX, 2, &launch_ks2<TR, R, SH, KF, BY, BX, NW>,               \
+        &prepare_ks2<TR, R, SH, KF, BY, BX, NW>,                                                        \
+        "k2d_split2<" #TR ",r" #R "," #SH ",kf" #KF "," #BY "x" #BX ",w" #NW ">",                       \
+        CfgS2<TR, R, SH, KF, BY, BX, NW>::CS                                                            \
+  }
+
 static const Variant kVariantsSplit[] = {
+    CS_VS2(61101, 1, TrBF16, 1, SHAPE_BOX, 1, 64, 64, 8),
+    CS_VS2(61102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2(61103, 1, TrBF16, 1, SHAPE_BOX, 3, 64, 64, 8),
+    CS_VS2(61104, 1, TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(61111, 1, TrBF16, 1, SHAPE_STAR, 1, 64, 64, 8),
+    CS_VS2(61112, 1, TrBF16, 1, SHAPE_STAR, 2, 64, 64, 8),
+    CS_VS2(61114, 1, TrBF16, 1, SHAPE_STAR, 4, 64, 64, 8),
+    CS_VS2(63101, 1, TrBF16, 3, SHAPE_BOX, 1, 64, 64, 8),
+    CS_VS2(63102, 1, TrBF16, 3, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2(63103, 1, TrBF16, 3, SHAPE_BOX, 3, 64, 64, 8),
+    CS_VS2(63104, 1, TrBF16, 3, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63111, 1, TrBF16, 3, SHAPE_STAR, 1, 64, 64, 8),
+    CS_VS2(63112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8),
+    CS_VS2(63113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8),
+    CS_VS2(63114, 1, TrBF16, 3, SHAPE_STAR, 4, 64, 64, 8),
+    CS_VS2(61301, 3, TrF16, 1, SHAPE_BOX, 1, 64, 64, 8),
+    CS_VS2(61302, 3, TrF16, 1, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2(61304, 3, TrF16, 1, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63301, 3, TrF16, 3, SHAPE_BOX, 1, 64, 64, 8),
+    CS_VS2(63302, 3, TrF16, 3, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2(63303, 3, TrF16, 3, SHAPE_BOX, 3, 64, 64, 8),
+    CS_VS2(63304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63311, 3, TrF16, 3, SHAPE_STAR, 1, 64, 64, 8),
+    CS_VS2(63312, 3, TrF16, 3, SHAPE_STAR, 2, 64, 64, 8),
+    // Toeplitz form
+
     CS_VS(51101, 1, TrBF16, 1, 1, 64, 64, 8), CS_VS(51102, 1, TrBF16, 1, 2, 64, 64, 8),
     CS_VS(51104, 1, TrBF16, 1, 4, 64, 64, 8), CS_VS(53101, 1, TrBF16, 3, 1, 64, 64, 8),
     CS_VS(53102, 1, TrBF16, 3, 2, 64, 64, 8), CS_VS(53301, 3, TrF16, 3, 1, 64, 64, 8),
@@ -84,6 +150,39 @@ static size_t build_split(int R, const double *w, unsigned char *out, size_t cap
           o[((size_t)(t * nins + m) * 64 + lane) * Tr::CK + e] = sl[t];
         }
   return bytes;
+}
+
+// A fragments of the dilated form: weight slice q, instruction i, lane l, element j is
+// W[t - v_m] with m = l & 15, chunk 4 i + (l >> 4) and t = (ty, tx0 + j).
+template <class Tr>
+static size_t build_split2(int R, int shape, const double *w, unsigned char *out, size_t cap) {
+  using T = typename Tr::T;
+  const CTable ct = make_ctable(R, shape);
+  const int nins = ct.n / 4;
+  const size_t bytes = (size_t)Tr::NSW * nins * 64 * 8 * sizeof(T);
+  if (bytes > cap) return 0;
+  T *o = reinterpret_cast<T *>(out);
+  const int D = 2 * R + 1;
+  for (int q = 0; q < Tr::NSW; ++q)
+    for (int i = 0; i < nins; ++i)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int j = 0; j < 8; ++j) {
+          const int m = lane & 15, c = 4 * i + (lane >> 4);
+          const int dy = ct.ty[c] - (m >> 3), dx = ct.tx[c] + j - (m & 7);
+          double v = 0.0;
+          if (ct.live[c] && dy >= -R && dy <= R && dx >= -R && dx <= R) v = w[(dy + R) * D + (dx + R)];
+          T sl[Tr::NSU];
+          split_value<Tr>(v, sl);
+          o[((size_t)(q * nins + i) * 64 + lane) * 8 + j] = sl[q];
+        }
+  return bytes;
+}
+
+size_t build_bfrag_split2(int prec, int R, int shape, const double *w, void *out, size_t cap) {
+  unsigned char *o = reinterpret_cast<unsigned char *>(out);
+  if (prec == 1) return build_split2<TrBF16>(R, shape, w, o, cap);
+  if (prec == 3) return build_split2<TrF16>(R, shape, w, o, cap);
+  return 0;
 }
 
 size_t build_bfrag_split(int prec, int R, const double *w, void *out, size_t cap) {

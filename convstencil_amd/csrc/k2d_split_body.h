@@ -79,7 +79,8 @@ struct CfgS {
   static constexpr int PLANE = EX * PR;                     // elements per slice plane
   static constexpr int NSU = Tr::NSU;
   static constexpr int BUF = NSU * PLANE;                   // one buffer: all slices
-  static constexpr size_t LDS = 2 * (size_t)BUF * sizeof(T);
+  static constexpr int SPARE = 2 * BUF;                     // scratch cells after the buffers
+  static constexpr size_t LDS = (2 * (size_t)BUF + 8) * sizeof(T);
   static constexpr int NENT = (2 * R + 1) * RPD;            // K entries (dx, rr)
   static constexpr int NINS = (NENT + Tr::KI - 1) / Tr::KI; // MFMA instructions per product
   static_assert(BY % 16 == 0 && BX % 16 == 0, "tile must be a multiple of 16");
@@ -91,16 +92,16 @@ struct CfgS {
 };
 
 // One workgroup advances tile `tile` of grid g by KF steps (smem: the kernel's dynamic LDS).
-template <class Tr, int R, int KF, int BY, int BX, int NW>
+template <class Tr, int R, int KF, int BY, int BX, int NW, int LG = 64>
 __device__ __forceinline__ void tile_split(const double *__restrict__ in, double *__restrict__ out,
                                            const typename Tr::T *__restrict__ bfrag, const Geo &g, int64_t tile,
-                                           unsigned char *__restrict__ smem) {
+                                           unsigned char *__restrict__ smem, int tid) {
   using C = CfgS<Tr, R, KF, BY, BX, NW>;
   using T = typename Tr::T;
   using V = typename Tr::V;
   constexpr int NINS = C::NINS, NSU = Tr::NSU, NSW = Tr::NSW, NL = Tr::NL, CK = Tr::CK;
   T *lds = reinterpret_cast<T *>(smem);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
   const int64_t pitch = g.ext[2], ny = g.n[1], nx = g.n[2];
   const int64_t Y0 = (tile / g.tiles[2]) * BY - KF * R;
   const int64_t X0 = (tile % g.tiles[2]) * BX - KF * R;
@@ -130,21 +131,42 @@ __device__ __forceinline__ void tile_split(const double *__restrict__ in, double
   }
 
   // ---- load: FP64 tile -> slices in both buffers (halo cells stay as loaded) ----
+  // Consecutive threads walk a row (coalesced loads); each thread issues its loads in
+  // groups of LG before the group's first LDS store.  A cell that is not read loads the
+  // array's first element (always valid) and is replaced by 0; threads past the plane
+  // store to the spare cells.
   const bool all_in = ay0 >= 0 && ay0 + C::EY <= g.ext[1] && ax0 >= 0 && ax0 + C::EX <= pitch;
-  for (int idx = tid; idx < C::PLANE; idx += NW * 64) {
-    const int c = idx / C::PR, sr = idx - c * C::PR;  // column-major: consecutive threads walk rows
-    const int r = sr - C::TOP;
-    double v = 0.0;
-    if (r >= 0 && r < C::EY) {
-      const int64_t ay = ay0 + r, ax = ax0 + c;
-      if (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < pitch)) v = in[ay * pitch + ax];
-    }
-    T s[NSU];
-    split_value<Tr>(v, s);
+  {
+    constexpr int NT = NW * 64, NIT = (C::PLANE + NT - 1) / NT;
+    constexpr int G = LG < NIT ? LG : NIT;
 #pragma unroll
-    for (int k = 0; k < NSU; ++k) {
-      lds[k * C::PLANE + idx] = s[k];
-      lds[C::BUF + k * C::PLANE + idx] = s[k];
+    for (int i0 = 0; i0 < NIT; i0 += G) {
+      double v[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int idx = tid + (i0 + j) * NT;
+        const int sr = idx / C::EX, c = idx - sr * C::EX;  // storage row sr holds tile row sr - TOP
+        const int r = sr - C::TOP;
+        const int64_t ay = ay0 + r, ax = ax0 + c;
+        const bool ok = i0 + j < NIT && idx < C::PLANE && r >= 0 && r < C::EY &&
+                        (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < pitch));
+        const double x = *(ok ? in + (ay * pitch + ax) : in);
+        v[j] = ok ? x : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int idx = tid + (i0 + j) * NT;
+        const int sr = idx / C::EX, c = idx - sr * C::EX;
+        const bool in_plane = i0 + j < NIT - 1 || (i0 + j == NIT - 1 && idx < C::PLANE);
+        const int li = c * C::PR + sr;  // column-major slice planes
+        T s[NSU];
+        split_value<Tr>(v[j], s);
+#pragma unroll
+        for (int k = 0; k < NSU; ++k) {
+          lds[in_plane ? k * C::PLANE + li : C::SPARE] = s[k];
+          lds[in_plane ? C::BUF + k * C::PLANE + li : C::SPARE] = s[k];
+        }
+      }
     }
   }
   __syncthreads();

@@ -89,6 +89,13 @@ class Layout:
         return v[self.org[0]:self.org[0] + self.n[0], self.org[1]:self.org[1] + self.n[1],
                  self.org[2]:self.org[2] + self.n[2]]
 
+    def interior_mask(self):
+        """Flat boolean mask of the interior cells of a padded array."""
+        m = np.zeros(self.ext, dtype=bool)
+        m[self.org[0]:self.org[0] + self.n[0], self.org[1]:self.org[1] + self.n[1],
+          self.org[2]:self.org[2] + self.n[2]] = True
+        return m.reshape(-1)
+
     def to_c(self):
         from . import _lib
         L = _lib.cs_layout()

@@ -55,6 +55,16 @@ void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49);
 /* Split-precision B fragments as raw slice elements (bf16 / fp32 / fp16 by precision), layout
  * [weight slice][instruction][lane][element]; returns the byte count, or 0 if cap is too small. */
 int64_t cs_debug_bfrag_split(int32_t precision, int32_t radius, const double *w, void *out, int64_t cap);
+/* Dilated split form (variants with staging 2): A fragments of the weights' slices, layout
+ * [weight slice][instruction][lane][8 elements], for shape class 0 box / 1 star; returns the byte
+ * count, or 0 if cap is too small or the precision has no dilated form (FP32x2). */
+int64_t cs_debug_bfrag_split2(int32_t precision, int32_t radius, int32_t shape, const double *w, void *out,
+                              int64_t cap);
+/* Chunk table of the dilated split form: *c0 = chunk start residue (mod 8); chunk e reads footprint
+ * row ty[e], columns tx[e] .. tx[e] + 7 (live[e] = 0: zero-weight padding).  Returns the chunk count
+ * (4 per MFMA), or -1 if cap is too small. */
+int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty, int32_t *tx, int32_t *live,
+                        int32_t cap);
 
 /* Times `reps` plain device copies of `count` doubles (needs a GPU); the kernel is the
  * reference for HBM byte counters.  Returns 0 on success. */

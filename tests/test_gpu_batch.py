@@ -1,0 +1,105 @@
+"""Batched persistent kernel (cs_batch_*) on the GPU, every grid against the CPU oracle.
+
+One persistent launch advances many independent 2-D grids of uneven sizes and
+mixed stencils (box2d1r, star2d3r); each grid must match what the single-grid
+path and the oracle give for it alone: FP64 to 1e-12 relative (max), BF16x3 to
+the split-precision bar (relative L2 <= 1e-6, relative max <= 5e-6).
+"""
+import numpy as np
+import pytest
+
+from convstencil_amd import Batch, Layout, shape_offsets, dense_weights, SHAPES
+from oracle import oracle
+from refimpl import rel_errors
+
+pytestmark = pytest.mark.gpu
+
+# (stencil, interior sizes): partial tiles, a grid smaller than one tile, one exact tile
+GRIDS = [("box2d1r", [64, 64]), ("star2d3r", [100, 130]), ("box2d1r", [257, 75]), ("star2d3r", [31, 200]),
+         ("box2d1r", [3, 5]), ("star2d3r", [150, 64]), ("box2d1r", [130, 190])]
+
+
+def _grid(name, sizes, seed, random_halo=False):
+    ndim, r, kind = SHAPES[name]
+    L = Layout.with_halo(sizes, r)
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), seed)
+    x = oracle.fill_data(L, seed)
+    if random_halo:
+        rng = np.random.default_rng(seed)
+        m = np.ones(L.ext, dtype=bool)
+        m[0, r:r + sizes[0], r:r + sizes[1]] = False
+        x = x.reshape(L.ext)
+        x[m] = rng.uniform(-1, 1, int(m.sum()))
+        x = x.reshape(-1)
+    return L, r, offs, w, dense_weights(2, r, offs, w), x
+
+
+def _check(batch, grids, steps, prec):
+    for i, (L, r, offs, w, dense, x) in enumerate(grids):
+        got = batch.download(i)
+        want = oracle.run(L, offs, w, x, steps)
+        rm, rl2 = rel_errors(L.interior(got), L.interior(want))
+        if prec == "fp64":
+            assert rm <= 1e-12, (i, rm)
+        else:
+            assert rl2 <= 1e-6 and rm <= 5e-6, (i, rm, rl2)
+        # halo cells are those of the buffer the state ends in, exactly
+        assert np.array_equal(got[~L.interior_mask()], want[~L.interior_mask()]), i
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("steps", [1, 7, 12])
+def test_batch_matches_oracle(gpu, prec, steps):
+    print(f"case batch {prec} T={steps}", flush=True)
+    grids = [_grid(n, s, 100 + k) for k, (n, s) in enumerate(GRIDS)]
+    with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids], precision=prec) as b:
+        for i, g in enumerate(grids):
+            b.upload(i, g[5])
+        b.run(steps)
+        _check(b, grids, steps, prec)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_consecutive_runs(gpu, graph):
+    # the state carries over between runs (buffer parity per grid); 5 + 3 + 5 steps
+    print(f"case batch consecutive graph={graph}", flush=True)
+    grids = [_grid(n, s, 200 + k) for k, (n, s) in enumerate(GRIDS)]
+    with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids], use_graph=graph) as b:
+        for i, g in enumerate(grids):
+            b.upload(i, g[5])
+        b.run(5)
+        b.run(3)
+        b.run(5)
+        _check(b, grids, 13, "fp64")
+
+
+def test_batch_nonzero_halo(gpu):
+    # a grid with a non-zero halo runs single steps (the halo alternates with the zero
+    # halo of the second buffer); the other grids still fuse
+    print("case batch nonzero halo", flush=True)
+    grids = [_grid(n, s, 300 + k, random_halo=(k == 1)) for k, (n, s) in enumerate(GRIDS[:4])]
+    with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids]) as b:
+        for i, g in enumerate(grids):
+            b.upload(i, g[5])
+        b.run(6)
+        _check(b, grids, 6, "fp64")
+
+
+def test_batch_device_fill_and_time(gpu):
+    # device-side fill equals the host generator; timing runs and reports
+    print("case batch fill/time", flush=True)
+    grids = [_grid(n, s, 400 + k) for k, (n, s) in enumerate(GRIDS[:3])]
+    with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids], use_graph=True) as b:
+        b.fill_random([7, 8, 9])
+        for i, g in enumerate(grids):
+            assert np.array_equal(b.download(i), oracle.fill_data(g[0], [7, 8, 9][i]))
+        rep = b.time(8, warmup=1, reps=2)
+        assert rep.elapsed_ms > 0 and rep.gstencil_per_s > 0
+        assert "grids=3" in b.describe()
+
+
+def test_batch_rejects_unsupported(gpu):
+    L, r, offs, w, dense, x = _grid("box2d2r", [40, 40], 1)
+    with pytest.raises(RuntimeError):
+        Batch([L], [r], [dense])

@@ -163,7 +163,7 @@ def emulate(v, layout, dense, x):
     EY, EX = by + 2 * kf * R, bx + 2 * kf * R
     BUF = EY * px
     nbuf = 2 if (kf > 1 and v["st"] == 0) else 1
-    assert v["lds"] == nbuf * BUF * 8
+    assert v["lds"] == (nbuf * BUF + 2) * 8  # two spare cells after the buffers
     assert v["lds"] <= 160 * 1024
     ext, org, n = layout.ext, layout.org, layout.n
     pitch = ext[2]
@@ -340,7 +340,7 @@ def emulate_1d(v, layout, w, x):
     E = L + 2 * kf * R
     assert P == E + 1
     nbuf = 2 if kf > 1 else 1
-    assert v["lds"] == nbuf * P * 8 and v["lds"] <= 160 * 1024
+    assert v["lds"] == (nbuf * P + 2) * 8 and v["lds"] <= 160 * 1024
     ext, org, n = layout.ext[2], layout.org[2], layout.n[2]
     out = np.zeros(layout.cells)
     for tile in range(-(-n // L)):
@@ -588,7 +588,7 @@ def emulate_split(v, layout, dense, x):
     PR = -(-(TOP + EY + RPD - 2 * R - 1) // 8) * 8
     assert v["px"] == PR
     PLANE = EX * PR
-    assert v["lds"] == 2 * NSU * PLANE * esz and v["lds"] <= 160 * 1024
+    assert v["lds"] == (2 * NSU * PLANE + 8) * esz and v["lds"] <= 160 * 1024
     lanes = np.arange(64)
     ai, q = lanes & 15, lanes >> 4
 
@@ -674,8 +674,10 @@ def emulate_split(v, layout, dense, x):
 VARIANTS_SPLIT = variants(4)
 
 
-@pytest.mark.parametrize("v", [v for v in VARIANTS_SPLIT if v["kf"] <= 2],
-                         ids=[v["name"] for v in VARIANTS_SPLIT if v["kf"] <= 2])
+VARIANTS_SPLIT1 = [v for v in VARIANTS_SPLIT if v["st"] == 0 and v["kf"] <= 2]
+
+
+@pytest.mark.parametrize("v", VARIANTS_SPLIT1, ids=[v["name"] for v in VARIANTS_SPLIT1])
 def test_variant_emulation_split(v):
     R = v["R"]
     dense = _dense(R, 0, v["id"])
@@ -687,3 +689,174 @@ def test_variant_emulation_split(v):
     gi, wi = layout.interior(got), layout.interior(want)
     err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
     assert err < 2e-6, (v["name"], err)
+
+
+# ------------------------------------------- 2-D split precision, dilated form (staging 2)
+
+def ctable(R, shape):
+    lib = _dbg()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.cs_debug_ctable.restype = ctypes.c_int32
+    lib.cs_debug_ctable.argtypes = [ctypes.c_int32, ctypes.c_int32, ip, ip, ip, ip, ctypes.c_int32]
+    cap = 128
+    c0 = ctypes.c_int32()
+    ty, tx, live = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+    n = lib.cs_debug_ctable(R, shape, ctypes.byref(c0), ty, tx, live, cap)
+    assert n > 0 and n % 4 == 0
+    return c0.value, np.array(ty[:n]), np.array(tx[:n]), np.array(live[:n])
+
+
+def _f32_to(x, kind):
+    """FP32 -> slice type, round to nearest even (as the kernel's conversions), back as FP32."""
+    return _round(np.asarray(x, dtype=np.float32), kind)
+
+
+def split_f32_np(v32, kind, ns, scale):
+    """The kernel's FP32 re-split (split_f32)."""
+    v32 = np.asarray(v32, dtype=np.float32)
+    if ns == 3:
+        h = _f32_to(v32, kind)
+        r1 = (v32 - h).astype(np.float32)
+        m = _f32_to(r1, kind)
+        lo = _f32_to((r1 - m).astype(np.float32), kind)
+        return [h, m, lo]
+    h = _f32_to(v32, kind)
+    return [h, _f32_to(((v32 - h) * np.float32(scale)).astype(np.float32), kind)]
+
+
+def emulate_split2(v, layout, dense, x):
+    prec, R, SH, KF, BY, BX = v["mf"], v["R"], v["shape"], v["kf"], v["by"], v["bx"]
+    kind, CK, KI, NSU, NSW, NL, SCALE = SPLIT[prec]
+    assert kind in ("bf16", "f16")
+    c0, cty, ctx, clive = ctable(R, SH)
+    NINS = len(cty) // 4
+    lib = _dbg()
+    lib.cs_debug_bfrag_split2.restype = ctypes.c_int64
+    lib.cs_debug_bfrag_split2.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _lib.c_double_p,
+                                          ctypes.c_void_p, ctypes.c_int64]
+    raw = np.zeros(1 << 20, dtype=np.uint8)
+    wd = np.ascontiguousarray(dense, dtype=np.float64)
+    nbytes = lib.cs_debug_bfrag_split2(prec, R, SH, wd.ctypes.data_as(_lib.c_double_p), raw.ctypes.data, raw.size)
+    assert nbytes == NSW * NINS * 64 * 8 * 2
+    wf = _to_f32(raw[:nbytes].copy().view(np.uint16), kind).reshape(NSW, NINS, 64, 8)
+    EY, EX, PX, CS = BY + 2 * KF * R, BX + 2 * KF * R, v["px"], v["cs"]
+    assert PX % 16 == 8
+    PLANE = EY * PX
+    assert v["lds"] == (NSU * PLANE + 8) * 2 and v["lds"] <= 160 * 1024
+    lanes = np.arange(64)
+    grp, nn = lanes >> 4, lanes & 15
+    # A[m][k] from lane l: m = l & 15, k = 8 (l >> 4) + j;  B[k][n]: n = l & 15, k = 8 (l >> 4) + j
+    A = np.zeros((NSW, NINS, 16, 32), dtype=np.float64)
+    for q in range(NSW):
+        for i in range(NINS):
+            A[q, i][nn[:, None], 8 * grp[:, None] + np.arange(8)[None, :]] = wf[q, i]
+    off = np.array([(2 * (nn & 7) + cty[4 * i + grp]) * PX + 8 * (nn >> 3) + ctx[4 * i + grp] + CS
+                    for i in range(NINS)])
+    # D lane l holds m = 4 (l >> 4) + v, n = l & 15: cell (2 (n & 7) + (m >> 3), 8 (n >> 3) + (m & 7))
+    orow = 2 * (nn & 7) + (grp >> 1)
+    ocol = 8 * (nn >> 3) + 4 * (grp & 1)
+    INV = np.float32(1.0 / SCALE)
+    ext, org, n = layout.ext, layout.org, layout.n
+    pitch = ext[2]
+    out = np.zeros(layout.cells)
+    x3 = x.reshape(ext)
+    ty_n, tx_n = -(-n[1] // BY), -(-n[2] // BX)
+    for tile in range(ty_n * tx_n):
+        Y0, X0 = (tile // tx_n) * BY - KF * R, (tile % tx_n) * BX - KF * R
+        ay0, ax0 = org[1] + Y0, org[2] + X0
+        full = Y0 >= 0 and X0 >= 0 and Y0 + EY <= n[1] and X0 + EX <= n[2]
+        vals = np.zeros(PLANE)
+        for idx in range(PLANE):
+            r, c = divmod(idx, PX)
+            c -= CS
+            ay, ax = ay0 + r, ax0 + c
+            if 0 <= c < EX and 0 <= ay < ext[1] and 0 <= ax < pitch:
+                vals[idx] = x3[0, ay, ax]
+        lds = [s_.astype(np.float32) for s_ in split_np(vals, kind, NSU, SCALE)]
+
+        def inside(ty, tx):
+            return full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
+        for st in range(KF):
+            last = st == KF - 1
+            H, W = BY + 2 * R * (KF - 1 - st), BX + 2 * R * (KF - 1 - st)
+            LO = R * (st + 1)
+            HI = LO + W
+            LOX = LO - ((LO + CS + c0) % 8)
+            nby, nbx = -(-H // 16), -(-(HI - LOX) // 16)
+            writes = []
+            for blk in range(nby * nbx):
+                ba, bb = divmod(blk, nbx)
+                y, xx = LO + min(16 * ba, H - 16), LOX + 16 * bb
+                d0 = np.zeros((16, 16), dtype=np.float32)
+                d1 = np.zeros((16, 16), dtype=np.float32)
+                for i in range(NINS):
+                    base = y * PX + xx + off[i]
+                    assert np.all(base % 8 == 0), (v["name"], "16-byte LDS read misaligned")
+                    assert base.min() >= 0 and base.max() + 8 <= PLANE
+                    assert np.all(base % PX + 8 <= PX), (v["name"], "read wraps into the next row")
+                    B = []
+                    for s in range(NSU):
+                        g = lds[s][base[:, None] + np.arange(8)[None, :]]  # [lane][j]
+                        Bs = np.zeros((32, 16), dtype=np.float64)
+                        Bs[8 * grp[:, None] + np.arange(8)[None, :], nn[:, None]] = g
+                        B.append(Bs)
+                    d0 = (d0 + (A[0, i] @ B[0]).astype(np.float32)).astype(np.float32)
+                    for s in range(NSU):
+                        for q in range(NSW):
+                            if 1 <= s + q < NL:
+                                d1 = (d1 + (A[q, i] @ B[s]).astype(np.float32)).astype(np.float32)
+                for ln in range(64):
+                    ty = y + orow[ln]
+                    for vv in range(4):
+                        m = 4 * grp[ln] + vv
+                        tx = xx + ocol[ln] + vv
+                        a0, a1 = d0[m, nn[ln]], d1[m, nn[ln]]
+                        if last:
+                            if LO <= tx < HI and inside(ty, tx):
+                                gi = (ay0 + ty) * pitch + ax0 + tx
+                                assert org[1] <= ay0 + ty < org[1] + n[1] and org[2] <= ax0 + tx < org[2] + n[2]
+                                out[gi] = float(a0) + float(a1) * float(INV)
+                        else:
+                            all4 = full or (inside(ty, xx + ocol[ln]) and inside(ty, xx + ocol[ln] + 3))
+                            if all4 or inside(ty, tx):
+                                li = ty * PX + tx + CS
+                                assert 0 <= li < PLANE and 0 <= tx + CS < PX
+                                writes.append((li, np.float32(a0 + a1 * INV)))
+            for li, val in writes:
+                for k, s_ in enumerate(split_f32_np(np.array([val], dtype=np.float32), kind, NSU, SCALE)):
+                    lds[k][li] = s_[0]
+    return out
+
+
+VARIANTS_SPLIT2 = [v for v in VARIANTS_SPLIT if v["st"] == 2 and v["kf"] <= 2]
+
+
+@pytest.mark.parametrize("v", VARIANTS_SPLIT2, ids=[v["name"] for v in VARIANTS_SPLIT2])
+def test_variant_emulation_split2(v):
+    R = v["R"]
+    dense = _dense(R, v["shape"], v["id"])
+    layout = Layout(2, [v["by"] // 2 + 2 * R + 3, v["bx"] + 2 * R + 9], [R + 1, R + 2], [v["by"] // 2, v["bx"] + 7])
+    x = oracle.fill_data(layout, v["id"])
+    got = emulate_split2(v, layout, dense, x)
+    offs, w = oracle.dense_to_taps(2, R, dense)
+    want = oracle.run(layout, offs, w, x, v["kf"])
+    gi, wi = layout.interior(got), layout.interior(want)
+    err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+    assert err < 2e-6, (v["name"], err)
+    # halo cells of the output are never written
+    assert np.all(got[~layout.interior_mask()] == 0.0)
+
+
+def test_ctables_cover_every_tap():
+    # every (tap, output offset v) pair lands in exactly one live chunk slot: slot t of a
+    # chunk serves output offset v_m = (m >> 3, m & 7) through tap t - v_m
+    for R in (1, 2, 3):
+        for shape in (0, 1):
+            c0, ty, tx, live = ctable(R, shape)
+            assert c0 in (0, 4) and np.all((tx - c0) % 8 == 0)
+            slots = [(ty[e], tx[e] + j) for e in range(len(ty)) if live[e] for j in range(8)]
+            assert len(slots) == len(set(slots))
+            taps = {(a, b) for a in range(-R, R + 1) for b in range(-R, R + 1) if shape == 0 or a == 0 or b == 0}
+            for (tdy, tdx) in taps:
+                for m in range(16):
+                    assert (tdy + (m >> 3), tdx + (m & 7)) in slots, (R, shape, tdy, tdx, m)
