@@ -53,13 +53,15 @@ def load_traffic(kernel_key):
 
 def cpu_baseline(layout, offs, w, seed, budget_s):
     """Times the oracle (OpenMP over all host cores) on the full grid for as many
-    whole time steps as fit in about budget_s seconds (at least one)."""
+    whole time steps as fit in about budget_s seconds (at least one).  Two untimed
+    steps first (page faults of the fresh arrays), then a 2-step calibration."""
     from oracle import oracle
     threads = oracle.max_threads()
     x = oracle.fill_data(layout, seed)
+    oracle.run(layout, offs, w, x, 2, threads=threads)
     t0 = time.perf_counter()
-    oracle.run(layout, offs, w, x, 1, threads=threads)
-    one = time.perf_counter() - t0
+    oracle.run(layout, offs, w, x, 2, threads=threads)
+    one = (time.perf_counter() - t0) / 2
     steps = max(1, int(budget_s / max(one, 1e-6)))
     t0 = time.perf_counter()
     oracle.run(layout, offs, w, x, steps, threads=threads)
@@ -67,6 +69,49 @@ def cpu_baseline(layout, offs, w, seed, budget_s):
     pts = layout.points
     return {"value": pts * steps / el / 1e9, "unit": "GStencil/s", "cores": threads, "kind": "port",
             "sample": f"full grid, {steps} time step(s) of the same stencil, {el:.1f} s, OpenMP {threads} threads"}
+
+
+def unfused_hbm(L, r, dense, T, seed, graph):
+    """The single-step (fusion depth 1) kernel on the same grid: its HBM throughput is the
+    streaming roofline figure (16 algorithmic bytes per point per step)."""
+    from convstencil_amd import Plan
+    with Plan(L, r, dense, fusion_depth=1, use_graph=graph) as p:
+        p.fill_random(seed)
+        rep = p.time(T, warmup=1, reps=3)
+        kernel = p.describe().split()[1].split("=", 1)[1]
+    ms_launch = rep.elapsed_ms / rep.launches
+    gbs = 16.0 * L.points / (ms_launch * 1e-3) / 1e9
+    return {"kernel": kernel, "gstencil_per_s": round(L.points * T / (rep.elapsed_ms * 1e-3) / 1e9, 2),
+            "hbm_gbs_alg": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def split_comparison(seed, graph, n=8192, T=64):
+    """box2d3r n x n, halo 3, T steps: the FP64 path and the BF16x3 split form, both with
+    the autotuner's choice, on the same seeded grid; the split result's relative L2 and
+    max error against the FP64 result (the FP64 path matches the CPU oracle to 1e-12
+    relative max in the test suite)."""
+    import convstencil_amd as cs
+    from convstencil_amd import Layout, Plan, shape_offsets, dense_weights
+    L = Layout.with_halo([n, n], 3)
+    offs = shape_offsets("box2d3r")
+    w = cs.fill_weights_host(len(offs), seed)
+    dense = dense_weights(2, 3, offs, w)
+    res, out = {}, {}
+    for prec in ("fp64", "bf16x3"):
+        with Plan(L, 3, dense, precision=prec, use_graph=graph, autotune=1) as p:
+            p.fill_random(seed)
+            rep = p.time(T, warmup=1, reps=3)
+            p.fill_random(seed)
+            p.run(T)
+            out[prec] = L.interior(p.download()).copy()
+            res[prec] = {"gstencil_per_s": round(L.points * T / (rep.elapsed_ms * 1e-3) / 1e9, 2),
+                         "kernel": p.describe().split()[1].split("=", 1)[1]}
+    d = out["bf16x3"] - out["fp64"]
+    res["bf16x3"]["rel_l2_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(out["fp64"] ** 2)))
+    res["bf16x3"]["rel_max_vs_fp64"] = float(np.max(np.abs(d)) / np.max(np.abs(out["fp64"])))
+    res["speedup_bf16x3_over_fp64"] = round(res["bf16x3"]["gstencil_per_s"] / res["fp64"]["gstencil_per_s"], 3)
+    res["workload"] = f"box2d3r {n}x{n} halo 3 T={T}"
+    return res
 
 
 def main():
@@ -80,6 +125,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's choice")
+    ap.add_argument("--no-extras", action="store_true", help="skip the unfused and split-precision lines")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "bf16x3", "fp16x2", "fp32x2"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
@@ -113,8 +160,9 @@ def main():
     w = cs.fill_weights_host(len(offs), args.seed)
     dense = dense_weights(ndim, r, offs, w)
 
+    tune = 0 if (args.variant or args.fusion or args.no_tune) else 1
     plan = Plan(L, r, dense, fusion_depth=args.fusion, variant=args.variant, use_graph=not args.no_graph,
-                precision=args.precision)
+                precision=args.precision, autotune=tune)
     plan.fill_random(args.seed)
     desc = plan.describe()
     # warmup + timed region: HIP events on the plan stream around exactly K runs of T steps
@@ -132,18 +180,23 @@ def main():
     hbm = alg_bytes / (ms_launch * 1e-3) / 1e9
     tfl = alg_flops / (ms_launch * 1e-3) / 1e12
     frac_hbm, frac_fl = hbm / HBM_PEAK_GBS, tfl / FP64_PEAK_TFLOPS
-    kernel_key = f"{args.workload}:variant{rep.variant}"
-    traffic = load_traffic(kernel_key)
+    kernel = desc.split()[1].split("=", 1)[1]
+    traffic = load_traffic(f"{args.workload}:{kernel}")
     if frac_hbm >= frac_fl:
         roof = {"bound": "hbm", "achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(frac_hbm, 4), "traffic": traffic}
     else:
         roof = {"bound": "mfma", "achieved": round(tfl, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(frac_fl, 4), "traffic": traffic}
-    roof.update({"kernel": desc.split()[1].split("=", 1)[1], "launches_per_step": launches,
+    roof.update({"kernel": kernel, "launches_per_step": launches,
                  "ms_per_launch": round(ms_launch, 5), "hbm_gbs_alg": round(hbm, 1), "fp64_tflops_alg": round(tfl, 2),
                  "alg_bytes_per_launch": alg_bytes, "alg_flops_per_launch": alg_flops})
     plan.close()
+
+    extras = {}
+    if not args.no_extras and args.workload == "box2d1r" and args.precision == "fp64":
+        extras["unfused"] = unfused_hbm(L, r, dense, T, args.seed, not args.no_graph)
+        extras["split_box2d3r"] = split_comparison(args.seed, not args.no_graph)
 
     cpu = None
     if not args.no_cpu:
@@ -166,6 +219,7 @@ def main():
                    "fusion_depth": kf, "plan": desc, "wall_s": round(wall, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(out), flush=True)
 

@@ -27,7 +27,7 @@ class cs_stencil(ctypes.Structure):
 
 class cs_options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("fusion_depth", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("tile_variant", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+                ("tile_variant", ctypes.c_int32), ("verbose", ctypes.c_int32), ("autotune", ctypes.c_int32)]
 
 
 class cs_report(ctypes.Structure):
@@ -63,6 +63,12 @@ SIGNATURES = {
     "cs_batch_time": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(cs_report)]),
     "cs_batch_download": (ctypes.c_int, [_P, ctypes.c_int32, c_double_p]),
     "cs_batch_describe": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_int32]),
+    "cs_tune_cache_path": (ctypes.c_char_p, []),
+    "cs_tune_cache_reset": (ctypes.c_int, []),
+    "cs_tune_lookup": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
+    "cs_tune_store": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double]),
+    "cs_tune_key": (ctypes.c_int, [ctypes.POINTER(cs_layout), ctypes.POINTER(cs_stencil), ctypes.POINTER(cs_options),
+                                   ctypes.c_char_p, ctypes.c_int32]),
     "cs_fill_random_host": (None, [c_double_p, ctypes.POINTER(cs_layout), ctypes.c_uint64]),
     "cs_fill_weights_host": (None, [c_double_p, ctypes.c_int32, ctypes.c_uint64]),
     "gpu_1d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int]),

@@ -41,13 +41,14 @@ class Report:
                 f"launches={self.launches}, fusion_depth={self.fusion_depth}, variant={self.variant})")
 
 
-def _options(precision="fp64", fusion_depth=0, use_graph=False, variant=0):
+def _options(precision="fp64", fusion_depth=0, use_graph=False, variant=0, autotune=0, verbose=0):
     o = _lib.cs_options()
     o.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     o.fusion_depth = int(fusion_depth)
     o.use_graph = int(bool(use_graph))
     o.tile_variant = int(variant)
-    o.verbose = 0
+    o.verbose = int(verbose)
+    o.autotune = int(autotune)
     return o
 
 

@@ -44,6 +44,7 @@ struct cs_plan {
   const Variant *vone = nullptr;      // kf == 1 variant of the same family
   std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
   int64_t launches = 0;
+  int tuned = 0;  // 0 defaults or a forced variant, 1 cached autotuner choice, 2 timed now
 };
 
 namespace {
@@ -263,6 +264,30 @@ void destroy_graphs(cs_plan *p) {
   p->graphs.clear();
 }
 
+std::string tune_key(const cs_layout &L, int ndim, int R, int shape, const cs_options &o) {
+  char k[160];
+  snprintf(k, sizeof k, "d%d_r%d_s%d_p%d_n%lldx%lldx%lld_g%d_f%d", ndim, R, shape, o.precision, (long long)L.n[0],
+           (long long)L.n[1], (long long)L.n[2], o.use_graph ? 1 : 0, o.fusion_depth);
+  return std::string(k);
+}
+
+// Variants a plan may run: same radius and precision, its shape class (split precision:
+// also the box variants), and the requested fusion depth if one is set.
+std::vector<const Variant *> tune_candidates(const cs_plan *p, int classified_shape) {
+  std::vector<const Variant *> out;
+  int cnt = 0;
+  const Variant *vs = family(p->ndim, &cnt, p->opt.precision);
+  for (int i = 0; i < cnt; ++i) {
+    const Variant &v = vs[i];
+    if (v.R != p->R) continue;
+    if (v.shape != classified_shape && !(p->opt.precision != CS_PREC_FP64 && v.shape == cs::SHAPE_BOX)) continue;
+    if (p->opt.precision != CS_PREC_FP64 && v.mf != p->opt.precision) continue;
+    if (p->opt.fusion_depth > 0 && v.kf != p->opt.fusion_depth) continue;
+    out.push_back(&v);
+  }
+  return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -316,6 +341,73 @@ void cs_fill_weights_host(double *w, int32_t count, uint64_t seed) {
   for (int i = 0; i < count; ++i) w[i] = w[i] / s;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Times every candidate variant for kTuneSteps steps (graph as the plan asks) on the
+// plan's own buffers, seeded synthetic data, and keeps the fastest; the buffers are
+// zeroed afterwards (the caller uploads or fills next).
+constexpr int64_t kTuneSteps = 24;
+
+cs_status autotune_plan(cs_plan *p, int classified) {
+  const std::vector<const Variant *> cand = tune_candidates(p, classified);
+  if (cand.empty()) return CS_OK;
+  const cs_options user = p->opt;
+  int best = 0;
+  double best_ms = 1e300;
+  for (const Variant *v : cand) {
+    destroy_graphs(p);
+    p->opt = user;
+    p->opt.tile_variant = v->id;
+    p->shape = classified;
+    if (select_variants(p) != CS_OK) continue;
+    CS_TRY(cs_plan_fill_random(p, 1));
+    cs_report r{};
+    CS_TRY(cs_plan_time(p, kTuneSteps, 1, 2, &r));
+    const double ms = r.elapsed_ms / (double)kTuneSteps;
+    if (p->opt.verbose) fprintf(stderr, "convstencil: autotune %s %.4f ms/step\n", v->name, ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = v->id;
+    }
+  }
+  destroy_graphs(p);
+  p->opt = user;
+  p->shape = classified;
+  if (best == 0) return select_variants(p);
+  p->opt.tile_variant = best;
+  CS_TRY(select_variants(p));
+  p->tuned = 2;
+  CS_HIP(cs::launch_zero(p->buf[0], (size_t)cell_count(&p->L), p->stream));
+  CS_HIP(cs::launch_zero(p->buf[1], (size_t)cell_count(&p->L), p->stream));
+  CS_HIP(hipStreamSynchronize(p->stream));
+  p->cur = 0;
+  p->steps_done = 0;
+  p->halo_uniform = true;
+  return cs_tune_store(tune_key(p->L, p->ndim, p->R, classified, user).c_str(), best, best_ms);
+}
+
+}  // namespace
+
+extern "C" {
+
+cs_status cs_tune_key(const cs_layout *layout, const cs_stencil *st, const cs_options *opt, char *buf, int32_t len) {
+  if (!layout || !st || !st->weights || !buf || len <= 0 || st->ndim != layout->ndim || st->radius < 1 ||
+      st->radius > 4)
+    return CS_ERR_INVALID;
+  const int R = st->radius;
+  const int shape = layout->ndim == 2 ? classify_2d(R, st->weights)
+                    : layout->ndim == 3 ? classify_3d(R, st->weights)
+                                        : cs::SHAPE_BOX;
+  cs_options o{};
+  if (opt) o = *opt;
+  const std::string k = tune_key(*layout, layout->ndim, R, shape, o);
+  if ((int32_t)k.size() >= len) return CS_ERR_INVALID;
+  memcpy(buf, k.c_str(), k.size() + 1);
+  return CS_OK;
+}
+
 cs_status cs_plan_create(cs_plan **out, const cs_layout *layout, const cs_stencil *st, const cs_options *opt) {
   if (!out || !layout || !st || !st->weights) return CS_ERR_INVALID;
   *out = nullptr;
@@ -334,7 +426,22 @@ cs_status cs_plan_create(cs_plan **out, const cs_layout *layout, const cs_stenci
   if (opt) p->opt = *opt;
   p->device = dev;
   p->shape = p->ndim == 2 ? classify_2d(p->R, p->w.data()) : p->ndim == 3 ? classify_3d(p->R, p->w.data()) : cs::SHAPE_BOX;
+  const int classified = p->shape;
+  if (p->opt.autotune > 0 && p->opt.tile_variant == 0) {  // a cached choice needs no timing
+    int32_t vid = 0;
+    if (p->opt.autotune == 1 &&
+        cs_tune_lookup(tune_key(p->L, p->ndim, p->R, classified, p->opt).c_str(), &vid, nullptr) == CS_OK) {
+      p->opt.tile_variant = vid;
+      p->tuned = 1;
+    }
+  }
   cs_status s = select_variants(p);
+  if (s != CS_OK && p->tuned) {  // a stale cache entry: choose again
+    p->opt.tile_variant = 0;
+    p->tuned = 0;
+    p->shape = classified;
+    s = select_variants(p);
+  }
   if (s != CS_OK) {
     cs_plan_destroy(p);
     return s;
@@ -348,6 +455,13 @@ cs_status cs_plan_create(cs_plan **out, const cs_layout *layout, const cs_stenci
     (void)hipGetLastError();
     cs_plan_destroy(p);
     return CS_ERR_ALLOC;
+  }
+  if (p->opt.autotune > 0 && p->opt.tile_variant == 0) {
+    s = autotune_plan(p, classified);
+    if (s != CS_OK) {
+      cs_plan_destroy(p);
+      return s;
+    }
   }
   *out = p;
   return CS_OK;
@@ -462,9 +576,11 @@ cs_status cs_plan_device_buffers(cs_plan *p, double **current, double **other) {
 
 cs_status cs_plan_describe(cs_plan *p, char *buf, int32_t len) {
   if (!p || !buf || len <= 0) return CS_ERR_INVALID;
-  snprintf(buf, (size_t)len, "variant=%d name=%s mf=%d R=%d shape=%s kf=%d tile=%dx%d waves=%d lds=%zu halo_uniform=%d",
+  static const char *how[3] = {"default", "cached", "timed"};
+  snprintf(buf, (size_t)len,
+           "variant=%d name=%s mf=%d R=%d shape=%s kf=%d tile=%dx%d waves=%d lds=%zu halo_uniform=%d autotune=%s",
            p->vmain->id, p->vmain->name, p->vmain->mf, p->R, p->shape == cs::SHAPE_BOX ? "box" : "star", p->vmain->kf,
-           p->vmain->by, p->vmain->bx, p->vmain->nw, p->vmain->lds_bytes, (int)p->halo_uniform);
+           p->vmain->by, p->vmain->bx, p->vmain->nw, p->vmain->lds_bytes, (int)p->halo_uniform, how[p->tuned]);
   return CS_OK;
 }
 

@@ -82,6 +82,8 @@ typedef struct cs_options {
   int32_t use_graph;     /* 1 = capture the launch sequence of cs_plan_run in a hipGraph */
   int32_t tile_variant;  /* 0 = choose; otherwise a kernel-variant id from cs_plan_describe */
   int32_t verbose;
+  int32_t autotune;      /* 0 = defaults; 1 = the autotuner's cached choice, timing every fitting
+                            variant on a miss; 2 = time them again.  Ignored with tile_variant. */
 } cs_options;
 
 typedef struct cs_report {
@@ -139,6 +141,23 @@ cs_status cs_batch_run(cs_batch *batch, int64_t steps, cs_report *report);
 cs_status cs_batch_time(cs_batch *batch, int64_t steps, int32_t warmup, int32_t reps, cs_report *report);
 cs_status cs_batch_download(cs_batch *batch, int32_t grid, double *host_out);
 cs_status cs_batch_describe(cs_batch *batch, char *buf, int32_t len);
+
+/* ---- autotuner cache ----
+ * No counterpart in the reference (fixed tiles).  A plan created with autotune times the
+ * fitting variants for 24 steps on its own buffers (before any data is uploaded) and keeps
+ * the fastest; choices are cached per key "d<ndim>_r<radius>_s<shape>_p<precision>_n<nz>x
+ * <ny>x<nx>_g<graph>" in memory and in a text file: $CONVSTENCIL_TUNE_CACHE (empty: memory
+ * only), else $XDG_CACHE_HOME/convstencil_amd/tune.txt, else $HOME/.cache/convstencil_amd/tune.txt. */
+const char *cs_tune_cache_path(void);
+/* Forgets the in-memory cache; the next use re-reads the environment and the file. */
+cs_status cs_tune_cache_reset(void);
+/* CS_OK and the cached variant (and its time per step, if ms_per_step is not NULL), or
+ * CS_ERR_UNSUPPORTED if the key has no entry; keys contain no white space. */
+cs_status cs_tune_lookup(const char *key, int32_t *variant, double *ms_per_step);
+cs_status cs_tune_store(const char *key, int32_t variant, double ms_per_step);
+/* The cache key of a plan with these arguments (host only; no device needed). */
+cs_status cs_tune_key(const cs_layout *layout, const cs_stencil *stencil, const cs_options *opt, char *buf,
+                      int32_t len);
 
 /* ---- one-shot host-to-host run (what the reference launchers do, minus printing) ---- */
 cs_status cs_run(const cs_layout *layout, const cs_stencil *stencil, const double *host_in, double *host_out,
