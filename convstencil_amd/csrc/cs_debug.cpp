@@ -137,6 +137,38 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
   return t.n;
 }
 
+int32_t cs_debug_dtable3(int32_t radius, int32_t shape, int32_t dz, int32_t *hdr, int32_t *ty, int32_t *tx,
+                         int32_t *live, int32_t cap) {
+  if (radius < 1 || radius > 3 || dz < -radius || dz > radius || !hdr) return -1;
+  const cs::DTable3 t3 = cs::make_dtable3(radius, shape);
+  const cs::DTable &t = t3.t[dz + radius];
+  const int n = 4 * (t.np + t.ns);
+  if (n > cap) return -1;
+  hdr[0] = t.np;
+  hdr[1] = t.ns;
+  hdr[2] = t.pp;
+  for (int ps = 0; ps < t.np; ++ps)
+    for (int q = 0; q < 4; ++q) {
+      ty[4 * ps + q] = t.pty[ps][q];
+      tx[4 * ps + q] = t.ptx[ps][q];
+      live[4 * ps + q] = t.plive[ps][q];
+    }
+  for (int ss = 0; ss < t.ns; ++ss)
+    for (int q = 0; q < 4; ++q) {
+      ty[4 * (t.np + ss) + q] = t.sty[ss][q];
+      tx[4 * (t.np + ss) + q] = t.stx[ss][q];
+      live[4 * (t.np + ss) + q] = t.slive[ss][q];
+    }
+  return n;
+}
+
+int32_t cs_debug_bfrag_3d2(int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap) {
+  const int nks = cs::bfrag_ksteps_3d2_f64(radius, shape);
+  if (nks * 64 > cap) return -1;
+  cs::build_bfrag_3d2_f64(radius, shape, w, bfrag);
+  return nks;
+}
+
 int32_t cs_debug_copy_bench(int64_t count, int32_t reps, double *ms_per_copy) {
   // Plain device-to-device copy of `count` doubles (one element per thread), `reps`
   // times: the reference point for HBM byte counters (FETCH_SIZE / WRITE_SIZE).

@@ -119,9 +119,13 @@ cs_status select_variants(cs_plan *p) {
     if (!v && prec != CS_PREC_FP64) v = find_variant(nd, 0, p->R, cs::SHAPE_BOX, 0, p->opt.tile_variant, prec);
     if (!v) return CS_ERR_INVALID;
   } else if (prec == CS_PREC_FP64) {
-    const int top = max_kf(nd, mf, p->R, p->shape, prec);
-    int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
-    for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, p->shape, kf, 0, prec);
+    // 3-D: the 8 x 8 block family (mf 8) where it exists, else the 4 x 16 strips
+    for (int fmf : {nd == 3 ? 8 : mf, mf}) {
+      const int top = max_kf(nd, fmf, p->R, p->shape, prec);
+      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
+      for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, p->R, p->shape, kf, 0, prec);
+      if (v) break;
+    }
   } else {
     const int tries[3][2] = {{p->shape, 2}, {cs::SHAPE_BOX, 2}, {cs::SHAPE_BOX, 0}};
     for (const auto &tr : tries) {
@@ -157,12 +161,15 @@ cs_status select_variants(cs_plan *p) {
   // B fragments for the family's MFMA shape
   const int nks = nd == 1   ? cs::bfrag_ksteps_1d_f64(p->R)
                   : nd == 2 ? cs::bfrag_ksteps_2d_f64(v->mf, p->R, p->shape)
-                            : cs::bfrag_ksteps_3d_f64(p->R, p->shape);
+                  : v->mf == 8 ? cs::bfrag_ksteps_3d2_f64(p->R, p->shape)
+                               : cs::bfrag_ksteps_3d_f64(p->R, p->shape);
   std::vector<double> h((size_t)nks * 64);
   if (nd == 1)
     cs::build_bfrag_1d_f64(p->R, p->w.data(), h.data());
   else if (nd == 2)
     cs::build_bfrag_2d_f64(v->mf, p->R, p->shape, p->w.data(), h.data());
+  else if (v->mf == 8)
+    cs::build_bfrag_3d2_f64(p->R, p->shape, p->w.data(), h.data());
   else
     cs::build_bfrag_3d_f64(p->R, p->shape, p->w.data(), h.data());
   if (p->bfrag) (void)hipFree(p->bfrag);

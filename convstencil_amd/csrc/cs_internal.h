@@ -37,7 +37,8 @@ struct Variant {
 
 const Variant *variants_1d_f64(int *count);
 const Variant *variants_2d_f64(int *count);
-const Variant *variants_3d_f64(int *count);
+const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
+const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only
 const Variant *variants_2d_split(int *count);
 
 // Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].
@@ -46,6 +47,8 @@ int bfrag_ksteps_2d_f64(int mf, int R, int shape);
 void build_bfrag_1d_f64(int R, const double *w_dense, double *bfrag);
 int bfrag_ksteps_1d_f64(int R);
 void build_bfrag_3d_f64(int R, int shape, const double *w_dense, double *bfrag);
+void build_bfrag_3d2_f64(int R, int shape, const double *w_dense, double *bfrag);
+int bfrag_ksteps_3d2_f64(int R, int shape);
 // Split-precision B fragments (prec: 1 BF16x3, 2 FP32x2, 3 FP16x2); returns bytes written, 0 if cap is short.
 size_t build_bfrag_split(int prec, int R, const double *w_dense, void *out, size_t cap);
 // A fragments of the dilated split form (Variant::st == 2; prec 1 BF16x3 or 3 FP16x2).

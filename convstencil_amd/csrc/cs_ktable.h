@@ -208,15 +208,28 @@ struct DTable {
   int cmin, cmax;      // columns t.x read (a pair's second column included), relative to 2m
 };
 
+// Row spans of a 2-D tap set of radius R: taps (dy, dx) with dx in [lo[dy + R], hi[dy + R]]
+// (lo > hi: no taps in that row).  The dilated tables below are built from these.
+struct Spans {
+  int R;
+  int lo[9], hi[9];
+};
+
+constexpr Spans spans_2d(int R, int shape) {
+  Spans s{};
+  s.R = R;
+  for (int dy = -R; dy <= R; ++dy) row_span(R, shape, dy, s.lo[dy + R], s.hi[dy + R]);
+  return s;
+}
+
 // Columns [lo, hi] of row ty of the dilated footprint T (lo > hi if the row is empty).
-constexpr void dilated_row(int R, int shape, int ty, int &lo, int &hi) {
+constexpr void dilated_row(const Spans &sp, int ty, int &lo, int &hi) {
   lo = 1 << 20;
   hi = -(1 << 20);
   for (int vy = 0; vy <= 1; ++vy) {
     const int dy = ty - vy;
-    if (dy < -R || dy > R) continue;
-    int a = 0, b = -1;
-    row_span(R, shape, dy, a, b);
+    if (dy < -sp.R || dy > sp.R) continue;
+    const int a = sp.lo[dy + sp.R], b = sp.hi[dy + sp.R];
     if (a > b) continue;
     lo = a < lo ? a : lo;
     hi = b + 1 > hi ? b + 1 : hi;
@@ -225,15 +238,18 @@ constexpr void dilated_row(int R, int shape, int ty, int &lo, int &hi) {
 
 constexpr int ceil4(int v) { return (v + 3) / 4; }
 
-constexpr DTable make_dtable(int R, int shape) {
+// pp_force: -1 chooses the pair parity, 0 or 1 imposes it (3-D planes share one parity).
+constexpr DTable make_dtable_spans(const Spans &sp, int pp_force) {
+  const int R = sp.R;
   DTable best{};
   int best_ks = 1 << 30, best_rd = 1 << 30;
   for (int pp = 0; pp < 2; ++pp) {
+    if (pp_force >= 0 && pp != pp_force) continue;
     int pty[64] = {}, ptx[64] = {}, np0 = 0;
     int sty[64] = {}, stx[64] = {}, side[64] = {}, ns0 = 0;
     for (int ty = -R; ty <= R + 1; ++ty) {
       int lo = 0, hi = -1;
-      dilated_row(R, shape, ty, lo, hi);
+      dilated_row(sp, ty, lo, hi);
       if (lo > hi) continue;
       int a = lo;
       if ((((a % 2) + 2) % 2) != pp) {  // left end of the wrong parity
@@ -327,6 +343,8 @@ constexpr DTable make_dtable(int R, int shape) {
   return best;
 }
 
+constexpr DTable make_dtable(int R, int shape) { return make_dtable_spans(spans_2d(R, shape), -1); }
+
 // Number of k-steps of a dilated table.
 constexpr int dtable_ksteps(const DTable &t) { return 2 * t.np + t.ns; }
 
@@ -402,6 +420,53 @@ constexpr CTable make_ctable(int R, int shape) {
     if (t.n < best.n) best = t;
   }
   return best;
+}
+
+}  // namespace cs
+
+namespace cs {
+
+// ---- dilated 4x4x4 tables of the 3-D kernels: one 2-D table per plane dz ----
+// Every plane's table uses the same pair parity (a block's column alignment holds for all
+// dz); the parity with the fewest k-steps (then LDS reads) over all planes is chosen.
+struct DTable3 {
+  int R, pp;
+  DTable t[7];  // plane dz at t[dz + R]
+};
+
+constexpr Spans spans_3d(int R, int shape, int dz) {
+  Spans s{};
+  s.R = R;
+  for (int dy = -R; dy <= R; ++dy) row_span_3d(R, shape, dz, dy, s.lo[dy + R], s.hi[dy + R]);
+  return s;
+}
+
+constexpr DTable3 make_dtable3(int R, int shape) {
+  DTable3 best{};
+  int best_ks = 1 << 30, best_rd = 1 << 30;
+  for (int pp = 0; pp < 2; ++pp) {
+    DTable3 c{};
+    c.R = R;
+    c.pp = pp;
+    int ks = 0, rd = 0;
+    for (int dz = -R; dz <= R; ++dz) {
+      c.t[dz + R] = make_dtable_spans(spans_3d(R, shape, dz), pp);
+      ks += dtable_ksteps(c.t[dz + R]);
+      rd += c.t[dz + R].np + c.t[dz + R].ns;
+    }
+    if (ks < best_ks || (ks == best_ks && rd < best_rd)) {
+      best = c;
+      best_ks = ks;
+      best_rd = rd;
+    }
+  }
+  return best;
+}
+
+constexpr int dtable3_ksteps(const DTable3 &t) {
+  int n = 0;
+  for (int dz = -t.R; dz <= t.R; ++dz) n += dtable_ksteps(t.t[dz + t.R]);
+  return n;
 }
 
 }  // namespace cs

@@ -1,0 +1,300 @@
+// k3d2_f64.hip -- FP64 3-D ConvStencil kernels on 8 x 8 blocks (dilated v_mfma_f64_4x4x4).
+//
+// Replaces gpu_box_3d1r_kernel (reference src/3d/gpu_box.cu:186-238) and
+// gpu_star_3d1r_step3_kernel (src/3d/gpu_star.cu:178-236) for radius 1 (the 2.5-D
+// kernel in k3d_f64.hip remains for larger radii):
+//   * a workgroup owns a BY x BX column tile and a chunk of ZC output planes and
+//     streams its input planes through a ring of 2R+1 LDS planes; the next plane's
+//     loads are all issued (into registers) before the current plane is computed, so
+//     HBM traffic overlaps the MFMAs;
+//   * an output plane is a sum over dz of 2-D products, each with the dilated table of
+//     its plane (cs_ktable.h, DTable3): one v_mfma_f64_4x4x4 chain per 8 x 8 output
+//     block, 16-byte LDS reads, 12 k-steps per 64 outputs for box3d1r, 5 for star3d1r;
+//     all planes share one pair parity so a block's column alignment holds for each dz;
+//   * the plane loop over dz is unrolled at compile time (every table entry, k-step
+//     count and LDS read kind is a constant), B stays in registers for the whole kernel.
+#include "cs_internal.h"
+
+namespace cs {
+
+typedef double dbl2v __attribute__((ext_vector_type(2)));
+
+template <int R_, int SH_, int BY_, int BX_, int ZC_, int NW_>
+struct Cfg3D2 {
+  static constexpr int R = R_, SH = SH_, BY = BY_, BX = BX_, ZC = ZC_, NW = NW_;
+  static constexpr DTable3 DT = make_dtable3(R, SH);
+  static constexpr int PP = DT.pp;
+  static constexpr int NKS = dtable3_ksteps(DT);
+  static constexpr int EY = BY + 2 * R;
+  static constexpr int EX = BX + 2 * R;
+  // k-step and LDS-read offsets of plane dz within b[] and off[]
+  static constexpr int kbase(int dz) {
+    int n = 0;
+    for (int d = -R; d < dz; ++d) n += dtable_ksteps(DT.t[d + R]);
+    return n;
+  }
+  static constexpr int rbase(int dz) {
+    int n = 0;
+    for (int d = -R; d < dz; ++d) n += DT.t[d + R].np + DT.t[d + R].ns;
+    return n;
+  }
+  static constexpr int NRD = rbase(R + 1);
+  static constexpr int cmin() {
+    int m = 1 << 20;
+    for (int d = -R; d <= R; ++d) m = DT.t[d + R].cmin < m ? DT.t[d + R].cmin : m;
+    return m;
+  }
+  static constexpr int cmax() {
+    int m = -(1 << 20);
+    for (int d = -R; d <= R; ++d) m = DT.t[d + R].cmax > m ? DT.t[d + R].cmax : m;
+    return m;
+  }
+  static constexpr int tmin() {
+    int m = 1 << 20;
+    for (int d = -R; d <= R; ++d) m = DT.t[d + R].tmin < m ? DT.t[d + R].tmin : m;
+    return m;
+  }
+  static constexpr int tmax() {
+    int m = -(1 << 20);
+    for (int d = -R; d <= R; ++d) m = DT.t[d + R].tmax > m ? DT.t[d + R].tmax : m;
+    return m;
+  }
+  // region: rows [R, R + BY), columns [R, R + BX); tile column c is stored at c + CS and
+  // blocks start at columns x with x + CS + PP even
+  static constexpr int lox(int cs) { return R - ((R + cs + PP) & 1); }
+  static constexpr int find_cs() {
+    for (int cs = 0; cs < 16; ++cs)
+      if (lox(cs) + cmin() + cs >= 0) return cs;
+    return 16;
+  }
+  static constexpr int CS = find_cs();
+  static constexpr int LOX = lox(CS);
+  static constexpr int LASTX = (R + BX - 8) + (((R + BX - 8) + CS + PP) & 1);
+  static constexpr int NBX = (LASTX - LOX + 7) / 8 + 1;
+  static constexpr int NBY = BY / 8;
+  static constexpr int NB = NBY * NBX;
+  static constexpr int PER = (NB + NW - 1) / NW;
+  static constexpr int PX0 = CS + (LASTX + 6 + cmax() + 1 > EX ? LASTX + 6 + cmax() + 1 : EX);
+  static constexpr int PX = PX0 + ((4 - PX0 % 8) + 8) % 8;  // 4 (mod 8), as the 2-D kernels
+  static constexpr int PLANE = EY * PX;
+  static constexpr int NP = 2 * R + 1;
+  static constexpr int SPARE = NP * PLANE;
+  static constexpr size_t LDS = ((size_t)NP * PLANE + 2) * sizeof(double);
+  static constexpr int PF = (PLANE + NW * 64 - 1) / (NW * 64);  // plane loads per thread
+  static_assert(BY % 8 == 0 && BX % 16 == 0, "tile must be a multiple of the 8 x 8 block");
+  static_assert(CS < 16 && PX % 2 == 0 && (LOX + CS + PP) % 2 == 0, "pair reads must be aligned");
+  static_assert(R + tmin() >= 0 && R + BY - 8 + 6 + tmax() < EY, "block reads outside the plane");
+  static_assert(LDS <= 160 * 1024, "ring does not fit gfx950 LDS");
+};
+
+template <class C>
+__device__ __forceinline__ void block_origin3(int blk, int &y, int &x) {
+  const int ba = blk / C::NBX, bb = blk - ba * C::NBX;
+  y = C::R + 8 * ba;
+  x = min(C::LOX + 8 * bb, C::LASTX);
+}
+
+// Adds plane dz's products to the block's accumulator (plane: the ring slot of z + dz).
+template <class C, int DZ>
+__device__ __forceinline__ double plane_chain(const double *plane, int boff, const int *off, const double *b,
+                                              double a) {
+  constexpr DTable T = C::DT.t[DZ + C::R];
+  constexpr int K0 = C::kbase(DZ), R0 = C::rbase(DZ);
+  const double *p = plane + boff;
+#pragma unroll
+  for (int ps = 0; ps < T.np; ++ps) {
+    const dbl2v pr = *reinterpret_cast<const dbl2v *>(p + off[R0 + ps]);
+    a = __builtin_amdgcn_mfma_f64_4x4x4f64(pr.x, b[K0 + 2 * ps], a, 0, 0, 0);
+    a = __builtin_amdgcn_mfma_f64_4x4x4f64(pr.y, b[K0 + 2 * ps + 1], a, 0, 0, 0);
+  }
+#pragma unroll
+  for (int ss = 0; ss < T.ns; ++ss)
+    a = __builtin_amdgcn_mfma_f64_4x4x4f64(p[off[R0 + T.np + ss]], b[K0 + 2 * T.np + ss], a, 0, 0, 0);
+  return a;
+}
+
+template <class C, int DZ>
+__device__ __forceinline__ double all_planes(const double *lds, const int *slot, int boff, const int *off,
+                                             const double *b, double a) {
+  if constexpr (DZ > C::R) {
+    return a;
+  } else {
+    a = plane_chain<C, DZ>(lds + slot[DZ + C::R] * C::PLANE, boff, off, b, a);
+    return all_planes<C, DZ + 1>(lds, slot, boff, off, b, a);
+  }
+}
+
+template <int R, int SH, int BY, int BX, int ZC, int NW>
+__global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ in, double *__restrict__ out,
+                                                    const double *__restrict__ bfrag, Geo g) {
+  using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
+  constexpr int PX = C::PX, NT = NW * 64;
+  extern __shared__ __align__(16) double lds[];
+
+  const int64_t tiles_yx = g.tiles[1] * g.tiles[2];
+  const int64_t ntiles = g.tiles[0] * tiles_yx;
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t tile = (blockIdx.x % 8) * per + blockIdx.x / 8;  // XCD-contiguous tile bands
+  if (tile >= ntiles) return;
+  const int64_t zc = tile / tiles_yx, rem = tile % tiles_yx;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int64_t ext1 = g.ext[1], ext2 = g.ext[2];
+  const int64_t nz = g.n[0], ny = g.n[1], nx = g.n[2];
+  const int64_t Y0 = (rem / g.tiles[2]) * BY - R, X0 = (rem % g.tiles[2]) * BX - R;
+  const int64_t ay0 = g.org[1] + Y0, ax0 = g.org[2] + X0;
+  const int64_t Z0 = zc * ZC;
+  const int64_t Z1 = Z0 + ZC < nz ? Z0 + ZC : nz;
+  const bool full = Y0 + C::EY <= ny && X0 + C::EX <= nx && Y0 >= 0 && X0 >= 0;
+  const bool all_in = ay0 >= 0 && ay0 + C::EY <= ext1 && ax0 >= 0 && ax0 + C::EX <= ext2;
+
+  double b[C::NKS];
+#pragma unroll
+  for (int ks = 0; ks < C::NKS; ++ks) b[ks] = bfrag[ks * 64 + lane];
+  int off[C::NRD];
+  {
+    const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
+#pragma unroll
+    for (int dz = -R; dz <= R; ++dz) {
+      const DTable &T = C::DT.t[dz + R];
+      const int r0 = C::rbase(dz);
+#pragma unroll
+      for (int ps = 0; ps < T.np; ++ps) off[r0 + ps] = (2 * b4 + T.pty[ps][q]) * PX + 2 * m + T.ptx[ps][q] + C::CS;
+#pragma unroll
+      for (int ss = 0; ss < T.ns; ++ss)
+        off[r0 + T.np + ss] = (2 * b4 + T.sty[ss][q]) * PX + 2 * m + T.stx[ss][q] + C::CS;
+    }
+  }
+  const int orow = 2 * ((lane >> 2) & 3) + ((lane & 3) >> 1), ocol = 2 * (lane >> 4) + (lane & 1);
+
+  // plane p (interior z) of the tile's column, cell idx of the LDS plane layout
+  auto issue = [&](int64_t p, double *v) {
+    const double *base = in + ((g.org[0] + p) * ext1 + ay0) * ext2 + ax0;
+#pragma unroll
+    for (int i = 0; i < C::PF; ++i) {
+      const int idx = tid + i * NT;
+      const int r = idx / PX, c = idx - r * PX - C::CS;
+      const bool ok = idx < C::PLANE && c >= 0 && c < C::EX &&
+                      (all_in || (ay0 + r >= 0 && ay0 + r < ext1 && ax0 + c >= 0 && ax0 + c < ext2));
+      const double x = *(ok ? base + ((int64_t)r * ext2 + c) : in);
+      v[i] = ok ? x : 0.0;
+    }
+  };
+  auto store = [&](int s, const double *v) {
+    double *dst = lds + s * C::PLANE;
+#pragma unroll
+    for (int i = 0; i < C::PF; ++i) {
+      const int idx = tid + i * NT;
+      dst[(i < C::PF - 1 || idx < C::PLANE) ? idx : C::SPARE - s * C::PLANE] = v[i];
+    }
+  };
+  auto slot_of = [&](int64_t p) -> int { return (int)((p - (Z0 - R)) % C::NP); };
+
+  // prologue: planes Z0 - R .. Z0 + R - 1 into the ring, plane Z0 + R in flight
+  double pf[C::PF];
+  for (int64_t p = Z0 - R; p < Z0 + R; ++p) {
+    issue(p, pf);
+    store(slot_of(p), pf);
+  }
+  issue(Z0 + R, pf);
+
+  for (int64_t z = Z0; z < Z1; ++z) {
+    store(slot_of(z + R), pf);
+    __syncthreads();  // planes z - R .. z + R are in the ring
+    if (z + 1 < Z1) issue(z + R + 1, pf);
+    int slot[C::NP];
+#pragma unroll
+    for (int d = 0; d < C::NP; ++d) slot[d] = __builtin_amdgcn_readfirstlane(slot_of(z - R + d));
+    double acc[C::PER];
+#pragma unroll
+    for (int j = 0; j < C::PER; ++j) {
+      int y, x;
+      block_origin3<C>(min(j * NW + wave, C::NB - 1), y, x);
+      acc[j] = all_planes<C, -R>(lds, slot, y * PX + x, off, b, 0.0);
+    }
+    double *oplane = out + (g.org[0] + z) * ext1 * ext2;
+#pragma unroll
+    for (int j = 0; j < C::PER; ++j) {
+      const int blk = j * NW + wave;
+      if ((C::NB % NW == 0 || j < C::PER - 1) || blk < C::NB) {
+        int y, x;
+        block_origin3<C>(blk, y, x);
+        const int ty = y + orow, tx = x + ocol;
+        if (tx >= R && tx < R + BX && (full || (Y0 + ty < ny && X0 + tx < nx)))
+          oplane[(ay0 + ty) * ext2 + ax0 + tx] = acc[j];
+      }
+    }
+    __syncthreads();  // every wave is done with plane z - R before its slot is refilled
+  }
+}
+
+template <int R, int SH, int BY, int BX, int ZC, int NW>
+static hipError_t prepare_k3d2() {
+  using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
+  return hipFuncSetAttribute((const void *)k3d2_f64<R, SH, BY, BX, ZC, NW>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
+}
+
+template <int R, int SH, int BY, int BX, int ZC, int NW>
+static hipError_t launch_k3d2(const double *in, double *out, const double *bfrag, const Geo &g0, hipStream_t s) {
+  using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
+  Geo g = g0;
+  g.tiles[0] = (g.n[0] + ZC - 1) / ZC;
+  const int64_t ntiles = g.tiles[0] * g.tiles[1] * g.tiles[2];
+  const int64_t grid = ((ntiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((k3d2_f64<R, SH, BY, BX, ZC, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in, out,
+                     bfrag, g);
+  return hipGetLastError();
+}
+
+// Variant::mf = 8 marks this family (8 x 8 blocks, dilated tables); st = z-chunk length.
+#define CS_V3D2(ID, R, SH, BY, BX, ZC, NW)                                                                     \
+  Variant {                                                                                                    \
+    ID, 8, R, SH, 1, BY, BX, NW, Cfg3D2<R, SH, BY, BX, ZC, NW>::LDS, Cfg3D2<R, SH, BY, BX, ZC, NW>::PX, ZC,    \
+        &launch_k3d2<R, SH, BY, BX, ZC, NW>, &prepare_k3d2<R, SH, BY, BX, ZC, NW>,                             \
+        "k3d2_f64<r" #R "," #SH "," #BY "x" #BX ",zc" #ZC ",w" #NW ">", Cfg3D2<R, SH, BY, BX, ZC, NW>::CS    \
+  }
+
+static const Variant kVariants3D2[] = {
+    CS_V3D2(41101, 1, SHAPE3_BOX, 32, 64, 32, 8),   CS_V3D2(41111, 1, SHAPE3_STAR, 32, 64, 32, 8),
+    CS_V3D2(41121, 1, SHAPE3_DIAMOND, 32, 64, 32, 8), CS_V3D2(41102, 1, SHAPE3_BOX, 16, 64, 32, 4),
+    CS_V3D2(41112, 1, SHAPE3_STAR, 16, 64, 32, 4),  CS_V3D2(41103, 1, SHAPE3_BOX, 32, 64, 64, 8),
+    CS_V3D2(41113, 1, SHAPE3_STAR, 32, 64, 64, 8),  CS_V3D2(41104, 1, SHAPE3_BOX, 16, 128, 32, 8),
+    CS_V3D2(41114, 1, SHAPE3_STAR, 16, 128, 32, 8),
+};
+
+const Variant *variants_3d2_f64(int *count) {
+  *count = (int)(sizeof(kVariants3D2) / sizeof(kVariants3D2[0]));
+  return kVariants3D2;
+}
+
+int bfrag_ksteps_3d2_f64(int R, int shape) { return dtable3_ksteps(make_dtable3(R, shape)); }
+
+// B fragments: plane dz's k-steps in table order (pair steps, then singles); lane group
+// k = lane >> 4 holds footprint cell t, output column n = lane & 3 is moved by
+// v_n = (n >> 1, n & 1), so B[t][n] = W[dz][t - v_n].
+void build_bfrag_3d2_f64(int R, int shape, const double *w, double *bfrag) {
+  const DTable3 t3 = make_dtable3(R, shape);
+  const int D = 2 * R + 1;
+  int ks = 0;
+  for (int dz = -R; dz <= R; ++dz) {
+    const DTable &t = t3.t[dz + R];
+    auto put = [&](int kk, int k, int ty, int tx, int live) {
+      for (int lane = 16 * k; lane < 16 * k + 16; ++lane) {
+        const int n = lane & 3;
+        const int dy = ty - (n >> 1), dx = tx - (n & 1);
+        double v = 0.0;
+        if (live && dy >= -R && dy <= R && dx >= -R && dx <= R) v = w[((dz + R) * D + (dy + R)) * D + (dx + R)];
+        bfrag[kk * 64 + lane] = v;
+      }
+    };
+    for (int ps = 0; ps < t.np; ++ps)
+      for (int k = 0; k < 4; ++k)
+        for (int h = 0; h < 2; ++h) put(ks + 2 * ps + h, k, t.pty[ps][k], t.ptx[ps][k] + h, t.plive[ps][k]);
+    for (int ss = 0; ss < t.ns; ++ss)
+      for (int k = 0; k < 4; ++k) put(ks + 2 * t.np + ss, k, t.sty[ss][k], t.stx[ss][k], t.slive[ss][k]);
+    ks += dtable_ksteps(t);
+  }
+}
+
+}  // namespace cs

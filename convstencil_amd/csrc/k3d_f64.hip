@@ -15,6 +15,8 @@
 //   * B (banded Toeplitz of the weights) is held in registers when the stencil
 //     needs at most BREG_MAX k-steps; the 343-point radius-3 box reloads each dz
 //     group's fragments once per plane and shares them across the wave's blocks.
+#include <vector>
+
 #include "cs_internal.h"
 
 namespace cs {
@@ -202,8 +204,16 @@ static const Variant kVariants3D[] = {
 };
 
 const Variant *variants_3d_f64(int *count) {
-  *count = (int)(sizeof(kVariants3D) / sizeof(kVariants3D[0]));
-  return kVariants3D;
+  // the 8 x 8 block family (k3d2_f64.hip) first: the host takes the first match
+  static const std::vector<Variant> all = [] {
+    int n2 = 0;
+    const Variant *v2 = variants_3d2_f64(&n2);
+    std::vector<Variant> v(v2, v2 + n2);
+    v.insert(v.end(), kVariants3D, kVariants3D + sizeof(kVariants3D) / sizeof(kVariants3D[0]));
+    return v;
+  }();
+  *count = (int)all.size();
+  return all.data();
 }
 
 int bfrag_ksteps_3d_f64(int R, int shape) { return make_ktable3d(R, shape).n; }

@@ -66,6 +66,13 @@ int64_t cs_debug_bfrag_split2(int32_t precision, int32_t radius, int32_t shape, 
 int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty, int32_t *tx, int32_t *live,
                         int32_t cap);
 
+/* Dilated table of plane dz of the 3-D 8 x 8 block kernels (variants with mfma shape 8), in the
+ * layout of cs_debug_dtable; all planes share the pair parity hdr[2].  Returns 4 * (np + ns) or -1. */
+int32_t cs_debug_dtable3(int32_t radius, int32_t shape, int32_t dz, int32_t *hdr, int32_t *ty, int32_t *tx,
+                         int32_t *live, int32_t cap);
+/* B fragments of the 3-D 8 x 8 block kernels, planes dz = -r .. r in order; returns the k-step count. */
+int32_t cs_debug_bfrag_3d2(int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap);
+
 /* Times `reps` plain device copies of `count` doubles (needs a GPU); the kernel is the
  * reference for HBM byte counters.  Returns 0 on success. */
 int32_t cs_debug_copy_bench(int64_t count, int32_t reps, double *ms_per_copy);

@@ -505,6 +505,119 @@ def emulate_3d(v, layout, dense, x):
     return out
 
 
+def dtable3(R, shape, dz):
+    lib = _dbg()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.cs_debug_dtable3.restype = ctypes.c_int32
+    lib.cs_debug_dtable3.argtypes = [ctypes.c_int32] * 3 + [ip, ip, ip, ip, ctypes.c_int32]
+    cap = 256
+    hdr = (ctypes.c_int32 * 3)()
+    ty, tx, live = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+    n = lib.cs_debug_dtable3(R, shape, dz, hdr, ty, tx, live, cap)
+    assert n > 0 and n == 4 * (hdr[0] + hdr[1])
+    return hdr[0], hdr[1], hdr[2], np.array(ty[:n]).reshape(-1, 4), np.array(tx[:n]).reshape(-1, 4), \
+        np.array(live[:n]).reshape(-1, 4)
+
+
+def bfrag3d2(R, shape, w):
+    lib = _dbg()
+    lib.cs_debug_bfrag_3d2.restype = ctypes.c_int32
+    lib.cs_debug_bfrag_3d2.argtypes = [ctypes.c_int32, ctypes.c_int32, _lib.c_double_p, _lib.c_double_p,
+                                       ctypes.c_int32]
+    cap = 64 * 256
+    out = np.zeros(cap)
+    wd = np.ascontiguousarray(w, dtype=np.float64)
+    nks = lib.cs_debug_bfrag_3d2(R, shape, wd.ctypes.data_as(_lib.c_double_p), out.ctypes.data_as(_lib.c_double_p),
+                                 cap)
+    assert nks > 0
+    return out[:nks * 64].reshape(nks, 64)
+
+
+def emulate_3d2(v, layout, dense, x):
+    """The 8 x 8 block 3-D kernel (mfma shape 8): ring of 2R+1 planes, dilated table per dz."""
+    R, BY, BX, ZC = v["R"], v["by"], v["bx"], v["st"]
+    tabs = {dz: dtable3(R, v["shape"], dz) for dz in range(-R, R + 1)}
+    pps = {t[2] for t in tabs.values()}
+    assert len(pps) == 1
+    pp = pps.pop()
+    bf = bfrag3d2(R, v["shape"], dense)
+    assert bf.shape[0] == sum(2 * t[0] + t[1] for t in tabs.values())
+    EY, EX, PX, CS = BY + 2 * R, BX + 2 * R, v["px"], v["cs"]
+    assert PX % 8 == 4
+    PLANE = EY * PX
+    NP = 2 * R + 1
+    assert v["lds"] == (NP * PLANE + 2) * 8 and v["lds"] <= 160 * 1024
+    lanes = np.arange(64)
+    q, b4, m4 = lanes >> 4, (lanes >> 2) & 3, lanes & 3
+    orow, ocol = 2 * b4 + ((lanes & 3) >> 1), 2 * (lanes >> 4) + (lanes & 1)
+    LOX = R - ((R + CS + pp) & 1)
+    LASTX = (R + BX - 8) + (((R + BX - 8) + CS + pp) & 1)
+    NBX = (LASTX - LOX + 7) // 8 + 1
+    ext, org, n = layout.ext, layout.org, layout.n
+    x3 = x.reshape(ext)
+    out = np.zeros(layout.cells)
+    o3 = out.reshape(ext)
+    tiles = (-(-n[0] // ZC), -(-n[1] // BY), -(-n[2] // BX))
+    for zc in range(tiles[0]):
+        for ty_ in range(tiles[1]):
+            for tx_ in range(tiles[2]):
+                Y0, X0 = ty_ * BY - R, tx_ * BX - R
+                ay0, ax0 = org[1] + Y0, org[2] + X0
+                Z0, Z1 = zc * ZC, min(zc * ZC + ZC, n[0])
+                ring = np.full((NP, PLANE), np.nan)
+
+                def load(p):
+                    pl = np.zeros(PLANE)
+                    az = org[0] + p
+                    assert 0 <= az < ext[0]
+                    for idx in range(PLANE):
+                        r, c = divmod(idx, PX)
+                        c -= CS
+                        ay, ax = ay0 + r, ax0 + c
+                        if 0 <= c < EX and 0 <= ay < ext[1] and 0 <= ax < ext[2]:
+                            pl[idx] = x3[az, ay, ax]
+                    return pl
+                slot = lambda p: (p - (Z0 - R)) % NP
+                for p in range(Z0 - R, Z0 + R + 1):
+                    ring[slot(p)] = load(p)
+                for z in range(Z0, Z1):
+                    for blk in range((BY // 8) * NBX):
+                        ba, bb = divmod(blk, NBX)
+                        y, xx = R + 8 * ba, min(LOX + 8 * bb, LASTX)
+                        acc = np.zeros((4, 4, 4))
+                        ks = 0
+                        for dz in range(-R, R + 1):
+                            npair, ns, _, tty, ttx, tlive = tabs[dz]
+                            pl = ring[slot(z + dz)]
+                            for e in range(npair + ns):
+                                idx = (y + 2 * b4 + tty[e][q]) * PX + xx + 2 * m4 + ttx[e][q] + CS
+                                assert idx.min() >= 0 and idx.max() + (1 if e < npair else 0) < PLANE
+                                if e < npair:
+                                    assert np.all(idx % 2 == 0), "misaligned pair read"
+                                    cols = [idx, idx + 1]
+                                else:
+                                    cols = [idx]
+                                for ci in cols:
+                                    a = pl[ci]
+                                    assert np.isfinite(a).all()
+                                    A = np.zeros((4, 4, 4))
+                                    B = np.zeros((4, 4, 4))
+                                    A[b4, m4, q] = a
+                                    B[b4, q, lanes & 3] = bf[ks]
+                                    acc += A @ B
+                                    ks += 1
+                        vals = acc[b4, lanes >> 4, lanes & 3]
+                        for ln in range(64):
+                            tyy, txx = y + orow[ln], xx + ocol[ln]
+                            if R <= txx < R + BX and Y0 + tyy < n[1] and X0 + txx < n[2]:
+                                gi = (org[0] + z, ay0 + tyy, ax0 + txx)
+                                assert all(org[d] <= gi[d] < org[d] + n[d] for d in range(3))
+                                o3[gi] = vals[ln]
+                    if z + 1 < Z1:
+                        ring[slot(z + R + 1)] = load(z + R + 1)
+    return out
+
+
 VARIANTS_3D = variants(3)
 
 
@@ -524,7 +637,7 @@ def test_variant_emulation_3d(v):
     layout = Layout(3, [v["st"] + 2 * R + 4, v["by"] + 2 * R + 5, v["bx"] // 2 + 2 * R + 9], [R + 1, R, R + 2],
                     [v["st"] + 3, v["by"] + 5, v["bx"] // 2 + 7])
     x = oracle.fill_data(layout, v["id"])
-    got = emulate_3d(v, layout, dense, x)
+    got = emulate_3d2(v, layout, dense, x) if v["mf"] == 8 else emulate_3d(v, layout, dense, x)
     offs, ww = oracle.dense_to_taps(3, R, dense)
     want = oracle.run(layout, offs, ww, x, 1)
     gi, wi = layout.interior(got), layout.interior(want)
