@@ -114,12 +114,74 @@ def split_comparison(seed, graph, n=8192, T=64):
     return res
 
 
+def batch_workload(args):
+    """256 independent 2-D grids, sides log-uniform in [128, 2048] (uneven, non-square),
+    stencil drawn from {box2d1r, star2d3r}, own seeded weights, zero halo of the radius,
+    T = 100; all grids advance in one persistent graph-captured launch per run, FP64 and
+    BF16x3.  Reports aggregate GStencil/s and, per grid, the error against the CPU oracle
+    (FP64: relative max; BF16x3: relative L2 and max)."""
+    import convstencil_amd as cs
+    from convstencil_amd import Batch, Layout, shape_offsets, dense_weights
+    from oracle import oracle
+    rng = np.random.default_rng(args.seed)
+    G, T = args.batch_grids, 100
+    sides = np.exp(rng.uniform(np.log(128), np.log(2048), (G, 2))).astype(int)
+    names = rng.choice(["box2d1r", "star2d3r"], G)
+    grids = []
+    for i in range(G):
+        r = 1 if names[i] == "box2d1r" else 3
+        L = Layout.with_halo([int(sides[i, 0]), int(sides[i, 1])], r)
+        offs = shape_offsets(names[i])
+        w = cs.fill_weights_host(len(offs), args.seed + i)
+        grids.append((L, r, offs, w, dense_weights(2, r, offs, w)))
+    pts = sum(g[0].points for g in grids)
+    seeds = [args.seed + 1000 + i for i in range(G)]
+    res = {}
+    finals = {}
+    for prec in ("fp64", "bf16x3"):
+        with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids], precision=prec,
+                   use_graph=True) as b:
+            b.fill_random(seeds)
+            rep = b.time(T, warmup=args.warmup, reps=args.steps)
+            b.fill_random(seeds)
+            b.run(T)
+            finals[prec] = [b.download(i) for i in range(G)]
+            res[prec] = {"gstencil_per_s": round(pts * T / (rep.elapsed_ms * 1e-3) / 1e9, 2),
+                         "ms_per_run": round(rep.elapsed_ms, 3), "describe": b.describe()}
+    threads = oracle.max_threads()
+    errs = {"fp64": [], "bf16x3": []}
+    t0 = time.perf_counter()
+    for i, (L, r, offs, w, dense) in enumerate(grids):
+        want = L.interior(oracle.run(L, offs, w, oracle.fill_data(L, seeds[i]), T, threads=threads))
+        for prec in ("fp64", "bf16x3"):
+            got = L.interior(finals[prec][i])
+            d = got - want
+            errs[prec].append((float(np.max(np.abs(d)) / np.max(np.abs(want))),
+                               float(np.sqrt(np.sum(d * d) / np.sum(want * want)))))
+    oracle_s = time.perf_counter() - t0
+    res["fp64"]["max_rel_max_err"] = max(e[0] for e in errs["fp64"])
+    res["bf16x3"]["max_rel_max_err"] = max(e[0] for e in errs["bf16x3"])
+    res["bf16x3"]["max_rel_l2_err"] = max(e[1] for e in errs["bf16x3"])
+    out = {
+        "metric": "GStencil/s at 1 GPU: FP64 MFMA vs split-precision at stated error; % roofline",
+        "value": res["fp64"]["gstencil_per_s"], "unit": "GStencil/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["fp64"]["ms_per_run"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (SplitMix64 seeded grids and weights)",
+        "config": {"workload": f"batch of {G} 2-D grids, sides log-uniform 128..2048, box2d1r/star2d3r, T={T}",
+                   "points": int(pts), "box2d1r_grids": int(np.sum(names == "box2d1r"))},
+        "batch": res,
+        "oracle_check": {"grids": G, "threads": threads, "seconds": round(oracle_s, 1)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="box2d1r", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="box2d1r", choices=sorted(WORKLOADS) + ["batch"])
+    ap.add_argument("--batch-grids", type=int, default=256)
     ap.add_argument("--fusion", type=int, default=0, help="fusion depth (0 = library default)")
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true")
@@ -149,6 +211,10 @@ def main():
         print("bench.py: this build runs on exactly one GPU; multi-GPU runs are not part of it "
               "(the sharding design is described in DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        batch_workload(args)
+        return
 
     import convstencil_amd as cs
     from convstencil_amd import Layout, Plan, shape_offsets, dense_weights, SHAPES

@@ -12,6 +12,9 @@
 //   --seed=S         seed of the synthetic grid and weights (physical mode)
 //   --fusion=K       fusion depth (0 = library default)
 //   --graph=0|1      capture the time loop in a hipGraph (default 1)
+//   --autotune=0|1|2 physical mode: 1 = the autotuner's cached choice (timed on a miss),
+//                    2 = time the variants again (default 0: library defaults)
+//   --precision=P    physical mode, 2-D: fp64 (default), bf16x3, fp32x2 or fp16x2
 //   --json           also print a JSON record (physical mode)
 //   --exact-fusion   2-D reference mode: compose the 3x3 weights exactly instead of
 //                    using the reference's 49-entry table
@@ -33,6 +36,8 @@ struct Ext {
   unsigned long long seed = 1234;
   int fusion = 0;
   int graph = 1;
+  int autotune = 0;
+  int precision = CS_PREC_FP64;
   bool json = false;
   bool exact_fusion = false;
 };
@@ -56,6 +61,18 @@ inline bool strip_flags(int &argc, char **argv, Ext &e) {
         e.fusion = std::stoi(v3);
       } else if (const char *v4 = val("--graph=")) {
         e.graph = std::stoi(v4);
+      } else if (const char *v5 = val("--autotune=")) {
+        e.autotune = std::stoi(v5);
+      } else if (const char *v6 = val("--precision=")) {
+        const std::string pn = v6;
+        if (pn == "fp64") e.precision = CS_PREC_FP64;
+        else if (pn == "bf16x3") e.precision = CS_PREC_BF16X3;
+        else if (pn == "fp32x2") e.precision = CS_PREC_FP32X2;
+        else if (pn == "fp16x2") e.precision = CS_PREC_FP16X2;
+        else {
+          std::cerr << "Invalid argument: unknown precision " << pn << ".\n";
+          return false;
+        }
       } else if (a == "--json") {
         e.json = true;
       } else if (a == "--exact-fusion") {
@@ -103,6 +120,8 @@ inline int run_physical(const char *tag, int ndim, int r, const std::vector<doub
   cs_options opt{};
   opt.fusion_depth = e.fusion;
   opt.use_graph = e.graph;
+  opt.autotune = e.autotune;
+  opt.precision = e.precision;
   cs_plan *p = nullptr;
   cs_status s = cs_plan_create(&p, &L, &st, &opt);
   if (s != CS_OK) {

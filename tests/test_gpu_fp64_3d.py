@@ -118,3 +118,40 @@ def test_reference_gpu_star_3d1r(gpu):
                 idx += 1
     want = oracle.run(L, offs, w, x, times)
     _check(L, out, want)
+
+
+def _variants_3d():
+    import ctypes
+    from convstencil_amd import _lib
+    lib = _lib.load()
+    lib.cs_debug_variant_count.restype = ctypes.c_int32
+    lib.cs_debug_variant_info.restype = ctypes.c_int32
+    lib.cs_debug_variant_info.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.c_char_p, ctypes.c_int32]
+    out = []
+    for i in range(lib.cs_debug_variant_count(3)):
+        info = (ctypes.c_int64 * 12)()
+        name = ctypes.create_string_buffer(128)
+        lib.cs_debug_variant_info(3, i, info, name, 128)
+        out.append((int(info[0]), int(info[2]), int(info[3]), name.value.decode()))
+    return out
+
+
+# radius-1 box and star variants of both families (the radius-1 diamond is the star)
+VARIANTS_3D = [v for v in _variants_3d() if v[1] == 1 and v[2] in (0, 1)]
+
+
+@pytest.mark.parametrize("vid,R,shape,vname", VARIANTS_3D, ids=[v[3] for v in VARIANTS_3D])
+def test_every_variant(gpu, vid, R, shape, vname):
+    print(f"case {vname}", flush=True)
+    name = "box3d1r" if shape == 0 else "star3d1r"
+    L = Layout.with_halo([37, 45, 140], 1)
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), vid)
+    x = oracle.fill_data(L, vid)
+    with Plan(L, 1, dense_weights(3, 1, offs, w), variant=vid) as p:
+        assert f"variant={vid}" in p.describe()
+        p.upload(x)
+        p.run(5)
+        got = p.download()
+    _check(L, got, oracle.run(L, offs, w, x, 5))
