@@ -66,7 +66,7 @@ Geo make_geo(const cs_layout &L, const Variant *v) {
 // cache, when present, overrides this).
 int default_kf(int ndim, int R, int top) {
   if (ndim == 1) return std::min(top, 32);
-  if (ndim == 3) return 1;
+  if (ndim == 3) return std::min(top, 2);
   return std::min(top, R == 1 ? 4 : 2);
 }
 

@@ -19,15 +19,14 @@ namespace cs {
 
 typedef double dbl2v __attribute__((ext_vector_type(2)));
 
-template <int R_, int SH_, int BY_, int BX_, int ZC_, int NW_>
-struct Cfg3D2 {
-  static constexpr int R = R_, SH = SH_, BY = BY_, BX = BX_, ZC = ZC_, NW = NW_;
+// Tables of a 3-D stencil on 8 x 8 blocks: plane dz's k-steps and LDS reads start at
+// kbase(dz) / rbase(dz) in the per-lane arrays b[] and off[].
+template <int R_, int SH_>
+struct Tab3D {
+  static constexpr int R = R_, SH = SH_;
   static constexpr DTable3 DT = make_dtable3(R, SH);
   static constexpr int PP = DT.pp;
   static constexpr int NKS = dtable3_ksteps(DT);
-  static constexpr int EY = BY + 2 * R;
-  static constexpr int EX = BX + 2 * R;
-  // k-step and LDS-read offsets of plane dz within b[] and off[]
   static constexpr int kbase(int dz) {
     int n = 0;
     for (int d = -R; d < dz; ++d) n += dtable_ksteps(DT.t[d + R]);
@@ -59,6 +58,20 @@ struct Cfg3D2 {
     for (int d = -R; d <= R; ++d) m = DT.t[d + R].tmax > m ? DT.t[d + R].tmax : m;
     return m;
   }
+};
+
+template <int R_, int SH_, int BY_, int BX_, int ZC_, int NW_>
+struct Cfg3D2 : Tab3D<R_, SH_> {
+  using T3 = Tab3D<R_, SH_>;
+  using T3::cmax;
+  using T3::cmin;
+  using T3::DT;
+  using T3::PP;
+  using T3::tmax;
+  using T3::tmin;
+  static constexpr int R = R_, SH = SH_, BY = BY_, BX = BX_, ZC = ZC_, NW = NW_;
+  static constexpr int EY = BY + 2 * R;
+  static constexpr int EX = BX + 2 * R;
   // region: rows [R, R + BY), columns [R, R + BX); tile column c is stored at c + CS and
   // blocks start at columns x with x + CS + PP even
   static constexpr int lox(int cs) { return R - ((R + cs + PP) & 1); }
@@ -228,6 +241,232 @@ __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ i
   }
 }
 
+// ---- temporal fusion: KF time levels streamed through z together ----
+//
+// Level 0 is the input; level l (1..KF) holds the state after l steps on a region
+// that shrinks by R per level (rows / columns [l R, E - l R) of the level-0 tile).
+// Levels 0 .. KF-1 each keep a ring of 2R+1 LDS planes.  Iteration p stores input
+// plane p (prefetched during the previous iteration) and then computes, level by
+// level, plane p - l R of level l from planes p - (l+1) R .. p - (l-1) R of level
+// l - 1; level KF writes output plane p - KF R to HBM.  A fused launch is exact
+// only for a zero halo (the plan fuses only then): halo planes and cells of every
+// level are 0, and the rings of levels >= 1 start zeroed, so the cells a level never
+// writes stay 0 (halo) or finite (outside its region, read only with zero weight).
+template <int R_, int SH_, int BY_, int BX_, int KF_, int ZC_, int NW_>
+struct Cfg3F : Tab3D<R_, SH_> {
+  using T3 = Tab3D<R_, SH_>;
+  using T3::cmax;
+  using T3::cmin;
+  using T3::DT;
+  using T3::PP;
+  using T3::tmax;
+  using T3::tmin;
+  static constexpr int R = R_, SH = SH_, BY = BY_, BX = BX_, KF = KF_, ZC = ZC_, NW = NW_;
+  static constexpr int EY = BY + 2 * KF * R;
+  static constexpr int EX = BX + 2 * KF * R;
+  static constexpr int LO(int l) { return R * l; }
+  static constexpr int H(int l) { return BY + 2 * R * (KF - l); }
+  static constexpr int HI(int l) { return LO(l) + BX + 2 * R * (KF - l); }
+  static constexpr int lox(int l, int cs) { return LO(l) - ((LO(l) + cs + PP) & 1); }
+  static constexpr int lastx(int l, int cs) { return (HI(l) - 8) + (((HI(l) - 8) + cs + PP) & 1); }
+  static constexpr int find_cs() {
+    for (int cs = 0; cs < 16; ++cs) {
+      bool ok = true;
+      for (int l = 1; l <= KF; ++l) ok = ok && lox(l, cs) + cmin() + cs >= 0;
+      if (ok) return cs;
+    }
+    return 16;
+  }
+  static constexpr int CS = find_cs();
+  static constexpr int LOX(int l) { return lox(l, CS); }
+  static constexpr int LASTX(int l) { return lastx(l, CS); }
+  static constexpr int NBY(int l) { return (H(l) + 7) / 8; }
+  static constexpr int NBX(int l) { return (LASTX(l) - LOX(l) + 7) / 8 + 1; }
+  static constexpr int NB(int l) { return NBY(l) * NBX(l); }
+  static constexpr int PERS(int l) { return (NB(l) + NW - 1) / NW; }
+  static constexpr int max_col() {
+    int m = EX - 1;
+    for (int l = 1; l <= KF; ++l) m = LASTX(l) + 6 + cmax() > m ? LASTX(l) + 6 + cmax() : m;
+    return m;
+  }
+  static constexpr int PX0 = CS + max_col() + 1;
+  static constexpr int PX = PX0 + ((4 - PX0 % 8) + 8) % 8;
+  static constexpr int PLANE = EY * PX;
+  static constexpr int NP = 2 * R + 1;
+  static constexpr int RING = NP * PLANE;
+  static constexpr int SPARE = KF * RING;
+  static constexpr size_t LDS = ((size_t)KF * RING + 2) * sizeof(double);
+  static constexpr int PF = (PLANE + NW * 64 - 1) / (NW * 64);
+  static_assert(KF >= 2, "the fused kernel runs two or more levels");
+  static_assert(BY % 8 == 0 && BX % 16 == 0, "tile must be a multiple of the 8 x 8 block");
+  static_assert(CS < 16 && PX % 2 == 0 && (LOX(1) + CS + PP) % 2 == 0, "pair reads must be aligned");
+  static_assert(LO(1) + tmin() >= 0 && LO(1) + H(1) - 8 + 6 + tmax() < EY, "block reads outside the plane");
+  static_assert(LDS <= 160 * 1024, "rings do not fit gfx950 LDS");
+};
+
+struct FusedCtx {
+  int tid, wave, orow, ocol;
+  bool full;
+  int64_t Y0, X0, ny, nx, nz, ay0, ax0, ext1, ext2, org0, Z0, Z1, base;
+};
+
+// Level L of iteration p (compile-time L), then the levels above it.
+template <class C, int L>
+__device__ __forceinline__ void fused_level(const FusedCtx &t, int64_t p, double *lds, double *out, const int *off,
+                                            const double *b) {
+  if constexpr (L <= C::KF) {
+    constexpr int R = C::R, NW = C::NW, PX = C::PX;
+    const int64_t q = p - (int64_t)L * R;
+    if (q >= t.Z0 - (int64_t)(C::KF - L) * R && q <= t.Z1 - 1 + (int64_t)(C::KF - L) * R) {
+      if (L < C::KF && (q < 0 || q >= t.nz)) {  // a halo plane of level L: zero
+        double *dst = lds + L * C::RING + (int)((q - t.base) % C::NP) * C::PLANE;
+        for (int idx = t.tid; idx < C::PLANE; idx += NW * 64) dst[idx] = 0.0;
+      } else {
+        int sl[C::NP];
+#pragma unroll
+        for (int d = 0; d < C::NP; ++d) sl[d] = __builtin_amdgcn_readfirstlane((int)((q - R + d - t.base) % C::NP));
+        const double *src = lds + (L - 1) * C::RING;
+        constexpr int NB = C::NB(L), PERS = C::PERS(L);
+        double acc[PERS];
+#pragma unroll
+        for (int j = 0; j < PERS; ++j) {
+          const int blk = min(j * NW + t.wave, NB - 1);
+          const int ba = blk / C::NBX(L), bb = blk - ba * C::NBX(L);
+          const int y = C::LO(L) + min(8 * ba, C::H(L) - 8), x = min(C::LOX(L) + 8 * bb, C::LASTX(L));
+          acc[j] = all_planes<C, -R>(src, sl, y * PX + x, off, b, 0.0);
+        }
+        double *dst = L < C::KF ? lds + L * C::RING + (int)((q - t.base) % C::NP) * C::PLANE : nullptr;
+#pragma unroll
+        for (int j = 0; j < PERS; ++j) {
+          const int blk = j * NW + t.wave;
+          if ((NB % NW == 0 || j < PERS - 1) || blk < NB) {
+            const int ba = blk / C::NBX(L), bb = blk - ba * C::NBX(L);
+            const int y = C::LO(L) + min(8 * ba, C::H(L) - 8), x = min(C::LOX(L) + 8 * bb, C::LASTX(L));
+            const int ty = y + t.orow, tx = x + t.ocol;
+            const bool in = tx >= C::LO(L) && tx < C::HI(L) &&
+                            (t.full || (t.Y0 + ty >= 0 && t.Y0 + ty < t.ny && t.X0 + tx >= 0 && t.X0 + tx < t.nx));
+            if (in) {
+              if constexpr (L < C::KF)
+                dst[ty * PX + tx + C::CS] = acc[j];
+              else
+                out[((t.org0 + q) * t.ext1 + t.ay0 + ty) * t.ext2 + t.ax0 + tx] = acc[j];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // level L of plane q is complete before level L + 1 reads it
+    fused_level<C, L + 1>(t, p, lds, out, off, b);
+  }
+}
+
+template <int R, int SH, int BY, int BX, int KF, int ZC, int NW>
+__global__ __launch_bounds__(NW * 64) void k3f_f64(const double *__restrict__ in, double *__restrict__ out,
+                                                   const double *__restrict__ bfrag, Geo g) {
+  using C = Cfg3F<R, SH, BY, BX, KF, ZC, NW>;
+  constexpr int PX = C::PX, NT = NW * 64;
+  extern __shared__ __align__(16) double lds[];
+  const int64_t tiles_yx = g.tiles[1] * g.tiles[2];
+  const int64_t ntiles = g.tiles[0] * tiles_yx;
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (tile >= ntiles) return;
+  const int64_t zc = tile / tiles_yx, rem = tile % tiles_yx;
+  FusedCtx t;
+  t.tid = threadIdx.x;
+  const int lane = t.tid & 63;
+  t.wave = __builtin_amdgcn_readfirstlane(t.tid >> 6);
+  t.ext1 = g.ext[1];
+  t.ext2 = g.ext[2];
+  t.org0 = g.org[0];
+  t.nz = g.n[0];
+  t.ny = g.n[1];
+  t.nx = g.n[2];
+  t.Y0 = (rem / g.tiles[2]) * BY - KF * R;
+  t.X0 = (rem % g.tiles[2]) * BX - KF * R;
+  t.ay0 = g.org[1] + t.Y0;
+  t.ax0 = g.org[2] + t.X0;
+  t.Z0 = zc * ZC;
+  t.Z1 = t.Z0 + ZC < t.nz ? t.Z0 + ZC : t.nz;
+  t.base = t.Z0 - (int64_t)KF * R;
+  t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
+  const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= t.ext1 && t.ax0 >= 0 && t.ax0 + C::EX <= t.ext2;
+  t.orow = 2 * ((lane >> 2) & 3) + ((lane & 3) >> 1);
+  t.ocol = 2 * (lane >> 4) + (lane & 1);
+
+  double b[C::NKS];
+#pragma unroll
+  for (int ks = 0; ks < C::NKS; ++ks) b[ks] = bfrag[ks * 64 + lane];
+  int off[C::NRD];
+  {
+    const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
+#pragma unroll
+    for (int dz = -R; dz <= R; ++dz) {
+      const DTable &T = C::DT.t[dz + R];
+      const int r0 = C::rbase(dz);
+#pragma unroll
+      for (int ps = 0; ps < T.np; ++ps) off[r0 + ps] = (2 * b4 + T.pty[ps][q]) * PX + 2 * m + T.ptx[ps][q] + C::CS;
+#pragma unroll
+      for (int ss = 0; ss < T.ns; ++ss)
+        off[r0 + T.np + ss] = (2 * b4 + T.sty[ss][q]) * PX + 2 * m + T.stx[ss][q] + C::CS;
+    }
+  }
+  // the rings of levels >= 1 start at zero
+  for (int idx = t.tid; idx < (KF - 1) * C::RING; idx += NT) lds[C::RING + idx] = 0.0;
+
+  // input plane p (interior z), zero outside the array
+  auto issue = [&](int64_t p, double *v) {
+    const bool plane_ok = t.org0 + p >= 0 && t.org0 + p < g.ext[0];
+#pragma unroll
+    for (int i = 0; i < C::PF; ++i) {
+      const int idx = t.tid + i * NT;
+      const int r = idx / PX, c = idx - r * PX - C::CS;
+      const bool ok = plane_ok && idx < C::PLANE && c >= 0 && c < C::EX &&
+                      (all_in || (t.ay0 + r >= 0 && t.ay0 + r < t.ext1 && t.ax0 + c >= 0 && t.ax0 + c < t.ext2));
+      const int64_t o = ((t.org0 + p) * t.ext1 + t.ay0 + r) * t.ext2 + t.ax0 + c;
+      const double x = *(ok ? in + o : in);
+      v[i] = ok ? x : 0.0;
+    }
+  };
+  auto store = [&](int64_t p, const double *v) {
+    double *dst = lds + (int)((p - t.base) % C::NP) * C::PLANE;
+#pragma unroll
+    for (int i = 0; i < C::PF; ++i) {
+      const int idx = t.tid + i * NT;
+      dst[(i < C::PF - 1 || idx < C::PLANE) ? idx : C::SPARE - (int)(dst - lds)] = v[i];
+    }
+  };
+
+  double pf[C::PF];
+  const int64_t p0 = t.base, p1 = t.Z1 - 1 + (int64_t)KF * R;
+  issue(p0, pf);
+  for (int64_t p = p0; p <= p1; ++p) {
+    store(p, pf);
+    __syncthreads();  // input plane p is in ring 0 (and, at p0, the zeroed rings are visible)
+    if (p < p1) issue(p + 1, pf);
+    fused_level<C, 1>(t, p, lds, out, off, b);
+  }
+}
+
+template <int R, int SH, int BY, int BX, int KF, int ZC, int NW>
+static hipError_t prepare_k3f() {
+  using C = Cfg3F<R, SH, BY, BX, KF, ZC, NW>;
+  return hipFuncSetAttribute((const void *)k3f_f64<R, SH, BY, BX, KF, ZC, NW>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
+}
+
+template <int R, int SH, int BY, int BX, int KF, int ZC, int NW>
+static hipError_t launch_k3f(const double *in, double *out, const double *bfrag, const Geo &g0, hipStream_t s) {
+  using C = Cfg3F<R, SH, BY, BX, KF, ZC, NW>;
+  Geo g = g0;
+  g.tiles[0] = (g.n[0] + ZC - 1) / ZC;
+  const int64_t ntiles = g.tiles[0] * g.tiles[1] * g.tiles[2];
+  const int64_t grid = ((ntiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((k3f_f64<R, SH, BY, BX, KF, ZC, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in, out,
+                     bfrag, g);
+  return hipGetLastError();
+}
+
 template <int R, int SH, int BY, int BX, int ZC, int NW>
 static hipError_t prepare_k3d2() {
   using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
@@ -255,12 +494,27 @@ static hipError_t launch_k3d2(const double *in, double *out, const double *bfrag
         "k3d2_f64<r" #R "," #SH "," #BY "x" #BX ",zc" #ZC ",w" #NW ">", Cfg3D2<R, SH, BY, BX, ZC, NW>::CS    \
   }
 
+// Fused family: same tables and B fragments (mf 8), fusion depth KF; st = z-chunk length.
+#define CS_V3F(ID, R, SH, BY, BX, KF, ZC, NW)                                                             \
+  Variant {                                                                                               \
+    ID, 8, R, SH, KF, BY, BX, NW, Cfg3F<R, SH, BY, BX, KF, ZC, NW>::LDS,                                  \
+        Cfg3F<R, SH, BY, BX, KF, ZC, NW>::PX, ZC, &launch_k3f<R, SH, BY, BX, KF, ZC, NW>,                 \
+        &prepare_k3f<R, SH, BY, BX, KF, ZC, NW>,                                                          \
+        "k3f_f64<r" #R "," #SH "," #BY "x" #BX ",kf" #KF ",zc" #ZC ",w" #NW ">",                          \
+        Cfg3F<R, SH, BY, BX, KF, ZC, NW>::CS                                                              \
+  }
+
 static const Variant kVariants3D2[] = {
     CS_V3D2(41101, 1, SHAPE3_BOX, 32, 64, 32, 8),   CS_V3D2(41111, 1, SHAPE3_STAR, 32, 64, 32, 8),
     CS_V3D2(41121, 1, SHAPE3_DIAMOND, 32, 64, 32, 8), CS_V3D2(41102, 1, SHAPE3_BOX, 16, 64, 32, 4),
     CS_V3D2(41112, 1, SHAPE3_STAR, 16, 64, 32, 4),  CS_V3D2(41103, 1, SHAPE3_BOX, 32, 64, 64, 8),
     CS_V3D2(41113, 1, SHAPE3_STAR, 32, 64, 64, 8),  CS_V3D2(41104, 1, SHAPE3_BOX, 16, 128, 32, 8),
     CS_V3D2(41114, 1, SHAPE3_STAR, 16, 128, 32, 8),
+    // fused (ids 42000 + shape * 10 + KF, then tile alternatives)
+    CS_V3F(42002, 1, SHAPE3_BOX, 16, 64, 2, 64, 8),  CS_V3F(42012, 1, SHAPE3_STAR, 16, 64, 2, 64, 8),
+    CS_V3F(42003, 1, SHAPE3_BOX, 16, 64, 3, 64, 8),  CS_V3F(42013, 1, SHAPE3_STAR, 16, 64, 3, 64, 8),
+    CS_V3F(42102, 1, SHAPE3_BOX, 16, 64, 2, 64, 4),  CS_V3F(42112, 1, SHAPE3_STAR, 16, 64, 2, 64, 4),
+    CS_V3F(42202, 1, SHAPE3_BOX, 8, 128, 2, 64, 4),  CS_V3F(42212, 1, SHAPE3_STAR, 8, 128, 2, 64, 4),
 };
 
 const Variant *variants_3d2_f64(int *count) {

@@ -618,6 +618,106 @@ def emulate_3d2(v, layout, dense, x):
     return out
 
 
+def emulate_3f(v, layout, dense, x):
+    """The fused 3-D kernel: KF levels streamed through z, a ring of 2R+1 planes per level."""
+    R, BY, BX, ZC, KF = v["R"], v["by"], v["bx"], v["st"], v["kf"]
+    tabs = {dz: dtable3(R, v["shape"], dz) for dz in range(-R, R + 1)}
+    pp = tabs[0][2]
+    bf = bfrag3d2(R, v["shape"], dense)
+    EY, EX, PX, CS = BY + 2 * KF * R, BX + 2 * KF * R, v["px"], v["cs"]
+    assert PX % 8 == 4
+    PLANE = EY * PX
+    NP = 2 * R + 1
+    assert v["lds"] == (KF * NP * PLANE + 2) * 8 and v["lds"] <= 160 * 1024
+    lanes = np.arange(64)
+    q4, b4, m4 = lanes >> 4, (lanes >> 2) & 3, lanes & 3
+    orow, ocol = 2 * b4 + ((lanes & 3) >> 1), 2 * (lanes >> 4) + (lanes & 1)
+
+    def geo(l):
+        LO, H = R * l, BY + 2 * R * (KF - l)
+        HI = LO + BX + 2 * R * (KF - l)
+        LOX = LO - ((LO + CS + pp) & 1)
+        LASTX = (HI - 8) + (((HI - 8) + CS + pp) & 1)
+        return LO, H, HI, LOX, LASTX, -(-H // 8), (LASTX - LOX + 7) // 8 + 1
+    ext, org, n = layout.ext, layout.org, layout.n
+    x3 = x.reshape(ext)
+    out = np.zeros(layout.cells)
+    o3 = out.reshape(ext)
+    tiles = (-(-n[0] // ZC), -(-n[1] // BY), -(-n[2] // BX))
+    for zc in range(tiles[0]):
+        for ty_ in range(tiles[1]):
+            for tx_ in range(tiles[2]):
+                Y0, X0 = ty_ * BY - KF * R, tx_ * BX - KF * R
+                ay0, ax0 = org[1] + Y0, org[2] + X0
+                Z0, Z1 = zc * ZC, min(zc * ZC + ZC, n[0])
+                base = Z0 - KF * R
+                full = Y0 >= 0 and X0 >= 0 and Y0 + EY <= n[1] and X0 + EX <= n[2]
+                rings = np.full((KF, NP, PLANE), np.nan)
+                rings[1:] = 0.0
+                slot = lambda p: (p - base) % NP
+
+                def load(p):
+                    pl = np.zeros(PLANE)
+                    az = org[0] + p
+                    if not (0 <= az < ext[0]):
+                        return pl
+                    for idx in range(PLANE):
+                        r, c = divmod(idx, PX)
+                        c -= CS
+                        ay, ax = ay0 + r, ax0 + c
+                        if 0 <= c < EX and 0 <= ay < ext[1] and 0 <= ax < ext[2]:
+                            pl[idx] = x3[az, ay, ax]
+                    return pl
+
+                def inside(ty, tx):
+                    return full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
+                for p in range(base, Z1 - 1 + KF * R + 1):
+                    rings[0, slot(p)] = load(p)
+                    for l in range(1, KF + 1):
+                        qz = p - l * R
+                        if not (Z0 - (KF - l) * R <= qz <= Z1 - 1 + (KF - l) * R):
+                            continue
+                        if l < KF and (qz < 0 or qz >= n[0]):
+                            rings[l, slot(qz)] = 0.0
+                            continue
+                        LO, H, HI, LOX, LASTX, NBY, NBX = geo(l)
+                        writes = []
+                        for blk in range(NBY * NBX):
+                            ba, bb = divmod(blk, NBX)
+                            y, xx = LO + min(8 * ba, H - 8), min(LOX + 8 * bb, LASTX)
+                            acc = np.zeros((4, 4, 4))
+                            ks = 0
+                            for dz in range(-R, R + 1):
+                                npair, ns, _, tty, ttx, tlive = tabs[dz]
+                                pl = rings[l - 1, slot(qz + dz)]
+                                for e in range(npair + ns):
+                                    idx = (y + 2 * b4 + tty[e][q4]) * PX + xx + 2 * m4 + ttx[e][q4] + CS
+                                    assert idx.min() >= 0 and idx.max() + (1 if e < npair else 0) < PLANE
+                                    assert e >= npair or np.all(idx % 2 == 0), "misaligned pair read"
+                                    for ci in ([idx, idx + 1] if e < npair else [idx]):
+                                        a = pl[ci]
+                                        assert np.isfinite(a).all(), (v["name"], "reads an unset LDS cell")
+                                        A = np.zeros((4, 4, 4))
+                                        B = np.zeros((4, 4, 4))
+                                        A[b4, m4, q4] = a
+                                        B[b4, q4, lanes & 3] = bf[ks]
+                                        acc += A @ B
+                                        ks += 1
+                            vals = acc[b4, lanes >> 4, lanes & 3]
+                            for ln in range(64):
+                                tyy, txx = y + orow[ln], xx + ocol[ln]
+                                if LO <= txx < HI and inside(tyy, txx):
+                                    writes.append((tyy, txx, vals[ln]))
+                        for tyy, txx, val in writes:
+                            if l < KF:
+                                rings[l, slot(qz), tyy * PX + txx + CS] = val
+                            else:
+                                gi = (org[0] + qz, ay0 + tyy, ax0 + txx)
+                                assert all(org[d] <= gi[d] < org[d] + n[d] for d in range(3))
+                                o3[gi] = val
+    return out
+
+
 VARIANTS_3D = variants(3)
 
 
@@ -637,9 +737,14 @@ def test_variant_emulation_3d(v):
     layout = Layout(3, [v["st"] + 2 * R + 4, v["by"] + 2 * R + 5, v["bx"] // 2 + 2 * R + 9], [R + 1, R, R + 2],
                     [v["st"] + 3, v["by"] + 5, v["bx"] // 2 + 7])
     x = oracle.fill_data(layout, v["id"])
-    got = emulate_3d2(v, layout, dense, x) if v["mf"] == 8 else emulate_3d(v, layout, dense, x)
+    if v["kf"] > 1:  # fused kernels run with a zero halo only
+        m = layout.interior_mask()
+        x = np.where(m, x, 0.0)
+        got = emulate_3f(v, layout, dense, x)
+    else:
+        got = emulate_3d2(v, layout, dense, x) if v["mf"] == 8 else emulate_3d(v, layout, dense, x)
     offs, ww = oracle.dense_to_taps(3, R, dense)
-    want = oracle.run(layout, offs, ww, x, 1)
+    want = oracle.run(layout, offs, ww, x, v["kf"])
     gi, wi = layout.interior(got), layout.interior(want)
     err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
     assert err < 1e-13, (v["name"], err)
