@@ -35,6 +35,9 @@ WORKLOADS = {
     "box2d1r": ("box2d1r", [10240, 10240], 1, 64),
     "star2d3r": ("star2d3r", [8192, 8192], 3, 64),
     "box2d3r": ("box2d3r", [8192, 8192], 3, 64),
+    "box3d1r": ("box3d1r", [512, 512, 512], 1, 32),
+    "star3d1r": ("star3d1r", [512, 512, 512], 1, 32),
+    "1d1r": ("1d1r", [1 << 20], 1, 64),
 }
 
 
@@ -67,8 +70,14 @@ def cpu_baseline(layout, offs, w, seed, budget_s):
     oracle.run(layout, offs, w, x, steps, threads=threads)
     el = time.perf_counter() - t0
     pts = layout.points
-    return {"value": pts * steps / el / 1e9, "unit": "GStencil/s", "cores": threads, "kind": "port",
-            "sample": f"full grid, {steps} time step(s) of the same stencil, {el:.1f} s, OpenMP {threads} threads"}
+    res = {"value": pts * steps / el / 1e9, "unit": "GStencil/s", "cores": threads, "kind": "port",
+           "sample": f"full grid, {steps} time step(s) of the same stencil, {el:.1f} s, OpenMP {threads} threads"}
+    # single thread, as the reference's loop runs (no OpenMP in the reference): 2 steps
+    t0 = time.perf_counter()
+    oracle.run(layout, offs, w, x, 2, threads=1)
+    el1 = time.perf_counter() - t0
+    res["single_thread"] = {"value": pts * 2 / el1 / 1e9, "cores": 1, "sample": f"full grid, 2 steps, {el1:.1f} s"}
+    return res
 
 
 def unfused_hbm(L, r, dense, T, seed, graph):

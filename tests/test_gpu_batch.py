@@ -100,6 +100,8 @@ def test_batch_device_fill_and_time(gpu):
 
 
 def test_batch_rejects_unsupported(gpu):
-    L, r, offs, w, dense, x = _grid("box2d2r", [40, 40], 1)
+    # a radius-2 box has no batch tile body: creation fails with a status, no launch
+    L = Layout.with_halo([40, 40], 2)
+    dense = np.full(25, 1.0 / 25)
     with pytest.raises(RuntimeError):
-        Batch([L], [r], [dense])
+        Batch([L], [2], [dense])
