@@ -30,6 +30,103 @@ static hipError_t launch_k2d(const double *in, double *out, const double *bfrag,
   return hipGetLastError();
 }
 
+// ---- persistent form: a fixed grid walks the tiles, prefetching the next one ----
+//
+// As many workgroups as fit on the device at once (a multiple of 8, so that each stays
+// on one XCD's band of tiles) take tiles w, w + G, w + 2G, ... in the XCD-aware order;
+// while a workgroup computes one tile, all loads of its next tile are in flight in
+// registers, so HBM traffic overlaps the MFMAs and barriers of the fused steps.
+
+// Next position >= k (step G) whose XCD-ordered tile exists; returns total if none.
+__device__ __forceinline__ int64_t next_tile_pos(int64_t k, int64_t G, int64_t total, int64_t ntiles) {
+  while (k < total && xcd_tile(k, ntiles) >= ntiles) k += G;
+  return k;
+}
+
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+__global__ __launch_bounds__(NW * 64) void k2d_f64p(const double *__restrict__ in, double *__restrict__ out,
+                                                    const double *__restrict__ bfrag, Geo g) {
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
+  extern __shared__ __align__(16) double lds[];
+  const int64_t ntiles = g.tiles[1] * g.tiles[2];
+  const int64_t total = ((ntiles + 7) / 8) * 8, G = gridDim.x;
+  int64_t k = next_tile_pos(blockIdx.x, G, total, ntiles);
+  if (k >= total) return;
+  TileCtx t;
+  double b[C::NKS];
+  int off[C::NOFF];
+  lane_setup<C>(bfrag, threadIdx.x, t, b, off);
+  TileRegs<C> rg;
+  tile_setup<C>(g, xcd_tile(k, ntiles), t);
+  issue_tile<C>(in, g, t, rg);
+  for (;;) {
+    store_tile<C>(t, rg, lds);
+    __syncthreads();
+    const int64_t kn = next_tile_pos(k + G, G, total, ntiles);
+    // opaque copies of the thread and wave indices: keep the compiler from hoisting the
+    // per-element load addresses and every block's addresses out of the tile loop
+    // (they would all stay live in registers)
+    int tid = t.tid, w = t.wave;
+    asm volatile("" : "+v"(tid), "+s"(w));
+    t.tid = tid;
+    t.wave = w;
+    TileCtx tn = t;
+    if (kn < total) {
+      tile_setup<C>(g, xcd_tile(kn, ntiles), tn);
+      issue_tile<C>(in, g, tn, rg);  // in flight while this tile is computed
+    }
+    fused_steps<C, 0>(t, b, off, lds, out);
+    if (kn >= total) break;
+    __syncthreads();  // every wave is done with this tile's LDS before the next tile is stored
+    k = kn;
+    t = tn;
+  }
+}
+
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+static int &k2dp_grid() {
+  static int grid = 0;
+  return grid;
+}
+
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+static hipError_t prepare_k2dp() {
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
+  const void *fn = (const void *)k2d_f64p<MF, R, SH, KF, BY, BX, NW, ST>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
+  if (e != hipSuccess) return e;
+  int per_cu = 0, dev = 0, cus = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NW * 64, C::LDS);
+  if (e != hipSuccess) return e;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  k2dp_grid<MF, R, SH, KF, BY, BX, NW, ST>() = ((per_cu > 0 ? per_cu : 1) * cus) / 8 * 8;
+  return hipSuccess;
+}
+
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+static hipError_t launch_k2dp(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s) {
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
+  const int64_t total = ((g.tiles[1] * g.tiles[2] + 7) / 8) * 8;
+  int64_t grid = k2dp_grid<MF, R, SH, KF, BY, BX, NW, ST>();
+  if (grid <= 0) return hipErrorNotInitialized;  // prepare() sizes the grid
+  if (grid > total) grid = total;
+  hipLaunchKernelGGL((k2d_f64p<MF, R, SH, KF, BY, BX, NW, ST>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in,
+                     out, bfrag, g);
+  return hipGetLastError();
+}
+
+#define CS_V2DP(ID, MF, R, SH, KF, BY, BX, NW, ST)                                             \
+  Variant {                                                                                    \
+    ID, MF, R, SH, KF, BY, BX, NW, Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>::LDS,                   \
+        Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>::PX, ST, &launch_k2dp<MF, R, SH, KF, BY, BX, NW, ST>, \
+        &prepare_k2dp<MF, R, SH, KF, BY, BX, NW, ST>,                                          \
+        "k2d_f64p<mf" #MF ",r" #R "," #SH ",kf" #KF "," #BY "x" #BX ",w" #NW ",st" #ST ">",   \
+        Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>::CS                                                \
+  }
+
 #define CS_V2D(ID, MF, R, SH, KF, BY, BX, NW, ST)                                            \
   Variant {                                                                                  \
     ID, MF, R, SH, KF, BY, BX, NW, Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>::LDS,                 \
@@ -40,7 +137,8 @@ static hipError_t launch_k2d(const double *in, double *out, const double *bfrag,
   }
 
 // Variant ids: ST * 10000 + R * 1000 + (MF == 4) * 100 + (SH == star) * 10 + KF % 10, plus
-// 100000 for alternative tile shapes and depths 6..9, 200000 for depths 10..19.  The host
+// 100000 for alternative tile shapes and depths 6..9, 200000 for depths 10..19, and
+// 500000 more for the persistent form.  The host
 // takes the first entry that matches (MFMA shape, radius, shape class, fusion depth), so
 // preferred variants come first.
 static const Variant kVariants[] = {
@@ -89,6 +187,23 @@ static const Variant kVariants[] = {
     CS_V2D(111204, 4, 1, SHAPE_BOX, 4, 64, 64, 4, 1),
     CS_V2D(111208, 4, 1, SHAPE_BOX, 8, 64, 64, 4, 1),
     CS_V2D(113202, 4, 3, SHAPE_BOX, 2, 64, 64, 4, 1),
+    // persistent form (ids 500000 + the id of the same configuration)
+    CS_V2DP(511101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 1),
+    CS_V2DP(511102, 4, 1, SHAPE_BOX, 2, 64, 64, 8, 1),
+    CS_V2DP(511104, 4, 1, SHAPE_BOX, 4, 64, 64, 8, 1),
+    CS_V2DP(611106, 4, 1, SHAPE_BOX, 6, 64, 64, 8, 1),
+    CS_V2DP(611108, 4, 1, SHAPE_BOX, 8, 64, 64, 8, 1),
+    CS_V2DP(611204, 4, 1, SHAPE_BOX, 4, 64, 64, 4, 1),
+    CS_V2DP(511114, 4, 1, SHAPE_STAR, 4, 64, 64, 8, 1),
+    CS_V2DP(611116, 4, 1, SHAPE_STAR, 6, 64, 64, 8, 1),
+    CS_V2DP(611118, 4, 1, SHAPE_STAR, 8, 64, 64, 8, 1),
+    CS_V2DP(512102, 4, 2, SHAPE_BOX, 2, 64, 64, 8, 1),
+    CS_V2DP(513101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 1),
+    CS_V2DP(513102, 4, 3, SHAPE_BOX, 2, 64, 64, 8, 1),
+    CS_V2DP(513103, 4, 3, SHAPE_BOX, 3, 64, 64, 8, 1),
+    CS_V2DP(513112, 4, 3, SHAPE_STAR, 2, 64, 64, 8, 1),
+    CS_V2DP(513113, 4, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
+    CS_V2DP(513114, 4, 3, SHAPE_STAR, 4, 64, 64, 8, 1),
 };
 
 const Variant *variants_2d_f64(int *count) {

@@ -116,7 +116,7 @@ typedef double dbl2 __attribute__((ext_vector_type(2)));
 struct TileCtx {
   int tid, lane, wave;
   int orow, ocol;  // 4x4x4: the output cell of this lane relative to the block
-  bool full;       // the whole tile (with halo) lies inside the domain interior
+  int full;        // 1 if the whole tile (with halo) lies inside the domain interior
   int64_t Y0, X0, ny, nx, ay0, ax0, pitch;
 };
 
@@ -259,6 +259,86 @@ __device__ __forceinline__ void load_tile(const double *__restrict__ in, const G
   }
 }
 
+// The tile's loads, all issued into v[] (one HBM round trip), and their LDS stores:
+// used by the persistent kernel to keep the next tile's loads in flight while the
+// current tile is computed.
+template <class C>
+struct TileRegs {
+  static constexpr int NT = C::NW * 64;
+  static constexpr int NIT = (C::BUF + NT - 1) / NT;
+  double v[NIT];
+};
+
+template <class C>
+__device__ __forceinline__ void issue_tile(const double *__restrict__ in, const Geo &g, const TileCtx &t,
+                                           TileRegs<C> &rg) {
+  constexpr int NT = TileRegs<C>::NT, NIT = TileRegs<C>::NIT;
+  const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
+  const double *base = in + t.ay0 * t.pitch + t.ax0;
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int idx = t.tid + i * NT;
+    const int r = idx / C::PX, c = idx - r * C::PX - C::CS;
+    const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
+    const bool ok = idx < C::BUF && c >= 0 && c < C::EX && (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch));
+    const double x = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
+    rg.v[i] = ok ? x : 0.0;
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void store_tile(const TileCtx &t, const TileRegs<C> &rg, double *__restrict__ lds) {
+  constexpr int NT = TileRegs<C>::NT, NIT = TileRegs<C>::NIT;
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int idx = t.tid + i * NT;
+    const bool in_buf = i < NIT - 1 || idx < C::BUF;
+    lds[in_buf ? idx : C::SPARE] = rg.v[i];
+    if constexpr (C::NBUF == 2) lds[in_buf ? C::BUF + idx : C::SPARE] = rg.v[i];
+  }
+}
+
+// Per-lane constants of a kernel: B fragments, A offsets, output cell of the lane.
+template <class C>
+__device__ __forceinline__ void lane_setup(const double *__restrict__ bfrag, int tid, TileCtx &t, double *b,
+                                           int *off) {
+  constexpr int PX = C::PX;
+  t.tid = tid;
+  t.lane = tid & 63;
+  t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = t.lane;
+#pragma unroll
+  for (int ks = 0; ks < C::NKS; ++ks) b[ks] = bfrag[ks * 64 + lane];
+  if constexpr (C::MF == 16) {
+#pragma unroll
+    for (int ks = 0; ks < C::NKS; ++ks) off[ks] = ((lane & 15) + C::KT.dy[ks]) * PX + C::KT.c0[ks] + (lane >> 4);
+    t.orow = 0;
+    t.ocol = 0;
+  } else {
+    const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
+#pragma unroll
+    for (int ps = 0; ps < C::DT.np; ++ps) off[ps] = (2 * b4 + C::DT.pty[ps][q]) * PX + 2 * m + C::DT.ptx[ps][q] + C::CS;
+#pragma unroll
+    for (int ss = 0; ss < C::DT.ns; ++ss)
+      off[C::DT.np + ss] = (2 * b4 + C::DT.sty[ss][q]) * PX + 2 * m + C::DT.stx[ss][q] + C::CS;
+    t.orow = 2 * b4 + ((lane & 3) >> 1);
+    t.ocol = 2 * (lane >> 4) + (lane & 1);
+  }
+}
+
+// Per-tile fields of the context.
+template <class C>
+__device__ __forceinline__ void tile_setup(const Geo &g, int64_t tile, TileCtx &t) {
+  t.pitch = g.ext[2];
+  t.ny = g.n[1];
+  t.nx = g.n[2];
+  t.Y0 = (tile / g.tiles[2]) * C::BY - C::KF * C::R;
+  t.X0 = (tile % g.tiles[2]) * C::BX - C::KF * C::R;
+  t.ay0 = g.org[1] + t.Y0;
+  t.ax0 = g.org[2] + t.X0;
+  t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
+}
+
 // One workgroup advances tile `tile` of grid g by KF steps (lds: the kernel's dynamic LDS;
 // LG: tile loads in flight per thread; tid: threadIdx.x).
 template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST, int LG = 64>
@@ -266,47 +346,11 @@ __device__ __forceinline__ void tile_f64(const double *__restrict__ in, double *
                                          const double *__restrict__ bfrag, const Geo &g, int64_t tile,
                                          double *__restrict__ lds, int tid) {
   using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
-  constexpr int NKS = C::NKS;
-  constexpr int PX = C::PX;
-  constexpr int CS = C::CS;
-
   TileCtx t;
-  t.tid = tid;
-  t.lane = tid & 63;
-  t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  t.pitch = g.ext[2];
-  t.ny = g.n[1];
-  t.nx = g.n[2];
-  // interior coordinates of tile cell (0, 0)
-  t.Y0 = (tile / g.tiles[2]) * BY - KF * R;
-  t.X0 = (tile % g.tiles[2]) * BX - KF * R;
-  t.ay0 = g.org[1] + t.Y0;
-  t.ax0 = g.org[2] + t.X0;
-  t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
-  const int lane = t.lane;
-
-  double b[NKS];
-#pragma unroll
-  for (int ks = 0; ks < NKS; ++ks) b[ks] = bfrag[ks * 64 + lane];
-
-  // Per-lane LDS offsets of the A operand relative to the block's (y, x) cell.
+  double b[C::NKS];
   int off[C::NOFF];
-  if constexpr (MF == 16) {
-#pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) off[ks] = ((lane & 15) + C::KT.dy[ks]) * PX + C::KT.c0[ks] + (lane >> 4);
-    t.orow = 0;
-    t.ocol = 0;
-  } else {
-    const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
-#pragma unroll
-    for (int ps = 0; ps < C::DT.np; ++ps) off[ps] = (2 * b4 + C::DT.pty[ps][q]) * PX + 2 * m + C::DT.ptx[ps][q] + CS;
-#pragma unroll
-    for (int ss = 0; ss < C::DT.ns; ++ss)
-      off[C::DT.np + ss] = (2 * b4 + C::DT.sty[ss][q]) * PX + 2 * m + C::DT.stx[ss][q] + CS;
-    // D_b[m][n] in lane 16 m + 4 b + n is cell (2 b + (n >> 1), 2 m + (n & 1))
-    t.orow = 2 * b4 + ((lane & 3) >> 1);
-    t.ocol = 2 * (lane >> 4) + (lane & 1);
-  }
+  lane_setup<C>(bfrag, tid, t, b, off);
+  tile_setup<C>(g, tile, t);
 
   // ---- load the tile (and its halo); cells outside the array and pad columns are 0 ----
   load_tile<C, LG>(in, g, t, lds);

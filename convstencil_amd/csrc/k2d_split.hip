@@ -44,6 +44,90 @@ static hipError_t launch_ks2(const double *in, double *out, const double *bfrag,
   return hipGetLastError();
 }
 
+// Persistent dilated form: a fixed grid (a multiple of 8) walks the tiles in the
+// XCD-aware order, keeping the next tile's loads in flight while computing the current.
+__device__ __forceinline__ int64_t next_tile_pos_s(int64_t k, int64_t G, int64_t total, int64_t ntiles) {
+  while (k < total && (k % 8) * ((ntiles + 7) / 8) + k / 8 >= ntiles) k += G;
+  return k;
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+__global__ __launch_bounds__(NW * 64) void k2d_split2p(const double *__restrict__ in, double *__restrict__ out,
+                                                       const typename Tr::T *__restrict__ bfrag, Geo g) {
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  using T = typename Tr::T;
+  using V = typename Tr::V;
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *lds = reinterpret_cast<T *>(smem);
+  const int64_t ntiles = g.tiles[1] * g.tiles[2];
+  const int64_t per = (ntiles + 7) / 8, total = per * 8, G = gridDim.x;
+  int64_t k = next_tile_pos_s(blockIdx.x, G, total, ntiles);
+  if (k >= total) return;
+  SplitCtx t;
+  V w[C::NSW][C::NINS];
+  int off[C::NINS];
+  split_lane_setup<C>(bfrag, threadIdx.x, t, w, off);
+  SplitRegs<C> rg;
+  split_tile_setup<C>(g, (k % 8) * per + k / 8, t);
+  issue_tile_split<C>(in, g, t, rg);
+  for (;;) {
+    store_tile_split<C>(t, rg, lds);
+    __syncthreads();
+    const int64_t kn = next_tile_pos_s(k + G, G, total, ntiles);
+    // opaque copies of the thread and wave indices: keep the compiler from hoisting the
+    // per-element and per-block addresses out of the tile loop
+    int tid = t.tid, wv = t.wave;
+    asm volatile("" : "+v"(tid), "+s"(wv));
+    t.tid = tid;
+    t.wave = wv;
+    SplitCtx tn = t;
+    if (kn < total) {
+      split_tile_setup<C>(g, (kn % 8) * per + kn / 8, tn);
+      issue_tile_split<C>(in, g, tn, rg);  // in flight while this tile is computed
+    }
+    split_steps<C, 0>(t, w, off, lds, out);
+    if (kn >= total) break;
+    __syncthreads();  // every wave is done with this tile's LDS before the next tile is stored
+    k = kn;
+    t = tn;
+  }
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+static int &ks2p_grid() {
+  static int grid = 0;
+  return grid;
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+static hipError_t prepare_ks2p() {
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  const void *fn = (const void *)k2d_split2p<Tr, R, SH, KF, BY, BX, NW>;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
+  if (e != hipSuccess) return e;
+  int per_cu = 0, dev = 0, cus = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NW * 64, C::LDS);
+  if (e != hipSuccess) return e;
+  e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  ks2p_grid<Tr, R, SH, KF, BY, BX, NW>() = ((per_cu > 0 ? per_cu : 1) * cus) / 8 * 8;
+  return hipSuccess;
+}
+
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+static hipError_t launch_ks2p(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s) {
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  const int64_t total = ((g.tiles[1] * g.tiles[2] + 7) / 8) * 8;
+  int64_t grid = ks2p_grid<Tr, R, SH, KF, BY, BX, NW>();
+  if (grid <= 0) return hipErrorNotInitialized;  // prepare() sizes the grid
+  if (grid > total) grid = total;
+  hipLaunchKernelGGL((k2d_split2p<Tr, R, SH, KF, BY, BX, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in,
+                     out, reinterpret_cast<const typename Tr::T *>(bfrag), g);
+  return hipGetLastError();
+}
+
 template <class Tr, int R, int KF, int BY, int BX, int NW>
 static hipError_t prepare_ks() {
   using C = CfgS<Tr, R, KF, BY, BX, NW>;
@@ -79,6 +163,15 @@ static hipError_t launch_ks(const double *in, double *out, const double *bfrag, 
         CfgS2<TR, R, SH, KF, BY, BX, NW>::CS                                                            \
   }
 
+#define CS_VS2P(ID, PREC, TR, R, SH, KF, BY, BX, NW)                                                    \
+  Variant {                                                                                             \
+    ID, PREC, R, SH, KF, BY, BX, NW, CfgS2<TR, R, SH, KF, BY, BX, NW>::LDS,                             \
+        CfgS2<TR, R, SH, KF, BY, BX, NW>::PX, 2, &launch_ks2p<TR, R, SH, KF, BY, BX, NW>,              \
+        &prepare_ks2p<TR, R, SH, KF, BY, BX, NW>,                                                       \
+        "k2d_split2p<" #TR ",r" #R "," #SH ",kf" #KF "," #BY "x" #BX ",w" #NW ">",                      \
+        CfgS2<TR, R, SH, KF, BY, BX, NW>::CS                                                            \
+  }
+
 static const Variant kVariantsSplit[] = {
     CS_VS2(61101, 1, TrBF16, 1, SHAPE_BOX, 1, 64, 64, 8),
     CS_VS2(61102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8),
@@ -104,6 +197,16 @@ static const Variant kVariantsSplit[] = {
     CS_VS2(63304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8),
     CS_VS2(63311, 3, TrF16, 3, SHAPE_STAR, 1, 64, 64, 8),
     CS_VS2(63312, 3, TrF16, 3, SHAPE_STAR, 2, 64, 64, 8),
+    // persistent dilated form (ids 500000 + the id of the same configuration)
+    CS_VS2P(561102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2P(561104, 1, TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2P(563102, 1, TrBF16, 3, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2P(563103, 1, TrBF16, 3, SHAPE_BOX, 3, 64, 64, 8),
+    CS_VS2P(563104, 1, TrBF16, 3, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2P(563112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8),
+    CS_VS2P(563113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8),
+    CS_VS2P(563303, 3, TrF16, 3, SHAPE_BOX, 3, 64, 64, 8),
+    CS_VS2P(563304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8),
     // Toeplitz form
 
     CS_VS(51101, 1, TrBF16, 1, 1, 64, 64, 8), CS_VS(51102, 1, TrBF16, 1, 2, 64, 64, 8),

@@ -100,7 +100,7 @@ struct CfgS2 {
 struct SplitCtx {
   int tid, lane, wave;
   int orow, ocol;  // output cells of this lane relative to a block: row orow, columns ocol .. ocol + 3
-  bool full;
+  int full;        // 1 if the whole tile (with halo) lies inside the domain interior
   int64_t Y0, X0, ny, nx, ay0, ax0, pitch;
 };
 
@@ -139,6 +139,81 @@ __device__ __forceinline__ void load_tile_split(const double *__restrict__ in, c
       for (int k = 0; k < C::NSU; ++k) lds[in_plane ? k * C::PLANE + idx : C::SPARE] = s[k];
     }
   }
+}
+
+// The tile's FP64 loads, all issued into v[] (used by the persistent kernel to keep the
+// next tile's loads in flight), and their split into LDS slices.
+template <class C>
+struct SplitRegs {
+  static constexpr int NT = C::NW * 64;
+  static constexpr int NIT = (C::PLANE + NT - 1) / NT;
+  double v[NIT];
+};
+
+template <class C>
+__device__ __forceinline__ void issue_tile_split(const double *__restrict__ in, const Geo &g, const SplitCtx &t,
+                                                 SplitRegs<C> &rg) {
+  constexpr int NT = SplitRegs<C>::NT, NIT = SplitRegs<C>::NIT;
+  const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
+  const double *base = in + t.ay0 * t.pitch + t.ax0;
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int idx = t.tid + i * NT;
+    const int r = idx / C::PX, c = idx - r * C::PX - C::CS;
+    const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
+    const bool ok = idx < C::PLANE && c >= 0 && c < C::EX &&
+                    (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch));
+    const double x = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
+    rg.v[i] = ok ? x : 0.0;
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void store_tile_split(const SplitCtx &t, const SplitRegs<C> &rg,
+                                                 typename C::T *__restrict__ lds) {
+  using T = typename C::T;
+  constexpr int NT = SplitRegs<C>::NT, NIT = SplitRegs<C>::NIT;
+#pragma unroll
+  for (int i = 0; i < NIT; ++i) {
+    const int idx = t.tid + i * NT;
+    const bool in_plane = i < NIT - 1 || idx < C::PLANE;
+    T sl[C::NSU];
+    split_value<typename C::Tr>(rg.v[i], sl);
+#pragma unroll
+    for (int k = 0; k < C::NSU; ++k) lds[in_plane ? k * C::PLANE + idx : C::SPARE] = sl[k];
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void split_tile_setup(const Geo &g, int64_t tile, SplitCtx &t) {
+  t.pitch = g.ext[2];
+  t.ny = g.n[1];
+  t.nx = g.n[2];
+  t.Y0 = (tile / g.tiles[2]) * C::BY - C::KF * C::R;
+  t.X0 = (tile % g.tiles[2]) * C::BX - C::KF * C::R;
+  t.ay0 = g.org[1] + t.Y0;
+  t.ax0 = g.org[2] + t.X0;
+  t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
+}
+
+template <class C>
+__device__ __forceinline__ void split_lane_setup(const typename C::T *__restrict__ bfrag, int tid, SplitCtx &t,
+                                                 typename C::Tr::V (*w)[C::NINS], int *off) {
+  using V = typename C::Tr::V;
+  t.tid = tid;
+  t.lane = tid & 63;
+  t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = t.lane, grp = lane >> 4, n = lane & 15;
+  t.orow = 2 * (n & 7) + (grp >> 1);
+  t.ocol = 8 * (n >> 3) + 4 * (grp & 1);
+#pragma unroll
+  for (int q = 0; q < C::NSW; ++q)
+#pragma unroll
+    for (int i = 0; i < C::NINS; ++i)
+      w[q][i] = *reinterpret_cast<const V *>(bfrag + ((size_t)(q * C::NINS + i) * 64 + lane) * 8);
+#pragma unroll
+  for (int i = 0; i < C::NINS; ++i)
+    off[i] = (2 * (n & 7) + C::CT.ty[4 * i + grp]) * C::PX + 8 * (n >> 3) + C::CT.tx[4 * i + grp] + C::CS;
 }
 
 // One block: D0 = level 0 (hi x hi), D1 = the kept lower levels, scaled by SCALE.
