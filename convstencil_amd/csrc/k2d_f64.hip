@@ -187,6 +187,19 @@ static const Variant kVariants[] = {
     CS_V2D(111204, 4, 1, SHAPE_BOX, 4, 64, 64, 4, 1),
     CS_V2D(111208, 4, 1, SHAPE_BOX, 8, 64, 64, 4, 1),
     CS_V2D(113202, 4, 3, SHAPE_BOX, 2, 64, 64, 4, 1),
+    // last step staged through LDS, coalesced output rows (ids 700000 + R * 1000 + ...)
+    CS_V2D(711101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 2),
+    CS_V2D(711102, 4, 1, SHAPE_BOX, 2, 64, 64, 8, 2),
+    CS_V2D(711104, 4, 1, SHAPE_BOX, 4, 64, 64, 8, 2),
+    CS_V2D(711106, 4, 1, SHAPE_BOX, 6, 64, 64, 8, 2),
+    CS_V2D(711108, 4, 1, SHAPE_BOX, 8, 64, 64, 8, 2),
+    CS_V2D(711114, 4, 1, SHAPE_STAR, 4, 64, 64, 8, 2),
+    CS_V2D(711118, 4, 1, SHAPE_STAR, 8, 64, 64, 8, 2),
+    CS_V2D(713101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 2),
+    CS_V2D(713102, 4, 3, SHAPE_BOX, 2, 64, 64, 8, 2),
+    CS_V2D(713112, 4, 3, SHAPE_STAR, 2, 64, 64, 8, 2),
+    CS_V2D(713113, 4, 3, SHAPE_STAR, 3, 64, 64, 8, 2),
+    CS_V2D(713114, 4, 3, SHAPE_STAR, 4, 64, 64, 8, 2),
     // persistent form (ids 500000 + the id of the same configuration)
     CS_V2DP(511101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 1),
     CS_V2DP(511102, 4, 1, SHAPE_BOX, 2, 64, 64, 8, 1),

@@ -80,6 +80,8 @@ struct Cfg2D {
   // 16x16x4: odd pitch; 4x4x4: pitch = 4 mod 8, so the four 64-byte row segments that
   // one k lane group reads (rows 2 apart) fall in distinct quarters of the banks.
   static constexpr int PX = MF == 16 ? (PX0 | 1) : PX0 + ((4 - PX0 % 8) + 8) % 8;
+  // ST: 0 = two LDS buffers, 1 = one buffer + registers, 2 = as 1 with the last step's
+  // results staged in LDS and written out row by row (coalesced)
   static constexpr int NBUF = (KF > 1 && ST == 0) ? 2 : 1;
   static constexpr int BUF = EY * PX;
   static constexpr int SPARE = NBUF * BUF;  // one scratch cell after the buffers
@@ -198,17 +200,32 @@ __device__ __forceinline__ void fused_step(const TileCtx &t, const double *b, co
     block_origin<C, S>(min(j * NW + t.wave, NB - 1), y, x);
     acc[j] = block_mfma<C>(src + y * C::PX + x, off, b);
   }
-  if constexpr (!LAST && C::NBUF == 1) __syncthreads();  // every wave has read this step's input
+  constexpr bool STAGE = LAST && C::ST == 2;  // last step through LDS, then coalesced rows
+  if constexpr ((!LAST || STAGE) && C::NBUF == 1) __syncthreads();  // every wave has read this step's input
 #pragma unroll
   for (int j = 0; j < PERS; ++j) {
     const int blk = j * NW + t.wave;
     if ((NB % NW == 0 || j < PERS - 1) || blk < NB) {
       int y, x;
       block_origin<C, S>(blk, y, x);
-      put_block<C, LAST>(acc[j], y, x, C::LO(S), C::HI(S), t, dst, out);
+      put_block<C, LAST && !STAGE>(acc[j], y, x, C::LO(S), C::HI(S), t, dst, out);
     }
   }
-  if constexpr (!LAST) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (!LAST || STAGE) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (STAGE) {
+    // the BY x BX output block, NT / BX rows per pass, one row segment per BX lanes;
+    // cells outside the domain interior are skipped (they were not written)
+    constexpr int NT = NW * 64, OFF = C::KF * C::R, RP = NT / C::BX;
+    static_assert(NT % C::BX == 0, "a pass covers whole rows");
+    const int r0 = t.tid / C::BX, c = t.tid - r0 * C::BX;
+    const bool col_in = t.full || t.X0 + OFF + c < t.nx;
+    double *o = out + (t.ay0 + OFF + r0) * t.pitch + t.ax0 + OFF + c;
+    const double *l = dst + (OFF + r0) * C::PX + OFF + c + C::CS;
+#pragma unroll
+    for (int i = 0; i < C::BY / RP; ++i) {
+      if (col_in && (t.full || t.Y0 + OFF + r0 + RP * i < t.ny)) o[(int64_t)RP * i * t.pitch] = l[RP * i * C::PX];
+    }
+  }
 }
 
 template <class C, int S>

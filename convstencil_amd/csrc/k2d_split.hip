@@ -16,7 +16,7 @@ __global__ __launch_bounds__(NW * 64) void k2d_split(const double *__restrict__ 
   tile_split<Tr, R, KF, BY, BX, NW>(in, out, bfrag, g, tile, smem, threadIdx.x);
 }
 
-template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW, int OS>
 __global__ __launch_bounds__(NW * 64) void k2d_split2(const double *__restrict__ in, double *__restrict__ out,
                                                       const typename Tr::T *__restrict__ bfrag, Geo g) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -24,23 +24,23 @@ __global__ __launch_bounds__(NW * 64) void k2d_split2(const double *__restrict__
   const int64_t per = (ntiles + 7) / 8;
   const int64_t tile = (blockIdx.x % 8) * per + blockIdx.x / 8;
   if (tile >= ntiles) return;
-  tile_split2<Tr, R, SH, KF, BY, BX, NW>(in, out, bfrag, g, tile, smem, threadIdx.x);
+  tile_split2<Tr, R, SH, KF, BY, BX, NW, 64, OS>(in, out, bfrag, g, tile, smem, threadIdx.x);
 }
 
-template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW, int OS>
 static hipError_t prepare_ks2() {
-  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
-  return hipFuncSetAttribute((const void *)k2d_split2<Tr, R, SH, KF, BY, BX, NW>,
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW, OS>;
+  return hipFuncSetAttribute((const void *)k2d_split2<Tr, R, SH, KF, BY, BX, NW, OS>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
 }
 
-template <class Tr, int R, int SH, int KF, int BY, int BX, int NW>
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW, int OS>
 static hipError_t launch_ks2(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s) {
-  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW, OS>;
   const int64_t ntiles = g.tiles[1] * g.tiles[2];
   const int64_t grid = ((ntiles + 7) / 8) * 8;
-  hipLaunchKernelGGL((k2d_split2<Tr, R, SH, KF, BY, BX, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in,
-                     out, reinterpret_cast<const typename Tr::T *>(bfrag), g);
+  hipLaunchKernelGGL((k2d_split2<Tr, R, SH, KF, BY, BX, NW, OS>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s,
+                     in, out, reinterpret_cast<const typename Tr::T *>(bfrag), g);
   return hipGetLastError();
 }
 
@@ -154,14 +154,15 @@ static hipError_t launch_ks(const double *in, double *out, const double *bfrag, 
   }
 
 // Dilated form: Variant::st = 2.  Ids 60000 + R * 1000 + precision * 100 + (star) * 10 + KF.
-#define CS_VS2(ID, PREC, TR, R, SH, KF, BY, BX, NW)                                                     \
+#define CS_VS2O(ID, PREC, TR, R, SH, KF, BY, BX, NW, OS)                                                \
   Variant {                                                                                             \
-    ID, PREC, R, SH, KF, BY, BX, NW, CfgS2<TR, R, SH, KF, BY, BX, NW>::LDS,                             \
-        CfgS2<TR, R, SH, KF, BY, BX, NW>::PX, 2, &launch_ks2<TR, R, SH, KF, BY, BX, NW>,               \
-        &prepare_ks2<TR, R, SH, KF, BY, BX, NW>,                                                        \
-        "k2d_split2<" #TR ",r" #R "," #SH ",kf" #KF "," #BY "x" #BX ",w" #NW ">",                       \
-        CfgS2<TR, R, SH, KF, BY, BX, NW>::CS                                                            \
+    ID, PREC, R, SH, KF, BY, BX, NW, CfgS2<TR, R, SH, KF, BY, BX, NW, OS>::LDS,                         \
+        CfgS2<TR, R, SH, KF, BY, BX, NW, OS>::PX, 2, &launch_ks2<TR, R, SH, KF, BY, BX, NW, OS>,       \
+        &prepare_ks2<TR, R, SH, KF, BY, BX, NW, OS>,                                                    \
+        "k2d_split2<" #TR ",r" #R "," #SH ",kf" #KF "," #BY "x" #BX ",w" #NW ",os" #OS ">",             \
+        CfgS2<TR, R, SH, KF, BY, BX, NW, OS>::CS                                                        \
   }
+#define CS_VS2(ID, PREC, TR, R, SH, KF, BY, BX, NW) CS_VS2O(ID, PREC, TR, R, SH, KF, BY, BX, NW, 0)
 
 #define CS_VS2P(ID, PREC, TR, R, SH, KF, BY, BX, NW)                                                    \
   Variant {                                                                                             \
@@ -197,6 +198,14 @@ static const Variant kVariantsSplit[] = {
     CS_VS2(63304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8),
     CS_VS2(63311, 3, TrF16, 3, SHAPE_STAR, 1, 64, 64, 8),
     CS_VS2(63312, 3, TrF16, 3, SHAPE_STAR, 2, 64, 64, 8),
+    // staged FP64 output of the last step (ids 700000 + the id of the same configuration)
+    CS_VS2O(761102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8, 1),
+    CS_VS2O(761104, 1, TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8, 1),
+    CS_VS2O(763102, 1, TrBF16, 3, SHAPE_BOX, 2, 64, 64, 8, 1),
+    CS_VS2O(763104, 1, TrBF16, 3, SHAPE_BOX, 4, 64, 64, 8, 1),
+    CS_VS2O(763112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8, 1),
+    CS_VS2O(763113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
+    CS_VS2O(763304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8, 1),
     // persistent dilated form (ids 500000 + the id of the same configuration)
     CS_VS2P(561102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8),
     CS_VS2P(561104, 1, TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8),

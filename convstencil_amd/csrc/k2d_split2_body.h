@@ -38,11 +38,13 @@ struct Quad<TrF16> {
   using V = f16x4;
 };
 
-template <class Tr_, int R_, int SH_, int KF_, int BY_, int BX_, int NW_>
+// OS = 1: the last step's FP64 results are staged in LDS (over the slice planes, which
+// are no longer read) and written out row by row, coalesced.
+template <class Tr_, int R_, int SH_, int KF_, int BY_, int BX_, int NW_, int OS_ = 0>
 struct CfgS2 {
   using Tr = Tr_;
   using T = typename Tr::T;
-  static constexpr int R = R_, SH = SH_, KF = KF_, BY = BY_, BX = BX_, NW = NW_;
+  static constexpr int R = R_, SH = SH_, KF = KF_, BY = BY_, BX = BX_, NW = NW_, OS = OS_;
   static constexpr int EY = BY + 2 * KF * R;
   static constexpr int EX = BX + 2 * KF * R;
   static constexpr CTable CT = make_ctable(R, SH);
@@ -95,6 +97,7 @@ struct CfgS2 {
   static_assert(PX % 8 == 0, "16-byte reads need a pitch that is a multiple of 8 elements");
   static_assert(LO(0) + CT.tmin >= 0 && LO(0) + H(0) - 16 + 14 + CT.tmax < EY, "block reads outside the tile");
   static_assert(LDS <= 160 * 1024, "tile does not fit gfx950 LDS");
+  static_assert(OS == 0 || (size_t)BY * BX * sizeof(double) <= LDS, "staged output does not fit");
 };
 
 struct SplitCtx {
@@ -280,7 +283,7 @@ __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::
     block_origin_s2<C, S>(min(j * NW + t.wave, NB - 1), y, x);
     block_split<C>(lds + y * C::PX + x, off, w, d0[j], d1[j]);
   }
-  if constexpr (!LAST) __syncthreads();  // every wave has read this step's input
+  if constexpr (!LAST || C::OS == 1) __syncthreads();  // every wave has read this step's input
 #pragma unroll
   for (int j = 0; j < PERS; ++j) {
     const int blk = j * NW + t.wave;
@@ -289,7 +292,16 @@ __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::
     block_origin_s2<C, S>(blk, y, x);
     const int ty = y + t.orow, tx0 = x + t.ocol;
     const bool row_in = t.full || (t.Y0 + ty >= 0 && t.Y0 + ty < t.ny);
-    if constexpr (LAST) {
+    if constexpr (LAST && C::OS == 1) {
+      // staged: the output block cell (ty - LO, tx - LO) of a compact BY x BX FP64 array
+      double *ob = reinterpret_cast<double *>(lds);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int tx = tx0 + v;
+        if (tx >= C::LO(S) && tx < C::HI(S))
+          ob[(ty - C::LO(S)) * C::BX + tx - C::LO(S)] = (double)d0[j][v] + (double)d1[j][v] * (double)INV;
+      }
+    } else if constexpr (LAST) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int tx = tx0 + v;
@@ -320,6 +332,18 @@ __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::
     }
   }
   if constexpr (!LAST) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (LAST && C::OS == 1) {
+    __syncthreads();  // the staged block is complete
+    constexpr int NT = NW * 64, RP = NT / C::BX, OFF = C::KF * C::R;
+    static_assert(NT % C::BX == 0, "a pass covers whole rows");
+    const double *ob = reinterpret_cast<const double *>(lds);
+    const int r0 = t.tid / C::BX, c = t.tid - r0 * C::BX;
+    const bool col_in = t.full || t.X0 + OFF + c < t.nx;
+    double *o = out + (t.ay0 + OFF + r0) * t.pitch + t.ax0 + OFF + c;
+#pragma unroll
+    for (int i = 0; i < C::BY / RP; ++i)
+      if (col_in && (t.full || t.Y0 + OFF + r0 + RP * i < t.ny)) o[(int64_t)RP * i * t.pitch] = ob[(r0 + RP * i) * C::BX + c];
+  }
 }
 
 template <class C, int S>
@@ -334,11 +358,11 @@ __device__ __forceinline__ void split_steps(const SplitCtx &t, const typename C:
 // One workgroup advances tile `tile` of grid g by KF steps (smem: the kernel's dynamic
 // LDS; bfrag: the weights' slices as A fragments; LG: tile loads in flight per thread;
 // tid: threadIdx.x).
-template <class Tr, int R, int SH, int KF, int BY, int BX, int NW, int LG = 64>
+template <class Tr, int R, int SH, int KF, int BY, int BX, int NW, int LG = 64, int OS = 0>
 __device__ __forceinline__ void tile_split2(const double *__restrict__ in, double *__restrict__ out,
                                             const typename Tr::T *__restrict__ bfrag, const Geo &g, int64_t tile,
                                             unsigned char *__restrict__ smem, int tid) {
-  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW>;
+  using C = CfgS2<Tr, R, SH, KF, BY, BX, NW, OS>;
   using T = typename Tr::T;
   using V = typename Tr::V;
   T *lds = reinterpret_cast<T *>(smem);

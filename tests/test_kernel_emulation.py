@@ -226,7 +226,7 @@ def emulate(v, layout, dense, x):
                     inside = full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
                     if not inside or tx < base or tx >= base + W:
                         continue
-                    if s < kf - 1:
+                    if s < kf - 1 or v["st"] == 2:  # ST 2 stages the last step in LDS too
                         li = dst + ty * px + tx + CS
                         assert dst <= li < dst + BUF
                         new_vals.append((li, vv))
@@ -237,6 +237,16 @@ def emulate(v, layout, dense, x):
                         out[gi] = vv
             for li, vv in new_vals:  # the kernel's barrier separates reads of src from these writes
                 lds[li] = vv
+        if v["st"] == 2:  # coalesced copy-out of the output block from LDS
+            off0 = kf * R
+            for idx in range(by * bx):
+                r, c = off0 + idx // bx, off0 + idx % bx
+                if full or (Y0 + r < n[1] and X0 + c < n[2]):
+                    li = r * px + c + CS
+                    assert 0 <= li < BUF and np.isfinite(lds[li])
+                    gi = (ay0 + r) * pitch + ax0 + c
+                    assert org[1] <= ay0 + r < org[1] + n[1] and org[2] <= ax0 + c < org[2] + n[2]
+                    out[gi] = lds[li]
     return out
 
 
@@ -991,6 +1001,8 @@ def emulate_split2(v, layout, dense, x):
             if 0 <= c < EX and 0 <= ay < ext[1] and 0 <= ax < pitch:
                 vals[idx] = x3[0, ay, ax]
         lds = [s_.astype(np.float32) for s_ in split_np(vals, kind, NSU, SCALE)]
+        staged = ",os1" in v["name"]
+        ob = np.full(BY * BX, np.nan)
 
         def inside(ty, tx):
             return full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
@@ -1029,7 +1041,12 @@ def emulate_split2(v, layout, dense, x):
                         m = 4 * grp[ln] + vv
                         tx = xx + ocol[ln] + vv
                         a0, a1 = d0[m, nn[ln]], d1[m, nn[ln]]
-                        if last:
+                        if last and staged:
+                            if LO <= tx < HI:  # compact BY x BX FP64 block over the slice planes
+                                oi = (ty - LO) * BX + tx - LO
+                                assert 0 <= oi < BY * BX and (oi + 1) * 8 <= v["lds"]
+                                ob[oi] = float(a0) + float(a1) * float(INV)
+                        elif last:
                             if LO <= tx < HI and inside(ty, tx):
                                 gi = (ay0 + ty) * pitch + ax0 + tx
                                 assert org[1] <= ay0 + ty < org[1] + n[1] and org[2] <= ax0 + tx < org[2] + n[2]
@@ -1043,10 +1060,19 @@ def emulate_split2(v, layout, dense, x):
             for li, val in writes:
                 for k, s_ in enumerate(split_f32_np(np.array([val], dtype=np.float32), kind, NSU, SCALE)):
                     lds[k][li] = s_[0]
+        if staged:  # coalesced copy-out of the staged block
+            off0 = KF * R
+            for idx in range(BY * BX):
+                r, c = off0 + idx // BX, off0 + idx % BX
+                if full or (Y0 + r < n[1] and X0 + c < n[2]):
+                    assert np.isfinite(ob[idx])
+                    gi = (ay0 + r) * pitch + ax0 + c
+                    assert org[1] <= ay0 + r < org[1] + n[1] and org[2] <= ax0 + c < org[2] + n[2]
+                    out[gi] = ob[idx]
     return out
 
 
-VARIANTS_SPLIT2 = [v for v in VARIANTS_SPLIT if v["st"] == 2 and v["kf"] <= 2]
+VARIANTS_SPLIT2 = [v for v in VARIANTS_SPLIT if v["st"] == 2 and (v["kf"] <= 2 or ",os1" in v["name"])]
 
 
 @pytest.mark.parametrize("v", VARIANTS_SPLIT2, ids=[v["name"] for v in VARIANTS_SPLIT2])
