@@ -470,3 +470,101 @@ constexpr int dtable3_ksteps(const DTable3 &t) {
 }
 
 }  // namespace cs
+
+namespace cs {
+
+// ---- LDS bank-conflict model of gfx950 (measured) ----
+//
+// 128 banks of 4 bytes (512-byte period).  A ds_read_b128 is serviced 32 lanes at a
+// time: its cost is the sum over the two half-waves of the largest number of distinct
+// 16-byte chunks that fall in one 16-byte bank slot (lanes reading the same chunk share
+// it).  8-byte accesses are serviced 64 lanes at a time in 8-byte slots.  This model
+// reproduces the measured rates of contiguous, strided-row and the kernels' own
+// patterns (224-240 B/clk conflict-free, 125 B/clk 2-way, 85 B/clk 3-way).
+// Addresses are in bytes relative to any base (a common shift does not change costs).
+constexpr int lds_cost(const int (&addr)[64], int bytes) {
+  const int lanes = bytes == 16 ? 32 : 64, slots = 512 / bytes;
+  int cost = 0;
+  for (int h = 0; h < 64 / lanes; ++h) {
+    int count[64] = {}, worst = 0;
+    for (int l = h * lanes; l < (h + 1) * lanes; ++l) {
+      const int c = addr[l] / bytes;
+      bool seen = false;  // lanes reading the same chunk share one access
+      for (int k = h * lanes; k < l && !seen; ++k) seen = addr[k] / bytes == c;
+      if (seen) continue;
+      const int s = ((c % slots) + slots) % slots;
+      ++count[s];
+      worst = count[s] > worst ? count[s] : worst;
+    }
+    cost += worst;
+  }
+  return cost;
+}
+
+// LDS cost of one 8 x 8 block of a dilated table (its pair and single reads) plus the
+// write-back of its results, at row pitch px doubles.
+constexpr int dtable_lds_cost(const DTable &t, int px) {
+  int cost = 0;
+  int a[64] = {};
+  for (int ps = 0; ps < t.np + t.ns; ++ps) {
+    const bool pair = ps < t.np;
+    for (int l = 0; l < 64; ++l) {
+      const int q = l >> 4, b4 = (l >> 2) & 3, m = l & 3;
+      const int ty = pair ? t.pty[ps][q] : t.sty[ps - t.np][q], tx = pair ? t.ptx[ps][q] : t.stx[ps - t.np][q];
+      a[l] = ((2 * b4 + ty + 16) * px + 2 * m + tx + 16) * 8;
+    }
+    cost += lds_cost(a, pair ? 16 : 8);
+  }
+  for (int l = 0; l < 64; ++l) {
+    const int b4 = (l >> 2) & 3, n = l & 3, m = l >> 4;
+    a[l] = ((2 * b4 + (n >> 1)) * px + 2 * m + (n & 1)) * 8;
+  }
+  return cost + lds_cost(a, 8);
+}
+
+// Pitch (doubles, >= px0, even, < px0 + 64) with the lowest cost over the given tables.
+constexpr int best_pitch_dtables(const DTable *ts, int nt, int px0) {
+  int best = -1, best_cost = 1 << 30;
+  for (int px = px0 + (px0 & 1); px < px0 + 64; px += 2) {
+    int c = 0;
+    for (int i = 0; i < nt; ++i) c += dtable_lds_cost(ts[i], px);
+    if (c < best_cost) {
+      best_cost = c;
+      best = px;
+    }
+  }
+  return best;
+}
+
+// LDS cost of one 16 x 16 block of the split form at pitch px (elements of `esz` bytes):
+// one 16-byte read per slice and instruction, one 8-byte write per slice.
+constexpr int ctable_lds_cost(const CTable &t, int px, int esz) {
+  int cost = 0;
+  int a[64] = {};
+  for (int i = 0; i < t.n / 4; ++i) {
+    for (int l = 0; l < 64; ++l) {
+      const int g = l >> 4, n = l & 15, c = 4 * i + g;
+      a[l] = ((2 * (n & 7) + t.ty[c] + 16) * px + 8 * (n >> 3) + t.tx[c] + 16) * esz;
+    }
+    cost += lds_cost(a, 16);
+  }
+  for (int l = 0; l < 64; ++l) {
+    const int g = l >> 4, n = l & 15;
+    a[l] = ((2 * (n & 7) + (g >> 1)) * px + 8 * (n >> 3) + 4 * (g & 1)) * esz;
+  }
+  return cost + lds_cost(a, 8);
+}
+
+constexpr int best_pitch_ctable(const CTable &t, int px0, int esz) {
+  int best = -1, best_cost = 1 << 30;
+  for (int px = (px0 + 7) / 8 * 8; px < px0 + 128; px += 8) {
+    const int c = ctable_lds_cost(t, px, esz);
+    if (c < best_cost) {
+      best_cost = c;
+      best = px;
+    }
+  }
+  return best;
+}
+
+}  // namespace cs

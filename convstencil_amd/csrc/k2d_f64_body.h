@@ -77,9 +77,9 @@ struct Cfg2D {
     return m;
   }
   static constexpr int PX0 = CS + (max_col() + 1 > EX ? max_col() + 1 : EX);
-  // 16x16x4: odd pitch; 4x4x4: pitch = 4 mod 8, so the four 64-byte row segments that
-  // one k lane group reads (rows 2 apart) fall in distinct quarters of the banks.
-  static constexpr int PX = MF == 16 ? (PX0 | 1) : PX0 + ((4 - PX0 % 8) + 8) % 8;
+  // 16x16x4: odd pitch; 4x4x4: the even pitch with the fewest LDS bank conflicts for
+  // this table's reads and the result writes (cs_ktable.h, lds_cost)
+  static constexpr int PX = MF == 16 ? (PX0 | 1) : best_pitch_dtables(&DT, 1, PX0);
   // ST: 0 = two LDS buffers, 1 = one buffer + registers, 2 = as 1 with the last step's
   // results staged in LDS and written out row by row (coalesced)
   static constexpr int NBUF = (KF > 1 && ST == 0) ? 2 : 1;

@@ -85,9 +85,9 @@ struct CfgS2 {
     return m;
   }
   static constexpr int PX0 = CS + max_col() + 1;
-  // pitch = 8 (mod 16) elements: the 8 row segments one k group reads (rows 2 apart,
-  // 32 bytes each) fall in distinct 32-byte bank groups
-  static constexpr int PX = PX0 + ((8 - PX0 % 16) + 16) % 16;
+  // a multiple of 8 elements (16-byte reads) with the fewest LDS bank conflicts for the
+  // chunk reads and the result writes (cs_ktable.h, lds_cost)
+  static constexpr int PX = best_pitch_ctable(CT, PX0, (int)sizeof(T));
   static constexpr int PLANE = EY * PX;
   static constexpr int BUF = NSU * PLANE;
   static constexpr int SPARE = BUF;  // scratch cells after the slice planes

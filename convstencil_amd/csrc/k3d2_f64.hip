@@ -88,7 +88,7 @@ struct Cfg3D2 : Tab3D<R_, SH_> {
   static constexpr int NB = NBY * NBX;
   static constexpr int PER = (NB + NW - 1) / NW;
   static constexpr int PX0 = CS + (LASTX + 6 + cmax() + 1 > EX ? LASTX + 6 + cmax() + 1 : EX);
-  static constexpr int PX = PX0 + ((4 - PX0 % 8) + 8) % 8;  // 4 (mod 8), as the 2-D kernels
+  static constexpr int PX = best_pitch_dtables(DT.t, 2 * R + 1, PX0);  // fewest LDS bank conflicts
   static constexpr int PLANE = EY * PX;
   static constexpr int NP = 2 * R + 1;
   static constexpr int SPARE = NP * PLANE;
@@ -290,7 +290,7 @@ struct Cfg3F : Tab3D<R_, SH_> {
     return m;
   }
   static constexpr int PX0 = CS + max_col() + 1;
-  static constexpr int PX = PX0 + ((4 - PX0 % 8) + 8) % 8;
+  static constexpr int PX = best_pitch_dtables(DT.t, 2 * R + 1, PX0);  // fewest LDS bank conflicts
   static constexpr int PLANE = EY * PX;
   static constexpr int NP = 2 * R + 1;
   static constexpr int RING = NP * PLANE;

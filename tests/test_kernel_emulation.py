@@ -157,7 +157,7 @@ def emulate(v, layout, dense, x):
     offs = lane_offsets(v, tab)
     CS = v["cs"]
     if v["mf"] == 4:
-        assert px % 8 == 4
+        assert px % 2 == 0
         pp = tab[2]
         assert bf.shape[0] == 2 * tab[0] + tab[1]
     EY, EX = by + 2 * kf * R, bx + 2 * kf * R
@@ -553,7 +553,7 @@ def emulate_3d2(v, layout, dense, x):
     bf = bfrag3d2(R, v["shape"], dense)
     assert bf.shape[0] == sum(2 * t[0] + t[1] for t in tabs.values())
     EY, EX, PX, CS = BY + 2 * R, BX + 2 * R, v["px"], v["cs"]
-    assert PX % 8 == 4
+    assert PX % 2 == 0
     PLANE = EY * PX
     NP = 2 * R + 1
     assert v["lds"] == (NP * PLANE + 2) * 8 and v["lds"] <= 160 * 1024
@@ -635,7 +635,7 @@ def emulate_3f(v, layout, dense, x):
     pp = tabs[0][2]
     bf = bfrag3d2(R, v["shape"], dense)
     EY, EX, PX, CS = BY + 2 * KF * R, BX + 2 * KF * R, v["px"], v["cs"]
-    assert PX % 8 == 4
+    assert PX % 2 == 0
     PLANE = EY * PX
     NP = 2 * R + 1
     assert v["lds"] == (KF * NP * PLANE + 2) * 8 and v["lds"] <= 160 * 1024
@@ -968,7 +968,7 @@ def emulate_split2(v, layout, dense, x):
     assert nbytes == NSW * NINS * 64 * 8 * 2
     wf = _to_f32(raw[:nbytes].copy().view(np.uint16), kind).reshape(NSW, NINS, 64, 8)
     EY, EX, PX, CS = BY + 2 * KF * R, BX + 2 * KF * R, v["px"], v["cs"]
-    assert PX % 16 == 8
+    assert PX % 8 == 0
     PLANE = EY * PX
     assert v["lds"] == (NSU * PLANE + 8) * 2 and v["lds"] <= 160 * 1024
     lanes = np.arange(64)
