@@ -218,7 +218,7 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
     const int R = st.radius;
     const int D = 2 * R + 1;
     std::vector<double> w(st.weights, st.weights + D * D);
-    const int shape = b->prec == 0 ? classify_2d(R, w.data()) : cs::SHAPE_BOX;
+    const int shape = classify_2d(R, w.data());
     const int kf = cs::batch_main_kf(b->prec, R);
     const int km = cs::batch_kind(b->prec, R, shape, kf), k1 = cs::batch_kind(b->prec, R, shape, 1);
     if (km < 0 || k1 < 0) {
@@ -241,7 +241,7 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
         s = CS_ERR_ALLOC;
     } else {
       std::vector<unsigned char> h(1 << 20);
-      const size_t bytes = cs::build_bfrag_split(CS_PREC_BF16X3, R, w.data(), h.data(), h.size());
+      const size_t bytes = cs::build_bfrag_split2(CS_PREC_BF16X3, R, shape, w.data(), h.data(), h.size());
       if (bytes == 0 || hipMalloc(&d, bytes) != hipSuccess ||
           hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
         s = CS_ERR_ALLOC;

@@ -18,7 +18,7 @@
 //     (k2d_f64_body.h, k2d_split_body.h).
 #include "cs_batch.h"
 #include "k2d_f64_body.h"
-#include "k2d_split_body.h"
+#include "k2d_split2_body.h"
 
 namespace cs {
 
@@ -53,7 +53,7 @@ __global__ void batch_reset(unsigned *sync, int n) {
 // batched workload's two stencils (box2d1r, star2d3r), fused and single-step.
 // Every extra body adds to the one kernel's register allocation.
 //   FP64:  0 box r1 kf4, 1 box r1 kf1, 2 star r3 kf2, 3 star r3 kf1
-//   split: 0 bf16x3 r1 kf4, 1 bf16x3 r1 kf1, 2 bf16x3 r3 kf2, 3 bf16x3 r3 kf1
+//   BF16x3 (dilated split form): the same four kinds
 template <int PREC>
 __device__ __forceinline__ void run_item(const BatchGrid &G, const BatchItem &m, unsigned char *smem, int tid) {
   const double *in = G.buf[m.src];
@@ -70,10 +70,10 @@ __device__ __forceinline__ void run_item(const BatchGrid &G, const BatchItem &m,
   } else {
     const TrBF16::T *bf = reinterpret_cast<const TrBF16::T *>(G.bfrag);
     switch (m.kind) {
-      case 0: tile_split<TrBF16, 1, 4, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
-      case 1: tile_split<TrBF16, 1, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
-      case 2: tile_split<TrBF16, 3, 2, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
-      case 3: tile_split<TrBF16, 3, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 0: tile_split2<TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 1: tile_split2<TrBF16, 1, SHAPE_BOX, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 2: tile_split2<TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 3: tile_split2<TrBF16, 3, SHAPE_STAR, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
       default: break;
     }
   }
@@ -122,7 +122,7 @@ static size_t batch_lds() {
     m = std::max(m, Cfg2D<4, 3, SHAPE_STAR, 2, 64, 64, 8, 1>::LDS);
     return m;
   } else {
-    return std::max(CfgS<TrBF16, 1, 4, 64, 64, 8>::LDS, CfgS<TrBF16, 3, 2, 64, 64, 8>::LDS);
+    return std::max(CfgS2<TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8>::LDS, CfgS2<TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8>::LDS);
   }
 }
 
@@ -164,8 +164,8 @@ int batch_kind(int prec, int R, int shape, int kf) {
     if (R == 3 && shape == SHAPE_STAR) return kf == 2 ? 2 : kf == 1 ? 3 : -1;
     return -1;
   }
-  if (R == 1) return kf == 4 ? 0 : kf == 1 ? 1 : -1;
-  if (R == 3) return kf == 2 ? 2 : kf == 1 ? 3 : -1;
+  if (R == 1 && shape == SHAPE_BOX) return kf == 4 ? 0 : kf == 1 ? 1 : -1;
+  if (R == 3 && shape == SHAPE_STAR) return kf == 2 ? 2 : kf == 1 ? 3 : -1;
   return -1;
 }
 
