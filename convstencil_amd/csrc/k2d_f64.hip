@@ -57,7 +57,7 @@ __global__ __launch_bounds__(NW * 64) void k2d_f64p(const double *__restrict__ i
   int off[C::NOFF];
   lane_setup<C>(bfrag, threadIdx.x, t, b, off);
   TileRegs<C> rg;
-  tile_setup<C>(g, xcd_tile(k, ntiles), t);
+  tile_setup<C>(g, xcd_tile(k, ntiles), t, out);
   issue_tile<C>(in, g, t, rg);
   for (;;) {
     store_tile<C>(t, rg, lds);
@@ -72,10 +72,10 @@ __global__ __launch_bounds__(NW * 64) void k2d_f64p(const double *__restrict__ i
     t.wave = w;
     TileCtx tn = t;
     if (kn < total) {
-      tile_setup<C>(g, xcd_tile(kn, ntiles), tn);
+      tile_setup<C>(g, xcd_tile(kn, ntiles), tn, out);
       issue_tile<C>(in, g, tn, rg);  // in flight while this tile is computed
     }
-    fused_steps<C, 0>(t, b, off, lds, out);
+    run_steps<C>(t, b, off, lds, out);
     if (kn >= total) break;
     __syncthreads();  // every wave is done with this tile's LDS before the next tile is stored
     k = kn;

@@ -30,7 +30,7 @@
 
 namespace cs {
 
-template <int MF_, int R_, int SH_, int KF_, int BY_, int BX_, int NW_, int ST_>
+template <int MF_, int R_, int SH_, int KF_, int BY_, int BX_, int NW_, int ST_, int PXM_ = -1>
 struct Cfg2D {
   static constexpr int MF = MF_, R = R_, SH = SH_, KF = KF_, BY = BY_, BX = BX_, NW = NW_, ST = ST_;
   static constexpr int EY = BY + 2 * KF * R;
@@ -77,15 +77,27 @@ struct Cfg2D {
     return m;
   }
   static constexpr int PX0 = CS + (max_col() + 1 > EX ? max_col() + 1 : EX);
-  // 16x16x4: odd pitch; 4x4x4: the even pitch with the fewest LDS bank conflicts for
-  // this table's reads and the result writes (cs_ktable.h, lds_cost)
-  static constexpr int PX = MF == 16 ? (PX0 | 1) : best_pitch_dtables(&DT, 1, PX0);
   // ST: 0 = two LDS buffers, 1 = one buffer + registers, 2 = as 1 with the last step's
   // results staged in LDS and written out row by row (coalesced)
   static constexpr int NBUF = (KF > 1 && ST == 0) ? 2 : 1;
+  // 16x16x4: odd pitch; 4x4x4: the even pitch with the fewest LDS bank conflicts for
+  // this table's reads and the result writes that keeps the most workgroups per CU
+  // (cs_ktable.h, lds_cost); PXM_ >= 0 forces the smallest pitch >= PX0 that is PXM_ mod 16
+  static constexpr int PX = MF == 16 ? (PX0 | 1)
+                            : PXM_ >= 0 ? PX0 + ((PXM_ - PX0 % 16) + 16) % 16
+                                        : best_pitch_dtables(&DT, 1, PX0, (long)NBUF * EY * 8, 16);
   static constexpr int BUF = EY * PX;
   static constexpr int SPARE = NBUF * BUF;  // one scratch cell after the buffers
   static constexpr size_t LDS = (size_t)(NBUF * BUF + 2) * sizeof(double);
+  // 4x4x4: the reads' shared vector addresses and immediate offsets at this pitch
+  static constexpr RdPlan RPL = make_rdplan(DT, PX);
+  // block `blk` of step s -> its first output cell (y, x) in tile coordinates; the last
+  // block row and column are moved back to end at the region's edge (they overlap)
+  static constexpr void origin(int s, int blk, int &y, int &x) {
+    const int ba = blk / NBX(s), bb = blk - ba * NBX(s);
+    y = LO(s) + (BH * ba < H(s) - BH ? BH * ba : H(s) - BH);
+    x = LOX(s) + BW * bb < LASTX(s) ? LOX(s) + BW * bb : LASTX(s);
+  }
   static_assert(BY % 16 == 0 && BX % 16 == 0, "tile must be a multiple of 16");
   static_assert(min_col() + CS >= 0, "block reads left of the LDS row");
   static_assert(MF == 16 || (PX % 2 == 0 && (CS + PP + LOX(0)) % 2 == 0), "pair reads must be aligned");
@@ -120,6 +132,11 @@ struct TileCtx {
   int orow, ocol;  // 4x4x4: the output cell of this lane relative to the block
   int full;        // 1 if the whole tile (with halo) lies inside the domain interior
   int64_t Y0, X0, ny, nx, ay0, ax0, pitch;
+  // 4x4x4 only
+  int vw;              // LDS byte offset of this lane's result cell from the block origin
+  uint32_t vgo;        // global byte offset of this lane's result cell from the block origin
+  double *gout;        // output cell of tile cell (0, 0)
+  int ry0, nry, cx0, ncx;  // tile rows [ry0, ry0 + nry) and columns [cx0, cx0 + ncx) are interior
 };
 
 // One block's MFMA chain: A from LDS at p + off[], B from registers.
@@ -237,6 +254,164 @@ __device__ __forceinline__ void fused_steps(const TileCtx &t, const double *b, c
   }
 }
 
+// ---- 4x4x4 path ----
+//
+// A wave's block origins in every fused step are compile-time functions of its wave
+// index, so each kernel tabulates them: o = LDS byte offset of the block's first cell,
+// yx = (y << 16) | x in tile coordinates.  The table sits in constant memory and is read
+// with scalar loads (eight blocks per load); a block then costs one vector add per read
+// class (cs_ktable.h, RdPlan) and one for its result, with no branches.  In the last
+// round of a step a wave without a block of its own recomputes the step's last block
+// and stores the same values again.
+template <class C>
+struct Org4 {
+  static constexpr int base(int s) {
+    int n = 0;
+    for (int k = 0; k < s; ++k) n += C::PERS(k);
+    return n;
+  }
+  static constexpr int TOT = base(C::KF);
+  int o[C::NW][TOT];
+  int yx[C::NW][TOT];
+};
+
+template <class C>
+constexpr Org4<C> make_org4() {
+  Org4<C> t{};
+  for (int w = 0; w < C::NW; ++w)
+    for (int s = 0; s < C::KF; ++s)
+      for (int j = 0; j < C::PERS(s); ++j) {
+        const int blk = j * C::NW + w < C::NB(s) ? j * C::NW + w : C::NB(s) - 1;
+        int y = 0, x = 0;
+        C::origin(s, blk, y, x);
+        t.o[w][Org4<C>::base(s) + j] = (y * C::PX + x) * 8;
+        t.yx[w][Org4<C>::base(s) + j] = (y << 16) | x;
+      }
+  return t;
+}
+
+template <class C>
+struct Org4Tab {
+  static constexpr Org4<C> t = make_org4<C>();
+};
+
+// One 8 x 8 block: p = LDS address of its origin; off = the lane's read offsets.
+template <class C>
+__device__ __forceinline__ double block4(const char *p, const int *off, const double *b) {
+  constexpr RdPlan P = C::RPL;
+  double a = 0.0;
+#pragma unroll
+  for (int ps = 0; ps < C::DT.np; ++ps) {  // one 16-byte read feeds two k-steps
+    const dbl2 pr = *reinterpret_cast<const dbl2 *>(p + off[P.rep[ps]] + P.delta[ps]);
+    a = __builtin_amdgcn_mfma_f64_4x4x4f64(pr.x, b[2 * ps], a, 0, 0, 0);
+    a = __builtin_amdgcn_mfma_f64_4x4x4f64(pr.y, b[2 * ps + 1], a, 0, 0, 0);
+  }
+#pragma unroll
+  for (int ss = 0; ss < C::DT.ns; ++ss) {
+    const int i = C::DT.np + ss;
+    const double v = *reinterpret_cast<const double *>(p + off[P.rep[i]] + P.delta[i]);
+    a = __builtin_amdgcn_mfma_f64_4x4x4f64(v, b[2 * C::DT.np + ss], a, 0, 0, 0);
+  }
+  return a;
+}
+
+__device__ __forceinline__ bool in_interior(const TileCtx &t, int ty, int tx) {
+  return (unsigned)(ty - t.ry0) < (unsigned)t.nry && (unsigned)(tx - t.cx0) < (unsigned)t.ncx;
+}
+
+// ST = 2: the BY x BX output block from LDS to the grid, NT / BX rows per pass, one row
+// segment per BX lanes; with MASK, cells outside the domain interior are skipped.
+template <class C, bool MASK>
+__device__ __forceinline__ void copy_out4(const TileCtx &t, const char *buf) {
+  constexpr int NT = C::NW * 64, OFF = C::KF * C::R, RP = NT / C::BX;
+  static_assert(NT % C::BX == 0, "a pass covers whole rows");
+  const int r0 = t.tid / C::BX, c = t.tid - r0 * C::BX;
+  const bool col_in = !MASK || (unsigned)(OFF + c - t.cx0) < (unsigned)t.ncx;
+  double *o = t.gout + (int64_t)(OFF + r0) * t.pitch + OFF + c;
+  const double *l = reinterpret_cast<const double *>(buf) + (OFF + r0) * C::PX + OFF + c + C::CS;
+#pragma unroll
+  for (int i = 0; i < C::BY / RP; ++i) {
+    if (!MASK || (col_in && (unsigned)(OFF + r0 + RP * i - t.ry0) < (unsigned)t.nry))
+      o[(int64_t)RP * i * t.pitch] = l[RP * i * C::PX];
+  }
+}
+
+// Fused step S.  MASK = false (a tile inside the domain interior): results are stored
+// without tests -- cells a block computes beyond the step's region are never read by
+// a later step's region -- except in the last step, which writes only the region's
+// columns to the grid.  MASK = true: cells outside the domain interior keep their value.
+template <class C, int S, bool MASK>
+__device__ __forceinline__ void step4(const TileCtx &t, const double *b, const int *off, char *lds) {
+  constexpr bool LAST = S == C::KF - 1, STAGE = LAST && C::ST == 2, GLOBAL = LAST && !STAGE;
+  constexpr int PERS = C::PERS(S), B0 = Org4<C>::base(S);
+  const char *src = lds + (C::NBUF == 2 ? (S & 1) * C::BUF * 8 : 0);
+  char *dst = lds + (C::NBUF == 2 ? ((S + 1) & 1) * C::BUF * 8 : 0);
+  const int *ot = Org4Tab<C>::t.o[t.wave] + B0;
+  const int *yt = Org4Tab<C>::t.yx[t.wave] + B0;
+  double acc[PERS];
+#pragma unroll
+  for (int j = 0; j < PERS; ++j) acc[j] = block4<C>(src + ot[j], off, b);
+  if constexpr ((!LAST || STAGE) && C::NBUF == 1) __syncthreads();  // every wave has read this step's input
+#pragma unroll
+  for (int j = 0; j < PERS; ++j) {
+    if constexpr (!GLOBAL) {
+      bool ok = true;
+      if constexpr (MASK) {
+        const int yx = yt[j];
+        ok = in_interior(t, (yx >> 16) + t.orow, (yx & 0xffff) + t.ocol);
+      }
+      if (ok) *reinterpret_cast<double *>(dst + ot[j] + t.vw) = acc[j];
+    } else {
+      const int yx = yt[j], y = yx >> 16, x = yx & 0xffff;
+      bool ok = (unsigned)(t.ocol + x - C::LO(S)) < (unsigned)(C::HI(S) - C::LO(S));
+      if constexpr (MASK) ok = ok && in_interior(t, y + t.orow, x + t.ocol);
+      if (ok) *reinterpret_cast<double *>(reinterpret_cast<char *>(t.gout + (int64_t)y * t.pitch + x) + t.vgo) = acc[j];
+    }
+  }
+  if constexpr (!LAST || STAGE) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (STAGE) copy_out4<C, MASK>(t, dst);
+}
+
+template <class C, int S, bool MASK>
+__device__ __forceinline__ void steps4(const TileCtx &t, const double *b, const int *off, char *lds) {
+  if constexpr (S < C::KF) {
+    step4<C, S, MASK>(t, b, off, lds);
+    steps4<C, S + 1, MASK>(t, b, off, lds);
+  }
+}
+
+// A tile inside the domain interior into LDS, row by row: RPP = NT / PX rows per pass,
+// one thread per storage column (pad columns get 0).  The threads past RPP * PX repeat
+// the first row of the next pass (same cells, same values); in the last pass, threads
+// past the tile's rows load a valid cell and store it to the spare cell.
+template <class C>
+__device__ __forceinline__ void load_tile_full(const double *__restrict__ in, const TileCtx &t,
+                                               double *__restrict__ lds) {
+  constexpr int NT = C::NW * 64, PX = C::PX, RPP = NT / PX;
+  static_assert(RPP >= 1, "a pass covers at least one row");
+  constexpr int NP = (C::EY + RPP - 1) / RPP;
+  const int r0 = t.tid / PX, c = t.tid - r0 * PX, cc = c - C::CS;
+  const bool colok = cc >= 0 && cc < C::EX;
+  const bool lastok = r0 + RPP * (NP - 1) < C::EY;
+  const uint32_t voff = (uint32_t)((r0 * t.pitch + (colok ? cc : 0)) * 8);
+  const char *gb = reinterpret_cast<const char *>(in + t.ay0 * t.pitch + t.ax0);
+  const int64_t pstep = (int64_t)RPP * t.pitch * 8;
+  double v[NP];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const bool ok = i < NP - 1 || lastok;
+    const double x = *reinterpret_cast<const double *>(gb + i * pstep + (ok ? voff : 0u));
+    v[i] = colok ? x : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const bool ok = i < NP - 1 || lastok;
+    const int idx = (r0 + RPP * i) * PX + c;
+    lds[ok ? idx : C::SPARE] = v[i];
+    if constexpr (C::NBUF == 2) lds[ok ? C::BUF + idx : C::SPARE] = v[i];
+  }
+}
+
 // Copies the tile with its halo (EY x EX cells from (ay0, ax0)) into LDS, zero for cells
 // outside the array and for pad columns.  Each thread issues its loads in groups of LG
 // (all of them by default) before the group's first LDS store, so a whole group of the
@@ -332,20 +507,25 @@ __device__ __forceinline__ void lane_setup(const double *__restrict__ bfrag, int
     t.orow = 0;
     t.ocol = 0;
   } else {
+    // off[i]: byte offset of read i's cell for this lane from the block origin, set for
+    // the representative reads of the read plan only (the others are immediates)
     const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
 #pragma unroll
-    for (int ps = 0; ps < C::DT.np; ++ps) off[ps] = (2 * b4 + C::DT.pty[ps][q]) * PX + 2 * m + C::DT.ptx[ps][q] + C::CS;
-#pragma unroll
-    for (int ss = 0; ss < C::DT.ns; ++ss)
-      off[C::DT.np + ss] = (2 * b4 + C::DT.sty[ss][q]) * PX + 2 * m + C::DT.stx[ss][q] + C::CS;
+    for (int i = 0; i < C::RPL.n; ++i) {
+      if (C::RPL.rep[i] != i) continue;
+      const int ty = i < C::DT.np ? C::DT.pty[i][q] : C::DT.sty[i - C::DT.np][q];
+      const int tx = i < C::DT.np ? C::DT.ptx[i][q] : C::DT.stx[i - C::DT.np][q];
+      off[i] = ((2 * b4 + ty) * PX + 2 * m + tx + C::CS) * 8;
+    }
     t.orow = 2 * b4 + ((lane & 3) >> 1);
     t.ocol = 2 * (lane >> 4) + (lane & 1);
+    t.vw = (t.orow * PX + t.ocol + C::CS) * 8;
   }
 }
 
 // Per-tile fields of the context.
 template <class C>
-__device__ __forceinline__ void tile_setup(const Geo &g, int64_t tile, TileCtx &t) {
+__device__ __forceinline__ void tile_setup(const Geo &g, int64_t tile, TileCtx &t, double *out) {
   t.pitch = g.ext[2];
   t.ny = g.n[1];
   t.nx = g.n[2];
@@ -354,26 +534,61 @@ __device__ __forceinline__ void tile_setup(const Geo &g, int64_t tile, TileCtx &
   t.ay0 = g.org[1] + t.Y0;
   t.ax0 = g.org[2] + t.X0;
   t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
+  if constexpr (C::MF == 4) {
+    t.gout = out + t.ay0 * t.pitch + t.ax0;
+    t.vgo = (uint32_t)((t.orow * t.pitch + t.ocol) * 8);
+    const int64_t ry0 = t.Y0 < 0 ? -t.Y0 : 0, cx0 = t.X0 < 0 ? -t.X0 : 0;
+    const int64_t ry1 = t.ny - t.Y0 < C::EY ? t.ny - t.Y0 : C::EY, cx1 = t.nx - t.X0 < C::EX ? t.nx - t.X0 : C::EX;
+    t.ry0 = (int)ry0;
+    t.cx0 = (int)cx0;
+    t.nry = ry1 > ry0 ? (int)(ry1 - ry0) : 0;
+    t.ncx = cx1 > cx0 ? (int)(cx1 - cx0) : 0;
+  }
+}
+
+// The fused steps of a tile already in LDS (the persistent kernels' tile loop).
+template <class C>
+__device__ __forceinline__ void run_steps(const TileCtx &t, const double *b, const int *off, double *lds,
+                                          double *out) {
+  if constexpr (C::MF == 4) {
+    if (t.full)
+      steps4<C, 0, false>(t, b, off, reinterpret_cast<char *>(lds));
+    else
+      steps4<C, 0, true>(t, b, off, reinterpret_cast<char *>(lds));
+  } else {
+    fused_steps<C, 0>(t, b, off, lds, out);
+  }
 }
 
 // One workgroup advances tile `tile` of grid g by KF steps (lds: the kernel's dynamic LDS;
 // LG: tile loads in flight per thread; tid: threadIdx.x).
-template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST, int LG = 64>
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST, int LG = 64, int PXM = -1>
 __device__ __forceinline__ void tile_f64(const double *__restrict__ in, double *__restrict__ out,
                                          const double *__restrict__ bfrag, const Geo &g, int64_t tile,
                                          double *__restrict__ lds, int tid) {
-  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST, PXM>;
   TileCtx t;
   double b[C::NKS];
   int off[C::NOFF];
   lane_setup<C>(bfrag, tid, t, b, off);
-  tile_setup<C>(g, tile, t);
+  tile_setup<C>(g, tile, t, out);
 
   // ---- load the tile (and its halo); cells outside the array and pad columns are 0 ----
-  load_tile<C, LG>(in, g, t, lds);
-  __syncthreads();
-
-  fused_steps<C, 0>(t, b, off, lds, out);
+  if constexpr (C::MF == 4) {
+    if (t.full) {
+      load_tile_full<C>(in, t, lds);
+      __syncthreads();
+      steps4<C, 0, false>(t, b, off, reinterpret_cast<char *>(lds));
+    } else {
+      load_tile<C, LG>(in, g, t, lds);
+      __syncthreads();
+      steps4<C, 0, true>(t, b, off, reinterpret_cast<char *>(lds));
+    }
+  } else {
+    load_tile<C, LG>(in, g, t, lds);
+    __syncthreads();
+    fused_steps<C, 0>(t, b, off, lds, out);
+  }
 }
 
 }  // namespace cs

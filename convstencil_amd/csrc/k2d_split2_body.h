@@ -86,8 +86,9 @@ struct CfgS2 {
   }
   static constexpr int PX0 = CS + max_col() + 1;
   // a multiple of 8 elements (16-byte reads) with the fewest LDS bank conflicts for the
-  // chunk reads and the result writes (cs_ktable.h, lds_cost)
-  static constexpr int PX = best_pitch_ctable(CT, PX0, (int)sizeof(T));
+  // chunk reads and the result writes that keeps the most workgroups per CU
+  // (cs_ktable.h, lds_cost)
+  static constexpr int PX = best_pitch_ctable(CT, PX0, (int)sizeof(T), (long)NSU * EY * sizeof(T), 8 * sizeof(T));
   static constexpr int PLANE = EY * PX;
   static constexpr int BUF = NSU * PLANE;
   static constexpr int SPARE = BUF;  // scratch cells after the slice planes

@@ -88,7 +88,8 @@ struct Cfg3D2 : Tab3D<R_, SH_> {
   static constexpr int NB = NBY * NBX;
   static constexpr int PER = (NB + NW - 1) / NW;
   static constexpr int PX0 = CS + (LASTX + 6 + cmax() + 1 > EX ? LASTX + 6 + cmax() + 1 : EX);
-  static constexpr int PX = best_pitch_dtables(DT.t, 2 * R + 1, PX0);  // fewest LDS bank conflicts
+  // fewest LDS bank conflicts among the pitches that keep the most workgroups per CU
+  static constexpr int PX = best_pitch_dtables(DT.t, 2 * R + 1, PX0, (long)(2 * R + 1) * EY * 8, 16);
   static constexpr int PLANE = EY * PX;
   static constexpr int NP = 2 * R + 1;
   static constexpr int SPARE = NP * PLANE;
@@ -290,7 +291,8 @@ struct Cfg3F : Tab3D<R_, SH_> {
     return m;
   }
   static constexpr int PX0 = CS + max_col() + 1;
-  static constexpr int PX = best_pitch_dtables(DT.t, 2 * R + 1, PX0);  // fewest LDS bank conflicts
+  // fewest LDS bank conflicts among the pitches that keep the most workgroups per CU
+  static constexpr int PX = best_pitch_dtables(DT.t, 2 * R + 1, PX0, (long)KF * (2 * R + 1) * EY * 8, 16);
   static constexpr int PLANE = EY * PX;
   static constexpr int NP = 2 * R + 1;
   static constexpr int RING = NP * PLANE;

@@ -14,9 +14,11 @@ from refimpl import rel_errors
 
 pytestmark = pytest.mark.gpu
 
-# (stencil, interior sizes): partial tiles, a grid smaller than one tile, one exact tile
+# (stencil, interior sizes): partial tiles, a grid smaller than one tile, one exact tile,
+# and 3 x 3 tiles whose centre tile lies inside the interior (the unmasked tile path)
 GRIDS = [("box2d1r", [64, 64]), ("star2d3r", [100, 130]), ("box2d1r", [257, 75]), ("star2d3r", [31, 200]),
-         ("box2d1r", [3, 5]), ("star2d3r", [150, 64]), ("box2d1r", [130, 190])]
+         ("box2d1r", [3, 5]), ("star2d3r", [150, 64]), ("box2d1r", [130, 190]), ("box2d1r", [190, 203]),
+         ("star2d3r", [189, 200])]
 
 
 def _grid(name, sizes, seed, random_halo=False):

@@ -56,21 +56,23 @@ def test_parity_default_plan(gpu, name, sizes, steps):
 
 def _variant_ids():
     import test_kernel_emulation as em
-    return [(v["id"], v["name"], v["R"], v["shape"], v["kf"]) for v in em.VARIANTS]
+    return [(v["id"], v["name"], v["R"], v["shape"], v["kf"], v["by"], v["bx"]) for v in em.VARIANTS]
 
 
-@pytest.mark.parametrize("vid,vname,R,shape,kf", _variant_ids())
-def test_every_variant(gpu, vid, vname, R, shape, kf):
+@pytest.mark.parametrize("vid,vname,R,shape,kf,by,bx", _variant_ids())
+def test_every_variant(gpu, vid, vname, R, shape, kf, by, bx):
+    # 3 x 3 tiles: partial tiles at the far edges and a centre tile inside the interior
     print(f"variant {vname}", flush=True)
+    sizes = [3 * by - 3, 3 * bx - 5]
     name = {(1, 0): "box2d1r", (1, 1): "star2d1r", (3, 0): "box2d3r", (3, 1): "star2d3r"}.get((R, shape))
     if name is None:  # radius 2 has no named shape: box or star of radius 2
         offs = [(0, dy, dx) for dy in range(-2, 3) for dx in range(-2, 3) if shape == 0 or dy == 0 or dx == 0]
-        L = Layout.with_halo([97, 150], 2)
+        L = Layout.with_halo(sizes, 2)
         w = oracle.fill_weights(len(offs), vid)
         dense = dense_weights(2, 2, offs, w)
         x = oracle.fill_data(L, vid)
     else:
-        L, R, offs, w, dense, x = setup(name, [97, 150], vid)
+        L, R, offs, w, dense, x = setup(name, sizes, vid)
     for steps in (kf, 2 * kf + 1):
         with Plan(L, R, dense, variant=vid) as p:
             p.upload(x)

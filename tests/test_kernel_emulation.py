@@ -215,26 +215,33 @@ def emulate(v, layout, dense, x):
                 y = base + min(BH * ba, H - BH)
                 xx = min(lox + BW * bb, lastx)
                 idx = src + y * px + xx + offs
-                top = idx.max() + (1 if v["mf"] == 4 else 0)  # a pair's second half
+                # a pair read's second cell; single reads (after the pairs) end at their cell
+                top = max(idx[:tab[0]].max() + 1, idx.max()) if v["mf"] == 4 else idx.max()
                 assert idx.min() >= src and top < src + BUF, (v["name"], s, y, xx)
                 # every LDS row read stays inside the row (no wrap into the next row)
                 a = a_operands(v, tab, lds, idx)
                 assert np.isfinite(a).all(), (v["name"], "reads an unwritten LDS cell")
                 rows, cols, vals = mfma_block(v, a, bf)
-                for rr, cc, vv in zip(rows.reshape(-1), cols.reshape(-1), vals.reshape(-1)):
-                    ty, tx = y + rr, xx + cc
-                    inside = full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
-                    if not inside or tx < base or tx >= base + W:
-                        continue
-                    if s < kf - 1 or v["st"] == 2:  # ST 2 stages the last step in LDS too
-                        li = dst + ty * px + tx + CS
-                        assert dst <= li < dst + BUF
-                        new_vals.append((li, vv))
-                    else:
-                        gi = (ay0 + ty) * pitch + ax0 + tx
-                        assert 0 <= gi < layout.cells
-                        assert org[1] <= ay0 + ty < org[1] + n[1] and org[2] <= ax0 + tx < org[2] + n[2]
-                        out[gi] = vv
+                ty, tx, vv = y + rows.reshape(-1), xx + cols.reshape(-1), vals.reshape(-1)
+                inside = np.ones(ty.shape, dtype=bool) if full else \
+                    (Y0 + ty >= 0) & (Y0 + ty < n[1]) & (X0 + tx >= 0) & (X0 + tx < n[2])
+                to_lds = s < kf - 1 or v["st"] == 2  # ST 2 stages the last step in LDS too
+                if v["mf"] == 16 or not to_lds:
+                    # 16x16x4, and every step's grid writes: only the step's region columns
+                    inside &= (tx >= base) & (tx < base + W)
+                # 4x4x4 LDS writes are not limited to the region: cells a block computes
+                # outside it are never read by a later step's region
+                if to_lds:
+                    li = dst + ty * px + tx + CS
+                    assert np.all(li >= dst) and np.all(li < dst + BUF) and np.all(ty < EY), (v["name"], s)
+                    new_vals.append((li[inside], vv[inside]))
+                else:
+                    gi = (ay0 + ty) * pitch + ax0 + tx
+                    g = gi[inside]
+                    assert np.all(g >= 0) and np.all(g < layout.cells)
+                    assert np.all((ay0 + ty[inside] >= org[1]) & (ay0 + ty[inside] < org[1] + n[1]))
+                    assert np.all((ax0 + tx[inside] >= org[2]) & (ax0 + tx[inside] < org[2] + n[2]))
+                    out[g] = vv[inside]
             for li, vv in new_vals:  # the kernel's barrier separates reads of src from these writes
                 lds[li] = vv
         if v["st"] == 2:  # coalesced copy-out of the output block from LDS
@@ -271,9 +278,11 @@ def test_variant_emulation(v):
     R, kf = v["R"], v["kf"]
     dense = _dense(R, v["shape"], v["id"])
     # partial tiles on both axes, and the reference's padded layout (extra columns, origin != halo)
+    # plus 3 x 3 tiles, whose centre tile lies inside the interior (the unmasked path)
     for layout in [Layout.with_halo([v["by"] + 5, v["bx"] + 19], R),
                    Layout(2, [v["by"] // 2 + 2 * R + 3, v["bx"] + 2 * R + 2 + 7], [R + 1, R + 2],
-                          [v["by"] // 2, v["bx"] + 7])]:
+                          [v["by"] // 2, v["bx"] + 7]),
+                   Layout.with_halo([3 * v["by"] - 3, 3 * v["bx"] - 5], R)]:
         x = oracle.fill_data(layout, v["id"])
         got = emulate(v, layout, dense, x)
         offs, w = oracle.dense_to_taps(2, R, dense)
