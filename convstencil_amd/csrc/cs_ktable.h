@@ -467,6 +467,41 @@ constexpr CTable make_ctable(int R, int shape) {
   return best;
 }
 
+// Address plan of a split table's B reads at row pitch px (elements of esz bytes): read i
+// (instruction i, chunk 4 i + g in k lane group g) is at lane-invariant distance
+// delta[i] bytes (>= 0) from read rep[i], as RdPlan above for the FP64 tables.
+constexpr int ctable_cell(const CTable &t, int i, int g, int px) { return t.ty[4 * i + g] * px + t.tx[4 * i + g]; }
+
+constexpr RdPlan make_splan(const CTable &t, int px, int esz) {
+  RdPlan p{};
+  p.n = t.n / 4;
+  for (int i = 0; i < p.n; ++i) {
+    p.rep[i] = i;
+    for (int k = 0; k < i && p.rep[i] == i; ++k) {
+      if (p.rep[k] != k) continue;
+      const int d = ctable_cell(t, i, 0, px) - ctable_cell(t, k, 0, px);
+      bool same = true;
+      for (int g = 1; g < 4; ++g) same = same && ctable_cell(t, i, g, px) - ctable_cell(t, k, g, px) == d;
+      if (same) p.rep[i] = k;
+    }
+  }
+  for (int i = 0; i < p.n; ++i) {
+    if (p.rep[i] != i) continue;
+    int lo = i;
+    for (int k = i; k < p.n; ++k)
+      if (p.rep[k] == i && ctable_cell(t, k, 0, px) < ctable_cell(t, lo, 0, px)) lo = k;
+    for (int k = i; k < p.n; ++k)
+      if (p.rep[k] == i) p.rep[k] = -1 - lo;
+  }
+  for (int i = 0; i < p.n; ++i) {
+    p.rep[i] = -1 - p.rep[i];
+    p.delta[i] = (ctable_cell(t, i, 0, px) - ctable_cell(t, p.rep[i], 0, px)) * esz;
+  }
+  p.nrep = 0;
+  for (int i = 0; i < p.n; ++i) p.nrep += p.rep[i] == i ? 1 : 0;
+  return p;
+}
+
 }  // namespace cs
 
 namespace cs {

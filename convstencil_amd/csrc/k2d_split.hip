@@ -68,7 +68,7 @@ __global__ __launch_bounds__(NW * 64) void k2d_split2p(const double *__restrict_
   int off[C::NINS];
   split_lane_setup<C>(bfrag, threadIdx.x, t, w, off);
   SplitRegs<C> rg;
-  split_tile_setup<C>(g, (k % 8) * per + k / 8, t);
+  split_tile_setup<C>(g, (k % 8) * per + k / 8, t, out);
   issue_tile_split<C>(in, g, t, rg);
   for (;;) {
     store_tile_split<C>(t, rg, lds);
@@ -82,10 +82,10 @@ __global__ __launch_bounds__(NW * 64) void k2d_split2p(const double *__restrict_
     t.wave = wv;
     SplitCtx tn = t;
     if (kn < total) {
-      split_tile_setup<C>(g, (kn % 8) * per + kn / 8, tn);
+      split_tile_setup<C>(g, (kn % 8) * per + kn / 8, tn, out);
       issue_tile_split<C>(in, g, tn, rg);  // in flight while this tile is computed
     }
-    split_steps<C, 0>(t, w, off, lds, out);
+    run_split_steps<C>(t, w, off, lds);
     if (kn >= total) break;
     __syncthreads();  // every wave is done with this tile's LDS before the next tile is stored
     k = kn;
@@ -153,7 +153,8 @@ static hipError_t launch_ks(const double *in, double *out, const double *bfrag, 
         "k2d_split<" #TR ",r" #R ",kf" #KF "," #BY "x" #BX ",w" #NW ">"                                            \
   }
 
-// Dilated form: Variant::st = 2.  Ids 60000 + R * 1000 + precision * 100 + (star) * 10 + KF.
+// Dilated form: Variant::st = 2.  Ids 60000 + R * 1000 + precision * 100 + (star) * 10 + KF;
+// 700000 + that: staged output (OS = 1); 800000 + that: 128 x 64 tiles.
 #define CS_VS2O(ID, PREC, TR, R, SH, KF, BY, BX, NW, OS)                                                \
   Variant {                                                                                             \
     ID, PREC, R, SH, KF, BY, BX, NW, CfgS2<TR, R, SH, KF, BY, BX, NW, OS>::LDS,                         \
@@ -185,6 +186,8 @@ static const Variant kVariantsSplit[] = {
     CS_VS2(63102, 1, TrBF16, 3, SHAPE_BOX, 2, 64, 64, 8),
     CS_VS2(63103, 1, TrBF16, 3, SHAPE_BOX, 3, 64, 64, 8),
     CS_VS2(63104, 1, TrBF16, 3, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63105, 1, TrBF16, 3, SHAPE_BOX, 5, 64, 64, 8),
+    CS_VS2(63106, 1, TrBF16, 3, SHAPE_BOX, 6, 64, 64, 8),
     CS_VS2(63111, 1, TrBF16, 3, SHAPE_STAR, 1, 64, 64, 8),
     CS_VS2(63112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8),
     CS_VS2(63113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8),
@@ -196,6 +199,8 @@ static const Variant kVariantsSplit[] = {
     CS_VS2(63302, 3, TrF16, 3, SHAPE_BOX, 2, 64, 64, 8),
     CS_VS2(63303, 3, TrF16, 3, SHAPE_BOX, 3, 64, 64, 8),
     CS_VS2(63304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63305, 3, TrF16, 3, SHAPE_BOX, 5, 64, 64, 8),
+    CS_VS2(63306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 64, 8),
     CS_VS2(63311, 3, TrF16, 3, SHAPE_STAR, 1, 64, 64, 8),
     CS_VS2(63312, 3, TrF16, 3, SHAPE_STAR, 2, 64, 64, 8),
     // staged FP64 output of the last step (ids 700000 + the id of the same configuration)
@@ -203,6 +208,9 @@ static const Variant kVariantsSplit[] = {
     CS_VS2O(761104, 1, TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8, 1),
     CS_VS2O(763102, 1, TrBF16, 3, SHAPE_BOX, 2, 64, 64, 8, 1),
     CS_VS2O(763104, 1, TrBF16, 3, SHAPE_BOX, 4, 64, 64, 8, 1),
+    CS_VS2O(763106, 1, TrBF16, 3, SHAPE_BOX, 6, 64, 64, 8, 1),
+    // 128 x 64 tiles (one workgroup per CU, less halo work per output)
+    CS_VS2O(863104, 1, TrBF16, 3, SHAPE_BOX, 4, 128, 64, 8, 0),
     CS_VS2O(763112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8, 1),
     CS_VS2O(763113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
     CS_VS2O(763304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8, 1),

@@ -93,6 +93,15 @@ struct CfgS2 {
   static constexpr int BUF = NSU * PLANE;
   static constexpr int SPARE = BUF;  // scratch cells after the slice planes
   static constexpr size_t LDS = ((size_t)BUF + 8) * sizeof(T);
+  // the reads' shared vector addresses and immediate offsets at this pitch
+  static constexpr RdPlan RPL = make_splan(CT, PX, (int)sizeof(T));
+  // block `blk` of step s -> its first output cell (y, x); the last block row is moved
+  // back to end at the region's edge, the last block column may extend past it
+  static constexpr void origin(int s, int blk, int &y, int &x) {
+    const int ba = blk / NBX(s), bb = blk - ba * NBX(s);
+    y = LO(s) + (16 * ba < H(s) - 16 ? 16 * ba : H(s) - 16);
+    x = LOX(s) + 16 * bb;
+  }
   static_assert(BY % 16 == 0 && BX % 16 == 0, "tile must be a multiple of 16");
   static_assert(CS < 64 && (CS + LOX(0) + CT.c0) % 8 == 0, "block columns must align the chunk reads");
   static_assert(PX % 8 == 0, "16-byte reads need a pitch that is a multiple of 8 elements");
@@ -106,7 +115,69 @@ struct SplitCtx {
   int orow, ocol;  // output cells of this lane relative to a block: row orow, columns ocol .. ocol + 3
   int full;        // 1 if the whole tile (with halo) lies inside the domain interior
   int64_t Y0, X0, ny, nx, ay0, ax0, pitch;
+  int vw;                  // LDS byte offset of this lane's first result cell from the block origin
+  double *gout;            // output cell of tile cell (0, 0)
+  int ry0, nry, cx0, ncx;  // tile rows [ry0, ry0 + nry) and columns [cx0, cx0 + ncx) are interior
 };
+
+// Splits one FP32 value into the slices of Tr (exact for BF16x3).  The FP64 state is
+// rounded to FP32 once where a tile is loaded (the MFMAs accumulate in FP32 anyway) and
+// split with this: a direct FP64 -> BF16 conversion would need round-to-odd fix-ups.
+template <class Tr>
+__device__ __forceinline__ void split_f32(float v, typename Tr::T *s) {
+  using T = typename Tr::T;
+  if constexpr (Tr::NSU == 3) {
+    const T h = (T)v;
+    const float r1 = v - (float)h;
+    const T m = (T)r1;
+    s[0] = h;
+    s[1] = m;
+    s[2] = (T)(r1 - (float)m);
+  } else {
+    const T h = (T)v;
+    s[0] = h;
+    s[1] = (T)((v - (float)h) * (float)Tr::SCALE);
+  }
+}
+
+// Splits four FP32 values into slice quads (one 8-byte LDS store per slice).  BF16x3:
+// by truncation -- h = v with the low 16 bits cleared, m likewise from v - h, l = the
+// rest -- which is exact (v = h + m + l) for every normal v and costs three 32-bit
+// operations per value, the quads being packed with one byte permute per value pair.
+template <class Tr>
+__device__ __forceinline__ void split4(const float (&v)[4], typename Quad<Tr>::V (&q)[Tr::NSU]) {
+  using T = typename Tr::T;
+  if constexpr (Tr::NSU == 3) {
+    uint32_t s[3][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t h = __float_as_uint(v[e]) & 0xffff0000u;
+      const float r1 = v[e] - __uint_as_float(h);
+      const uint32_t m = __float_as_uint(r1) & 0xffff0000u;
+      const float r2 = r1 - __uint_as_float(m);
+      s[0][e] = h;
+      s[1][e] = m;
+      s[2][e] = __float_as_uint(r2);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      // high halves of (value 0, value 1) and (value 2, value 3): bytes 2,3 of each word
+      const uint32_t w0 = __builtin_amdgcn_perm(s[k][1], s[k][0], 0x07060302u);
+      const uint32_t w1 = __builtin_amdgcn_perm(s[k][3], s[k][2], 0x07060302u);
+      typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+      const u2v w = {w0, w1};
+      q[k] = __builtin_bit_cast(typename Quad<Tr>::V, w);
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      T s[Tr::NSU];
+      split_f32<Tr>(v[e], s);
+#pragma unroll
+      for (int k = 0; k < Tr::NSU; ++k) q[k][e] = s[k];
+    }
+  }
+}
 
 // Loads the FP64 tile (EY x EX cells, zero outside the array and in pad columns) and
 // stores its slices; loads in groups of LG per thread, all issued before the group's
@@ -138,10 +209,57 @@ __device__ __forceinline__ void load_tile_split(const double *__restrict__ in, c
       const int idx = t.tid + (i0 + j) * NT;
       const bool in_plane = i0 + j < NIT - 1 || (i0 + j == NIT - 1 && idx < C::PLANE);
       T s[C::NSU];
-      split_value<typename C::Tr>(v[j], s);
+      split_f32<typename C::Tr>((float)v[j], s);
 #pragma unroll
       for (int k = 0; k < C::NSU; ++k) lds[in_plane ? k * C::PLANE + idx : C::SPARE] = s[k];
     }
+  }
+}
+
+// A tile inside the domain interior: each thread takes four adjacent storage columns
+// of a row (QPR = PX / 4 threads per row, RPP = NT / QPR rows per pass) and stores each
+// slice of the four values with one 8-byte write; pad columns get 0.  The threads past
+// RPP * QPR repeat the first row of the next pass (same cells, same values); in the last
+// pass, threads past the tile's rows load valid cells and store to the spare cells.
+template <class C>
+__device__ __forceinline__ void load_tile_split_full(const double *__restrict__ in, const SplitCtx &t,
+                                                     typename C::T *__restrict__ lds) {
+  using T = typename C::T;
+  using Q = typename Quad<typename C::Tr>::V;
+  constexpr int NT = C::NW * 64, QPR = C::PX / 4, RPP = NT / QPR;
+  static_assert(C::PX % 4 == 0 && RPP >= 1, "quads tile the rows");
+  constexpr int NP = (C::EY + RPP - 1) / RPP;
+  const int r0 = t.tid / QPR, c4 = 4 * (t.tid - r0 * QPR);
+  const bool lastok = r0 + RPP * (NP - 1) < C::EY;
+  uint32_t voff[4];
+  bool colok[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int cc = c4 + e - C::CS;
+    colok[e] = cc >= 0 && cc < C::EX;
+    voff[e] = (uint32_t)((r0 * t.pitch + (colok[e] ? cc : 0)) * 8);
+  }
+  const char *gb = reinterpret_cast<const char *>(in + t.ay0 * t.pitch + t.ax0);
+  const int64_t pstep = (int64_t)RPP * t.pitch * 8;
+  double v[NP][4];
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const bool ok = i < NP - 1 || lastok;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const double x = *reinterpret_cast<const double *>(gb + i * pstep + (ok ? voff[e] : 0u));
+      v[i][e] = colok[e] ? x : 0.0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NP; ++i) {
+    const bool ok = i < NP - 1 || lastok;
+    Q q[C::NSU];
+    const float f[4] = {(float)v[i][0], (float)v[i][1], (float)v[i][2], (float)v[i][3]};
+    split4<typename C::Tr>(f, q);
+    const int idx = (r0 + RPP * i) * C::PX + c4;
+#pragma unroll
+    for (int k = 0; k < C::NSU; ++k) *reinterpret_cast<Q *>(lds + (ok ? k * C::PLANE + idx : C::SPARE)) = q[k];
   }
 }
 
@@ -182,14 +300,14 @@ __device__ __forceinline__ void store_tile_split(const SplitCtx &t, const SplitR
     const int idx = t.tid + i * NT;
     const bool in_plane = i < NIT - 1 || idx < C::PLANE;
     T sl[C::NSU];
-    split_value<typename C::Tr>(rg.v[i], sl);
+    split_f32<typename C::Tr>((float)rg.v[i], sl);
 #pragma unroll
     for (int k = 0; k < C::NSU; ++k) lds[in_plane ? k * C::PLANE + idx : C::SPARE] = sl[k];
   }
 }
 
 template <class C>
-__device__ __forceinline__ void split_tile_setup(const Geo &g, int64_t tile, SplitCtx &t) {
+__device__ __forceinline__ void split_tile_setup(const Geo &g, int64_t tile, SplitCtx &t, double *out) {
   t.pitch = g.ext[2];
   t.ny = g.n[1];
   t.nx = g.n[2];
@@ -198,41 +316,94 @@ __device__ __forceinline__ void split_tile_setup(const Geo &g, int64_t tile, Spl
   t.ay0 = g.org[1] + t.Y0;
   t.ax0 = g.org[2] + t.X0;
   t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
+  t.gout = out + t.ay0 * t.pitch + t.ax0;
+  const int64_t ry0 = t.Y0 < 0 ? -t.Y0 : 0, cx0 = t.X0 < 0 ? -t.X0 : 0;
+  const int64_t ry1 = t.ny - t.Y0 < C::EY ? t.ny - t.Y0 : C::EY, cx1 = t.nx - t.X0 < C::EX ? t.nx - t.X0 : C::EX;
+  t.ry0 = (int)ry0;
+  t.cx0 = (int)cx0;
+  t.nry = ry1 > ry0 ? (int)(ry1 - ry0) : 0;
+  t.ncx = cx1 > cx0 ? (int)(cx1 - cx0) : 0;
 }
 
+// Per-lane constants: A fragments (the weights' slices), the byte offsets of the
+// representative B reads, the lane's output cells.
 template <class C>
 __device__ __forceinline__ void split_lane_setup(const typename C::T *__restrict__ bfrag, int tid, SplitCtx &t,
                                                  typename C::Tr::V (*w)[C::NINS], int *off) {
   using V = typename C::Tr::V;
+  using T = typename C::T;
   t.tid = tid;
   t.lane = tid & 63;
   t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = t.lane, grp = lane >> 4, n = lane & 15;
+  // D[m][n], m = 4 grp + v: cell (2 (n & 7) + (grp >> 1), 8 (n >> 3) + 4 (grp & 1) + v)
   t.orow = 2 * (n & 7) + (grp >> 1);
   t.ocol = 8 * (n >> 3) + 4 * (grp & 1);
+  t.vw = (t.orow * C::PX + t.ocol + C::CS) * (int)sizeof(T);
 #pragma unroll
   for (int q = 0; q < C::NSW; ++q)
 #pragma unroll
     for (int i = 0; i < C::NINS; ++i)
       w[q][i] = *reinterpret_cast<const V *>(bfrag + ((size_t)(q * C::NINS + i) * 64 + lane) * 8);
+  // B reads: chunk 4 i + grp of instruction i, anchor u_n = (2 (n & 7), 8 (n >> 3))
 #pragma unroll
-  for (int i = 0; i < C::NINS; ++i)
-    off[i] = (2 * (n & 7) + C::CT.ty[4 * i + grp]) * C::PX + 8 * (n >> 3) + C::CT.tx[4 * i + grp] + C::CS;
+  for (int i = 0; i < C::NINS; ++i) {
+    if (C::RPL.rep[i] != i) continue;
+    off[i] = ((2 * (n & 7) + C::CT.ty[4 * i + grp]) * C::PX + 8 * (n >> 3) + C::CT.tx[4 * i + grp] + C::CS) *
+             (int)sizeof(T);
+  }
 }
 
-// One block: D0 = level 0 (hi x hi), D1 = the kept lower levels, scaled by SCALE.
+// Block origins of every fused step, per wave, as for the FP64 kernels (k2d_f64_body.h,
+// Org4): o = LDS byte offset of the block's first cell, yx = (y << 16) | x.
 template <class C>
-__device__ __forceinline__ void block_split(const typename C::T *p, const int *off,
-                                            const typename C::Tr::V (*w)[C::NINS], f4 &d0, f4 &d1) {
+struct OrgS {
+  static constexpr int base(int s) {
+    int n = 0;
+    for (int k = 0; k < s; ++k) n += C::PERS(k);
+    return n;
+  }
+  static constexpr int TOT = base(C::KF);
+  int o[C::NW][TOT];
+  int yx[C::NW][TOT];
+};
+
+template <class C>
+constexpr OrgS<C> make_orgs() {
+  OrgS<C> t{};
+  for (int w = 0; w < C::NW; ++w)
+    for (int s = 0; s < C::KF; ++s)
+      for (int j = 0; j < C::PERS(s); ++j) {
+        const int blk = j * C::NW + w < C::NB(s) ? j * C::NW + w : C::NB(s) - 1;
+        int y = 0, x = 0;
+        C::origin(s, blk, y, x);
+        t.o[w][OrgS<C>::base(s) + j] = (y * C::PX + x) * (int)sizeof(typename C::T);
+        t.yx[w][OrgS<C>::base(s) + j] = (y << 16) | x;
+      }
+  return t;
+}
+
+template <class C>
+struct OrgSTab {
+  static constexpr OrgS<C> t = make_orgs<C>();
+};
+
+// One block: D0 = level 0 (hi x hi), D1 = the kept lower levels, scaled by SCALE.
+// p = LDS address of the block origin (slice 0).
+template <class C>
+__device__ __forceinline__ void block_split(const char *p, const int *off, const typename C::Tr::V (*w)[C::NINS],
+                                            f4 &d0, f4 &d1) {
   using Tr = typename C::Tr;
   using V = typename Tr::V;
+  constexpr RdPlan P = C::RPL;
+  constexpr int SB = C::PLANE * (int)sizeof(typename C::T);  // bytes between slice planes
   d0 = f4{0.f, 0.f, 0.f, 0.f};
   d1 = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int i = 0; i < C::NINS; ++i) {
     V u[C::NSU];
 #pragma unroll
-    for (int s = 0; s < C::NSU; ++s) u[s] = *reinterpret_cast<const V *>(p + s * C::PLANE + off[i]);
+    for (int s = 0; s < C::NSU; ++s) u[s] = *reinterpret_cast<const V *>(p + off[P.rep[i]] + P.delta[i] + s * SB);
     d0 = Tr::mma(w[0][i], u[0], d0);
 #pragma unroll
     for (int s = 0; s < C::NSU; ++s)
@@ -242,93 +413,73 @@ __device__ __forceinline__ void block_split(const typename C::T *p, const int *o
   }
 }
 
-// Re-splits one FP32 value into the slices of Tr (exact for BF16x3).
-template <class Tr>
-__device__ __forceinline__ void split_f32(float v, typename Tr::T *s) {
-  using T = typename Tr::T;
-  if constexpr (Tr::NSU == 3) {
-    const T h = (T)v;
-    const float r1 = v - (float)h;
-    const T m = (T)r1;
-    s[0] = h;
-    s[1] = m;
-    s[2] = (T)(r1 - (float)m);
-  } else {
-    const T h = (T)v;
-    s[0] = h;
-    s[1] = (T)((v - (float)h) * (float)Tr::SCALE);
-  }
+__device__ __forceinline__ bool split_in_interior(const SplitCtx &t, int ty, int tx) {
+  return (unsigned)(ty - t.ry0) < (unsigned)t.nry && (unsigned)(tx - t.cx0) < (unsigned)t.ncx;
 }
 
-template <class C, int S>
-__device__ __forceinline__ void block_origin_s2(int blk, int &y, int &x) {
-  constexpr int NBX = C::NBX(S);
-  const int ba = blk / NBX, bb = blk - ba * NBX;
-  y = C::LO(S) + min(16 * ba, C::H(S) - 16);
-  x = C::LOX(S) + 16 * bb;
-}
-
-template <class C, int S>
+// Fused step S.  MASK = false (a tile inside the domain interior): intermediate results
+// are stored without tests -- cells a block computes beyond the step's region are never
+// read by a later step's region -- and the last step writes only the region's columns.
+// MASK = true: cells outside the domain interior keep their value.
+template <class C, int S, bool MASK>
 __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::Tr::V (*w)[C::NINS], const int *off,
-                                           typename C::T *lds, double *out) {
+                                           char *lds) {
   using Tr = typename C::Tr;
   using T = typename C::T;
   using Q = typename Quad<Tr>::V;
   constexpr bool LAST = S == C::KF - 1;
-  constexpr int NB = C::NB(S), PERS = C::PERS(S), NW = C::NW;
+  constexpr int PERS = C::PERS(S), B0 = OrgS<C>::base(S), NW = C::NW;
+  constexpr int SB = C::PLANE * (int)sizeof(T);
   constexpr float INV = 1.0f / (float)Tr::SCALE;
+  const int *ot = OrgSTab<C>::t.o[t.wave] + B0;
+  const int *yt = OrgSTab<C>::t.yx[t.wave] + B0;
   f4 d0[PERS], d1[PERS];
 #pragma unroll
-  for (int j = 0; j < PERS; ++j) {
-    int y, x;
-    block_origin_s2<C, S>(min(j * NW + t.wave, NB - 1), y, x);
-    block_split<C>(lds + y * C::PX + x, off, w, d0[j], d1[j]);
-  }
+  for (int j = 0; j < PERS; ++j) block_split<C>(lds + ot[j], off, w, d0[j], d1[j]);
   if constexpr (!LAST || C::OS == 1) __syncthreads();  // every wave has read this step's input
 #pragma unroll
   for (int j = 0; j < PERS; ++j) {
-    const int blk = j * NW + t.wave;
-    if (!((NB % NW == 0 || j < PERS - 1) || blk < NB)) continue;
-    int y, x;
-    block_origin_s2<C, S>(blk, y, x);
-    const int ty = y + t.orow, tx0 = x + t.ocol;
-    const bool row_in = t.full || (t.Y0 + ty >= 0 && t.Y0 + ty < t.ny);
-    if constexpr (LAST && C::OS == 1) {
-      // staged: the output block cell (ty - LO, tx - LO) of a compact BY x BX FP64 array
-      double *ob = reinterpret_cast<double *>(lds);
+    if constexpr (!LAST) {
+      Q q[C::NSU];
+      const float f[4] = {d0[j][0] + d1[j][0] * INV, d0[j][1] + d1[j][1] * INV, d0[j][2] + d1[j][2] * INV,
+                          d0[j][3] + d1[j][3] * INV};
+      split4<Tr>(f, q);
+      char *dst = lds + ot[j] + t.vw;
+      if constexpr (!MASK) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int tx = tx0 + v;
-        if (tx >= C::LO(S) && tx < C::HI(S))
-          ob[(ty - C::LO(S)) * C::BX + tx - C::LO(S)] = (double)d0[j][v] + (double)d1[j][v] * (double)INV;
-      }
-    } else if constexpr (LAST) {
+        for (int k = 0; k < C::NSU; ++k) *reinterpret_cast<Q *>(dst + k * SB) = q[k];
+      } else {
+        const int yx = yt[j], ty = (yx >> 16) + t.orow, tx0 = (yx & 0xffff) + t.ocol;
+        const bool row_in = (unsigned)(ty - t.ry0) < (unsigned)t.nry;
+        if (row_in && (unsigned)(tx0 - t.cx0) < (unsigned)t.ncx && (unsigned)(tx0 + 3 - t.cx0) < (unsigned)t.ncx) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int tx = tx0 + v;
-        const bool in = tx >= C::LO(S) && tx < C::HI(S) && row_in && (t.full || (t.X0 + tx >= 0 && t.X0 + tx < t.nx));
-        if (in) out[(t.ay0 + ty) * t.pitch + t.ax0 + tx] = (double)d0[j][v] + (double)d1[j][v] * (double)INV;
+          for (int k = 0; k < C::NSU; ++k) *reinterpret_cast<Q *>(dst + k * SB) = q[k];
+        } else if (row_in) {  // a block at the domain edge: keep the halo cells
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            if ((unsigned)(tx0 + v - t.cx0) < (unsigned)t.ncx)
+#pragma unroll
+              for (int k = 0; k < C::NSU; ++k) reinterpret_cast<T *>(dst + k * SB)[v] = q[k][v];
+        }
       }
     } else {
-      Q q[C::NSU];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        T s[C::NSU];
-        split_f32<Tr>(d0[j][v] + d1[j][v] * INV, s);
-#pragma unroll
-        for (int k = 0; k < C::NSU; ++k) q[k][v] = s[k];
-      }
-      const int li = ty * C::PX + tx0 + C::CS;
-      const bool all4 = t.full || (row_in && t.X0 + tx0 >= 0 && t.X0 + tx0 + 3 < t.nx);
-      if (all4) {
-#pragma unroll
-        for (int k = 0; k < C::NSU; ++k) *reinterpret_cast<Q *>(lds + k * C::PLANE + li) = q[k];
-      } else if (row_in) {  // a block at the domain edge: keep the halo cells
+      const int yx = yt[j], y = yx >> 16, x = yx & 0xffff, ty = y + t.orow, tx0 = x + t.ocol;
+      if constexpr (C::OS == 1) {
+        // staged: the output block cell (ty - LO, tx - LO) of a compact BY x BX FP64 array
+        double *ob = reinterpret_cast<double *>(lds) + (ty - C::LO(S)) * C::BX + tx0 - C::LO(S);
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          if (t.X0 + tx0 + v >= 0 && t.X0 + tx0 + v < t.nx)
+          if ((unsigned)(tx0 + v - C::LO(S)) < (unsigned)(C::HI(S) - C::LO(S)))
+            ob[v] = (double)d0[j][v] + (double)d1[j][v] * (double)INV;
+      } else {
+        double *o = t.gout + (int64_t)ty * t.pitch + tx0;
+        const bool row_in = !MASK || (unsigned)(ty - t.ry0) < (unsigned)t.nry;
 #pragma unroll
-            for (int k = 0; k < C::NSU; ++k) lds[k * C::PLANE + li + v] = q[k][v];
+        for (int v = 0; v < 4; ++v) {
+          bool in = row_in && (unsigned)(tx0 + v - C::LO(S)) < (unsigned)(C::HI(S) - C::LO(S));
+          if constexpr (MASK) in = in && (unsigned)(tx0 + v - t.cx0) < (unsigned)t.ncx;
+          if (in) o[v] = (double)d0[j][v] + (double)d1[j][v] * (double)INV;
+        }
       }
     }
   }
@@ -339,26 +490,37 @@ __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::
     static_assert(NT % C::BX == 0, "a pass covers whole rows");
     const double *ob = reinterpret_cast<const double *>(lds);
     const int r0 = t.tid / C::BX, c = t.tid - r0 * C::BX;
-    const bool col_in = t.full || t.X0 + OFF + c < t.nx;
-    double *o = out + (t.ay0 + OFF + r0) * t.pitch + t.ax0 + OFF + c;
+    const bool col_in = !MASK || (unsigned)(OFF + c - t.cx0) < (unsigned)t.ncx;
+    double *o = t.gout + (int64_t)(OFF + r0) * t.pitch + OFF + c;
 #pragma unroll
     for (int i = 0; i < C::BY / RP; ++i)
-      if (col_in && (t.full || t.Y0 + OFF + r0 + RP * i < t.ny)) o[(int64_t)RP * i * t.pitch] = ob[(r0 + RP * i) * C::BX + c];
+      if (!MASK || (col_in && (unsigned)(OFF + r0 + RP * i - t.ry0) < (unsigned)t.nry))
+        o[(int64_t)RP * i * t.pitch] = ob[(r0 + RP * i) * C::BX + c];
   }
 }
 
-template <class C, int S>
+template <class C, int S, bool MASK>
 __device__ __forceinline__ void split_steps(const SplitCtx &t, const typename C::Tr::V (*w)[C::NINS], const int *off,
-                                            typename C::T *lds, double *out) {
+                                            char *lds) {
   if constexpr (S < C::KF) {
-    split_step<C, S>(t, w, off, lds, out);
-    split_steps<C, S + 1>(t, w, off, lds, out);
+    split_step<C, S, MASK>(t, w, off, lds);
+    split_steps<C, S + 1, MASK>(t, w, off, lds);
   }
+}
+
+// The fused steps of a tile already in LDS (the persistent kernels' tile loop).
+template <class C>
+__device__ __forceinline__ void run_split_steps(const SplitCtx &t, const typename C::Tr::V (*w)[C::NINS],
+                                                const int *off, typename C::T *lds) {
+  if (t.full)
+    split_steps<C, 0, false>(t, w, off, reinterpret_cast<char *>(lds));
+  else
+    split_steps<C, 0, true>(t, w, off, reinterpret_cast<char *>(lds));
 }
 
 // One workgroup advances tile `tile` of grid g by KF steps (smem: the kernel's dynamic
-// LDS; bfrag: the weights' slices as A fragments; LG: tile loads in flight per thread;
-// tid: threadIdx.x).
+// LDS; bfrag: the weights' slices as A fragments; LG: tile loads in flight per thread
+// for tiles at the domain edge; tid: threadIdx.x).
 template <class Tr, int R, int SH, int KF, int BY, int BX, int NW, int LG = 64, int OS = 0>
 __device__ __forceinline__ void tile_split2(const double *__restrict__ in, double *__restrict__ out,
                                             const typename Tr::T *__restrict__ bfrag, const Geo &g, int64_t tile,
@@ -368,37 +530,19 @@ __device__ __forceinline__ void tile_split2(const double *__restrict__ in, doubl
   using V = typename Tr::V;
   T *lds = reinterpret_cast<T *>(smem);
   SplitCtx t;
-  t.tid = tid;
-  t.lane = tid & 63;
-  t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  t.pitch = g.ext[2];
-  t.ny = g.n[1];
-  t.nx = g.n[2];
-  t.Y0 = (tile / g.tiles[2]) * BY - KF * R;
-  t.X0 = (tile % g.tiles[2]) * BX - KF * R;
-  t.ay0 = g.org[1] + t.Y0;
-  t.ax0 = g.org[2] + t.X0;
-  t.full = t.Y0 >= 0 && t.X0 >= 0 && t.Y0 + C::EY <= t.ny && t.X0 + C::EX <= t.nx;
-  const int lane = t.lane, grp = lane >> 4, n = lane & 15;
-  // D[m][n], m = 4 grp + v: cell (2 (n & 7) + (grp >> 1), 8 (n >> 3) + 4 (grp & 1) + v)
-  t.orow = 2 * (n & 7) + (grp >> 1);
-  t.ocol = 8 * (n >> 3) + 4 * (grp & 1);
-
   V w[C::NSW][C::NINS];
-#pragma unroll
-  for (int q = 0; q < C::NSW; ++q)
-#pragma unroll
-    for (int i = 0; i < C::NINS; ++i)
-      w[q][i] = *reinterpret_cast<const V *>(bfrag + ((size_t)(q * C::NINS + i) * 64 + lane) * 8);
-  // B offsets: chunk 4 i + grp of instruction i, anchor u_n = (2 (n & 7), 8 (n >> 3))
   int off[C::NINS];
-#pragma unroll
-  for (int i = 0; i < C::NINS; ++i)
-    off[i] = (2 * (n & 7) + C::CT.ty[4 * i + grp]) * C::PX + 8 * (n >> 3) + C::CT.tx[4 * i + grp] + C::CS;
-
-  load_tile_split<C, LG>(in, g, t, lds);
-  __syncthreads();
-  split_steps<C, 0>(t, w, off, lds, out);
+  split_lane_setup<C>(bfrag, tid, t, w, off);
+  split_tile_setup<C>(g, tile, t, out);
+  if (t.full) {
+    load_tile_split_full<C>(in, t, lds);
+    __syncthreads();
+    split_steps<C, 0, false>(t, w, off, reinterpret_cast<char *>(smem));
+  } else {
+    load_tile_split<C, LG>(in, g, t, lds);
+    __syncthreads();
+    split_steps<C, 0, true>(t, w, off, reinterpret_cast<char *>(smem));
+  }
 }
 
 }  // namespace cs

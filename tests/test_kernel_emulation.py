@@ -948,9 +948,21 @@ def _f32_to(x, kind):
     return _round(np.asarray(x, dtype=np.float32), kind)
 
 
+def _trunc16(x):
+    """FP32 with the low 16 bits of its encoding cleared (a BF16 by truncation)."""
+    return (np.asarray(x, dtype=np.float32).view(np.uint32) & np.uint32(0xFFFF0000)).view(np.float32)
+
+
 def split_f32_np(v32, kind, ns, scale):
-    """The kernel's FP32 re-split (split_f32)."""
+    """The dilated split kernels' FP32 split (split4 / split_f32): BF16x3 by truncation
+    (exact), FP16x2 by rounding the high slice and the scaled remainder."""
     v32 = np.asarray(v32, dtype=np.float32)
+    if ns == 3 and kind == "bf16":
+        h = _trunc16(v32)
+        r1 = (v32 - h).astype(np.float32)
+        m = _trunc16(r1)
+        lo = _trunc16((r1 - m).astype(np.float32))
+        return [h, m, lo]
     if ns == 3:
         h = _f32_to(v32, kind)
         r1 = (v32 - h).astype(np.float32)
@@ -1009,7 +1021,8 @@ def emulate_split2(v, layout, dense, x):
             ay, ax = ay0 + r, ax0 + c
             if 0 <= c < EX and 0 <= ay < ext[1] and 0 <= ax < pitch:
                 vals[idx] = x3[0, ay, ax]
-        lds = [s_.astype(np.float32) for s_ in split_np(vals, kind, NSU, SCALE)]
+        # the FP64 tile is rounded to FP32 once and split like the step results
+        lds = [s_.astype(np.float32) for s_ in split_f32_np(vals.astype(np.float32), kind, NSU, SCALE)]
         staged = ",os1" in v["name"]
         ob = np.full(BY * BX, np.nan)
 
@@ -1088,16 +1101,20 @@ VARIANTS_SPLIT2 = [v for v in VARIANTS_SPLIT if v["st"] == 2 and (v["kf"] <= 2 o
 def test_variant_emulation_split2(v):
     R = v["R"]
     dense = _dense(R, v["shape"], v["id"])
-    layout = Layout(2, [v["by"] // 2 + 2 * R + 3, v["bx"] + 2 * R + 9], [R + 1, R + 2], [v["by"] // 2, v["bx"] + 7])
-    x = oracle.fill_data(layout, v["id"])
-    got = emulate_split2(v, layout, dense, x)
-    offs, w = oracle.dense_to_taps(2, R, dense)
-    want = oracle.run(layout, offs, w, x, v["kf"])
-    gi, wi = layout.interior(got), layout.interior(want)
-    err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
-    assert err < 2e-6, (v["name"], err)
-    # halo cells of the output are never written
-    assert np.all(got[~layout.interior_mask()] == 0.0)
+    # the reference's padded layout (partial tiles), and 3 x 3 tiles whose centre tile
+    # lies inside the interior (the unmasked path)
+    for layout in [Layout(2, [v["by"] // 2 + 2 * R + 3, v["bx"] + 2 * R + 9], [R + 1, R + 2],
+                          [v["by"] // 2, v["bx"] + 7]),
+                   Layout.with_halo([3 * v["by"] - 3, 3 * v["bx"] - 5], R)]:
+        x = oracle.fill_data(layout, v["id"])
+        got = emulate_split2(v, layout, dense, x)
+        offs, w = oracle.dense_to_taps(2, R, dense)
+        want = oracle.run(layout, offs, w, x, v["kf"])
+        gi, wi = layout.interior(got), layout.interior(want)
+        err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+        assert err < 2e-6, (v["name"], err)
+        # halo cells of the output are never written
+        assert np.all(got[~layout.interior_mask()] == 0.0)
 
 
 def test_ctables_cover_every_tap():
