@@ -95,10 +95,10 @@ def unfused_hbm(L, r, dense, T, seed, graph):
 
 
 def split_comparison(seed, graph, n=8192, T=64):
-    """box2d3r n x n, halo 3, T steps: the FP64 path and the BF16x3 split form, both with
-    the autotuner's choice, on the same seeded grid; the split result's relative L2 and
-    max error against the FP64 result (the FP64 path matches the CPU oracle to 1e-12
-    relative max in the test suite)."""
+    """box2d3r n x n, halo 3, T steps: the FP64 path and the split forms (BF16x3; FP16
+    with three weight slices, fp16w3), each with the autotuner's choice, on the same
+    seeded grid; each split result's relative L2 and max error against the FP64 result
+    (the FP64 path matches the CPU oracle to 1e-12 relative max in the test suite)."""
     import convstencil_amd as cs
     from convstencil_amd import Layout, Plan, shape_offsets, dense_weights
     L = Layout.with_halo([n, n], 3)
@@ -106,7 +106,8 @@ def split_comparison(seed, graph, n=8192, T=64):
     w = cs.fill_weights_host(len(offs), seed)
     dense = dense_weights(2, 3, offs, w)
     res, out = {}, {}
-    for prec in ("fp64", "bf16x3"):
+    splits = ("bf16x3", "fp16w3")
+    for prec in ("fp64",) + splits:
         with Plan(L, 3, dense, precision=prec, use_graph=graph, autotune=1) as p:
             p.fill_random(seed)
             rep = p.time(T, warmup=1, reps=3)
@@ -115,10 +116,11 @@ def split_comparison(seed, graph, n=8192, T=64):
             out[prec] = L.interior(p.download()).copy()
             res[prec] = {"gstencil_per_s": round(L.points * T / (rep.elapsed_ms * 1e-3) / 1e9, 2),
                          "kernel": p.describe().split()[1].split("=", 1)[1]}
-    d = out["bf16x3"] - out["fp64"]
-    res["bf16x3"]["rel_l2_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(out["fp64"] ** 2)))
-    res["bf16x3"]["rel_max_vs_fp64"] = float(np.max(np.abs(d)) / np.max(np.abs(out["fp64"])))
-    res["speedup_bf16x3_over_fp64"] = round(res["bf16x3"]["gstencil_per_s"] / res["fp64"]["gstencil_per_s"], 3)
+    for prec in splits:
+        d = out[prec] - out["fp64"]
+        res[prec]["rel_l2_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(out["fp64"] ** 2)))
+        res[prec]["rel_max_vs_fp64"] = float(np.max(np.abs(d)) / np.max(np.abs(out["fp64"])))
+        res[f"speedup_{prec}_over_fp64"] = round(res[prec]["gstencil_per_s"] / res["fp64"]["gstencil_per_s"], 3)
     res["workload"] = f"box2d3r {n}x{n} halo 3 T={T}"
     return res
 
@@ -199,7 +201,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="library defaults instead of the autotuner's choice")
     ap.add_argument("--no-extras", action="store_true", help="skip the unfused and split-precision lines")
     ap.add_argument("--seed", type=int, default=1234)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "bf16x3", "fp16x2", "fp32x2"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "bf16x3", "fp16x2", "fp16w3", "fp32x2"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="N",
                     help="only run N plain device copies of 2^27 doubles (reference for HBM byte counters)")
     args = ap.parse_args()

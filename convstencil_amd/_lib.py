@@ -11,7 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libconvstencil.so")
 
 CS_OK, CS_ERR_INVALID, CS_ERR_UNSUPPORTED, CS_ERR_ALLOC, CS_ERR_DEVICE, CS_ERR_NO_DEVICE = range(6)
-CS_PREC_FP64, CS_PREC_BF16X3, CS_PREC_FP32X2, CS_PREC_FP16X2 = range(4)
+CS_PREC_FP64, CS_PREC_BF16X3, CS_PREC_FP32X2, CS_PREC_FP16X2, CS_PREC_FP16W3 = range(5)
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
 

@@ -14,7 +14,7 @@ from ._lib import check
 from .shapes import Layout
 
 PRECISIONS = {"fp64": _lib.CS_PREC_FP64, "bf16x3": _lib.CS_PREC_BF16X3, "fp32x2": _lib.CS_PREC_FP32X2,
-              "fp16x2": _lib.CS_PREC_FP16X2}
+              "fp16x2": _lib.CS_PREC_FP16X2, "fp16w3": _lib.CS_PREC_FP16W3}
 
 
 def _dptr(a):

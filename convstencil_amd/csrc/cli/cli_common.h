@@ -14,7 +14,7 @@
 //   --graph=0|1      capture the time loop in a hipGraph (default 1)
 //   --autotune=0|1|2 physical mode: 1 = the autotuner's cached choice (timed on a miss),
 //                    2 = time the variants again (default 0: library defaults)
-//   --precision=P    physical mode, 2-D: fp64 (default), bf16x3, fp32x2 or fp16x2
+//   --precision=P    physical mode, 2-D: fp64 (default), bf16x3, fp32x2, fp16x2 or fp16w3
 //   --json           also print a JSON record (physical mode)
 //   --exact-fusion   2-D reference mode: compose the 3x3 weights exactly instead of
 //                    using the reference's 49-entry table
@@ -69,6 +69,7 @@ inline bool strip_flags(int &argc, char **argv, Ext &e) {
         else if (pn == "bf16x3") e.precision = CS_PREC_BF16X3;
         else if (pn == "fp32x2") e.precision = CS_PREC_FP32X2;
         else if (pn == "fp16x2") e.precision = CS_PREC_FP16X2;
+        else if (pn == "fp16w3") e.precision = CS_PREC_FP16W3;
         else {
           std::cerr << "Invalid argument: unknown precision " << pn << ".\n";
           return false;

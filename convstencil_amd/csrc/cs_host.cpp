@@ -107,7 +107,7 @@ cs_status select_variants(cs_plan *p) {
   const int nd = p->ndim;
   const int prec = p->opt.precision;
   if (nd < 1 || nd > 3) return CS_ERR_UNSUPPORTED;
-  if (prec != CS_PREC_FP64 && (prec < CS_PREC_BF16X3 || prec > CS_PREC_FP16X2)) return CS_ERR_INVALID;
+  if (prec != CS_PREC_FP64 && (prec < CS_PREC_BF16X3 || prec > CS_PREC_FP16W3)) return CS_ERR_INVALID;
   if (prec != CS_PREC_FP64 && nd != 2) return CS_ERR_UNSUPPORTED;
   // Split precision (Variant::mf holds the precision): the dilated form (st 2) for the
   // stencil's shape class, else its box variants (zero weights off the stencil), else the

@@ -7,6 +7,13 @@
 
 namespace cs {
 
+// Workgroup barrier for data that waves exchange through LDS only: waits for this wave's
+// LDS (and scalar) operations, then for the other waves.  __syncthreads() also waits for
+// every outstanding global load, which would stall each fused step behind loads issued
+// ahead for the next tile.  The memory clobber keeps the compiler from moving memory
+// accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -17,6 +24,39 @@ struct Geo {
   int64_t n[3];      // interior sizes
   int64_t tiles[3];  // tiles per axis for this launch
 };
+
+// Prefetch of a tile into L2 for the persistent kernels: one 8-byte load per thread and
+// pass touches every 128-byte line of the tile's rows (EY rows, EX columns from
+// (ay0, ax0), clamped to the array).  The loaded values go to pf[] and must be consumed
+// (prefetch_consume) after the work they should overlap; nothing waits for them before.
+template <class C>
+struct Prefetch {
+  static constexpr int NT = C::NW * 64;
+  static constexpr int LPR = (C::EX + 15) / 16 + 1;  // lines per row (a row may straddle one more)
+  static constexpr int RPP = NT / LPR;
+  static constexpr int N = (C::EY + RPP - 1) / RPP;
+};
+
+template <class C, class Ctx>
+__device__ __forceinline__ void prefetch_tile(const double *__restrict__ in, const Geo &g, const Ctx &t,
+                                              double (&pf)[Prefetch<C>::N]) {
+  using P = Prefetch<C>;
+  const int r0 = t.tid / P::LPR, k = t.tid - r0 * P::LPR;
+  int64_t ax = t.ax0 + 16 * k < t.ax0 + C::EX - 1 ? t.ax0 + 16 * k : t.ax0 + C::EX - 1;
+  ax = ax < 0 ? 0 : (ax > t.pitch - 1 ? t.pitch - 1 : ax);
+#pragma unroll
+  for (int i = 0; i < P::N; ++i) {
+    int64_t ay = t.ay0 + r0 + (int64_t)P::RPP * i;
+    ay = ay < 0 ? 0 : (ay > g.ext[1] - 1 ? g.ext[1] - 1 : ay);
+    pf[i] = in[ay * t.pitch + ax];
+  }
+}
+
+template <class C>
+__device__ __forceinline__ void prefetch_consume(const double (&pf)[Prefetch<C>::N]) {
+#pragma unroll
+  for (int i = 0; i < Prefetch<C>::N; ++i) asm volatile("" ::"v"(pf[i]));
+}
 
 typedef hipError_t (*LaunchFn)(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s);
 

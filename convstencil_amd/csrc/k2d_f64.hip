@@ -33,9 +33,11 @@ static hipError_t launch_k2d(const double *in, double *out, const double *bfrag,
 // ---- persistent form: a fixed grid walks the tiles, prefetching the next one ----
 //
 // As many workgroups as fit on the device at once (a multiple of 8, so that each stays
-// on one XCD's band of tiles) take tiles w, w + G, w + 2G, ... in the XCD-aware order;
-// while a workgroup computes one tile, all loads of its next tile are in flight in
-// registers, so HBM traffic overlaps the MFMAs and barriers of the fused steps.
+// on one XCD's band of tiles) take tiles w, w + G, w + 2G, ... in the XCD-aware order.
+// While a workgroup computes one tile, one load per thread touches every cache line of
+// its next tile, so that tile's own load (at the start of its turn) comes from L2 rather
+// than HBM; the fused steps synchronise through LDS only (lds_barrier), so nothing waits
+// for those loads until the tile is done.
 
 // Next position >= k (step G) whose XCD-ordered tile exists; returns total if none.
 __device__ __forceinline__ int64_t next_tile_pos(int64_t k, int64_t G, int64_t total, int64_t ntiles) {
@@ -56,13 +58,8 @@ __global__ __launch_bounds__(NW * 64) void k2d_f64p(const double *__restrict__ i
   double b[C::NKS];
   int off[C::NOFF];
   lane_setup<C>(bfrag, threadIdx.x, t, b, off);
-  TileRegs<C> rg;
   tile_setup<C>(g, xcd_tile(k, ntiles), t, out);
-  issue_tile<C>(in, g, t, rg);
   for (;;) {
-    store_tile<C>(t, rg, lds);
-    __syncthreads();
-    const int64_t kn = next_tile_pos(k + G, G, total, ntiles);
     // opaque copies of the thread and wave indices: keep the compiler from hoisting the
     // per-element load addresses and every block's addresses out of the tile loop
     // (they would all stay live in registers)
@@ -70,14 +67,27 @@ __global__ __launch_bounds__(NW * 64) void k2d_f64p(const double *__restrict__ i
     asm volatile("" : "+v"(tid), "+s"(w));
     t.tid = tid;
     t.wave = w;
+    if constexpr (C::MF == 4) {
+      if (t.full)
+        load_tile_full<C>(in, t, lds);
+      else
+        load_tile<C, 64>(in, g, t, lds);
+    } else {
+      load_tile<C, 64>(in, g, t, lds);
+    }
+    lds_barrier();
+    const int64_t kn = next_tile_pos(k + G, G, total, ntiles);
+    const bool more = kn < total;
     TileCtx tn = t;
-    if (kn < total) {
+    double pf[Prefetch<C>::N];
+    if (more) {
       tile_setup<C>(g, xcd_tile(kn, ntiles), tn, out);
-      issue_tile<C>(in, g, tn, rg);  // in flight while this tile is computed
+      prefetch_tile<C>(in, g, tn, pf);  // in flight while this tile is computed
     }
     run_steps<C>(t, b, off, lds, out);
-    if (kn >= total) break;
-    __syncthreads();  // every wave is done with this tile's LDS before the next tile is stored
+    if (!more) break;
+    prefetch_consume<C>(pf);
+    lds_barrier();  // every wave is done with this tile's LDS before the next tile is stored
     k = kn;
     t = tn;
   }

@@ -218,7 +218,7 @@ __device__ __forceinline__ void fused_step(const TileCtx &t, const double *b, co
     acc[j] = block_mfma<C>(src + y * C::PX + x, off, b);
   }
   constexpr bool STAGE = LAST && C::ST == 2;  // last step through LDS, then coalesced rows
-  if constexpr ((!LAST || STAGE) && C::NBUF == 1) __syncthreads();  // every wave has read this step's input
+  if constexpr ((!LAST || STAGE) && C::NBUF == 1) lds_barrier();  // every wave has read this step's input
 #pragma unroll
   for (int j = 0; j < PERS; ++j) {
     const int blk = j * NW + t.wave;
@@ -228,7 +228,7 @@ __device__ __forceinline__ void fused_step(const TileCtx &t, const double *b, co
       put_block<C, LAST && !STAGE>(acc[j], y, x, C::LO(S), C::HI(S), t, dst, out);
     }
   }
-  if constexpr (!LAST || STAGE) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (!LAST || STAGE) lds_barrier();  // this step's output is complete before it is read
   if constexpr (STAGE) {
     // the BY x BX output block, NT / BX rows per pass, one row segment per BX lanes;
     // cells outside the domain interior are skipped (they were not written)
@@ -351,7 +351,7 @@ __device__ __forceinline__ void step4(const TileCtx &t, const double *b, const i
   double acc[PERS];
 #pragma unroll
   for (int j = 0; j < PERS; ++j) acc[j] = block4<C>(src + ot[j], off, b);
-  if constexpr ((!LAST || STAGE) && C::NBUF == 1) __syncthreads();  // every wave has read this step's input
+  if constexpr ((!LAST || STAGE) && C::NBUF == 1) lds_barrier();  // every wave has read this step's input
 #pragma unroll
   for (int j = 0; j < PERS; ++j) {
     if constexpr (!GLOBAL) {
@@ -368,7 +368,7 @@ __device__ __forceinline__ void step4(const TileCtx &t, const double *b, const i
       if (ok) *reinterpret_cast<double *>(reinterpret_cast<char *>(t.gout + (int64_t)y * t.pitch + x) + t.vgo) = acc[j];
     }
   }
-  if constexpr (!LAST || STAGE) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (!LAST || STAGE) lds_barrier();  // this step's output is complete before it is read
   if constexpr (STAGE) copy_out4<C, MASK>(t, dst);
 }
 
@@ -458,7 +458,9 @@ template <class C>
 struct TileRegs {
   static constexpr int NT = C::NW * 64;
   static constexpr int NIT = (C::BUF + NT - 1) / NT;
+  static_assert(NIT <= 64, "one validity bit per load");
   double v[NIT];
+  uint64_t ok;  // bit i: v[i] is a cell of the array (else it is stored as 0)
 };
 
 template <class C>
@@ -467,14 +469,16 @@ __device__ __forceinline__ void issue_tile(const double *__restrict__ in, const 
   constexpr int NT = TileRegs<C>::NT, NIT = TileRegs<C>::NIT;
   const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
   const double *base = in + t.ay0 * t.pitch + t.ax0;
+  // the loaded values are not touched until store_tile (a select here would wait for them)
+  rg.ok = 0;
 #pragma unroll
   for (int i = 0; i < NIT; ++i) {
     const int idx = t.tid + i * NT;
     const int r = idx / C::PX, c = idx - r * C::PX - C::CS;
     const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
     const bool ok = idx < C::BUF && c >= 0 && c < C::EX && (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch));
-    const double x = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
-    rg.v[i] = ok ? x : 0.0;
+    rg.v[i] = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
+    rg.ok |= (uint64_t)ok << i;
   }
 }
 
@@ -485,8 +489,9 @@ __device__ __forceinline__ void store_tile(const TileCtx &t, const TileRegs<C> &
   for (int i = 0; i < NIT; ++i) {
     const int idx = t.tid + i * NT;
     const bool in_buf = i < NIT - 1 || idx < C::BUF;
-    lds[in_buf ? idx : C::SPARE] = rg.v[i];
-    if constexpr (C::NBUF == 2) lds[in_buf ? C::BUF + idx : C::SPARE] = rg.v[i];
+    const double v = (rg.ok >> i) & 1 ? rg.v[i] : 0.0;
+    lds[in_buf ? idx : C::SPARE] = v;
+    if constexpr (C::NBUF == 2) lds[in_buf ? C::BUF + idx : C::SPARE] = v;
   }
 }
 
@@ -577,16 +582,16 @@ __device__ __forceinline__ void tile_f64(const double *__restrict__ in, double *
   if constexpr (C::MF == 4) {
     if (t.full) {
       load_tile_full<C>(in, t, lds);
-      __syncthreads();
+      lds_barrier();
       steps4<C, 0, false>(t, b, off, reinterpret_cast<char *>(lds));
     } else {
       load_tile<C, LG>(in, g, t, lds);
-      __syncthreads();
+      lds_barrier();
       steps4<C, 0, true>(t, b, off, reinterpret_cast<char *>(lds));
     }
   } else {
     load_tile<C, LG>(in, g, t, lds);
-    __syncthreads();
+    lds_barrier();
     fused_steps<C, 0>(t, b, off, lds, out);
   }
 }

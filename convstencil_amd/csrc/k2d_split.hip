@@ -45,7 +45,8 @@ static hipError_t launch_ks2(const double *in, double *out, const double *bfrag,
 }
 
 // Persistent dilated form: a fixed grid (a multiple of 8) walks the tiles in the
-// XCD-aware order, keeping the next tile's loads in flight while computing the current.
+// XCD-aware order; while a tile is computed, one load per thread touches every cache line
+// of the next tile (cs_internal.h, prefetch_tile), so its load comes from L2.
 __device__ __forceinline__ int64_t next_tile_pos_s(int64_t k, int64_t G, int64_t total, int64_t ntiles) {
   while (k < total && (k % 8) * ((ntiles + 7) / 8) + k / 8 >= ntiles) k += G;
   return k;
@@ -67,27 +68,31 @@ __global__ __launch_bounds__(NW * 64) void k2d_split2p(const double *__restrict_
   V w[C::NSW][C::NINS];
   int off[C::NINS];
   split_lane_setup<C>(bfrag, threadIdx.x, t, w, off);
-  SplitRegs<C> rg;
   split_tile_setup<C>(g, (k % 8) * per + k / 8, t, out);
-  issue_tile_split<C>(in, g, t, rg);
   for (;;) {
-    store_tile_split<C>(t, rg, lds);
-    __syncthreads();
-    const int64_t kn = next_tile_pos_s(k + G, G, total, ntiles);
     // opaque copies of the thread and wave indices: keep the compiler from hoisting the
     // per-element and per-block addresses out of the tile loop
     int tid = t.tid, wv = t.wave;
     asm volatile("" : "+v"(tid), "+s"(wv));
     t.tid = tid;
     t.wave = wv;
+    if (t.full)
+      load_tile_split_full<C>(in, t, lds);
+    else
+      load_tile_split<C, 64>(in, g, t, lds);
+    lds_barrier();
+    const int64_t kn = next_tile_pos_s(k + G, G, total, ntiles);
+    const bool more = kn < total;
     SplitCtx tn = t;
-    if (kn < total) {
+    double pf[Prefetch<C>::N];
+    if (more) {
       split_tile_setup<C>(g, (kn % 8) * per + kn / 8, tn, out);
-      issue_tile_split<C>(in, g, tn, rg);  // in flight while this tile is computed
+      prefetch_tile<C>(in, g, tn, pf);  // in flight while this tile is computed
     }
     run_split_steps<C>(t, w, off, lds);
-    if (kn >= total) break;
-    __syncthreads();  // every wave is done with this tile's LDS before the next tile is stored
+    if (!more) break;
+    prefetch_consume<C>(pf);
+    lds_barrier();  // every wave is done with this tile's LDS before the next tile is stored
     k = kn;
     t = tn;
   }
@@ -145,7 +150,8 @@ static hipError_t launch_ks(const double *in, double *out, const double *bfrag, 
   return hipGetLastError();
 }
 
-// Variant::mf carries the precision (1 = BF16x3, 2 = FP32x2, 3 = FP16x2); st = 0.
+// Variant::mf carries the precision (1 = BF16x3, 2 = FP32x2, 3 = FP16x2, 4 = FP16 with
+// three weight slices); st = 0.
 #define CS_VS(ID, PREC, TR, R, KF, BY, BX, NW)                                                                     \
   Variant {                                                                                                        \
     ID, PREC, R, SHAPE_BOX, KF, BY, BX, NW, CfgS<TR, R, KF, BY, BX, NW>::LDS, CfgS<TR, R, KF, BY, BX, NW>::PR, 0,  \
@@ -154,7 +160,8 @@ static hipError_t launch_ks(const double *in, double *out, const double *bfrag, 
   }
 
 // Dilated form: Variant::st = 2.  Ids 60000 + R * 1000 + precision * 100 + (star) * 10 + KF;
-// 700000 + that: staged output (OS = 1); 800000 + that: 128 x 64 tiles.
+// 700000 + that: staged output (OS = 1); 800000 + that: 128 x 64 tiles; 900000 + that:
+// 128 x 128 tiles with 16 waves.
 #define CS_VS2O(ID, PREC, TR, R, SH, KF, BY, BX, NW, OS)                                                \
   Variant {                                                                                             \
     ID, PREC, R, SH, KF, BY, BX, NW, CfgS2<TR, R, SH, KF, BY, BX, NW, OS>::LDS,                         \
@@ -202,6 +209,19 @@ static const Variant kVariantsSplit[] = {
     CS_VS2(63305, 3, TrF16, 3, SHAPE_BOX, 5, 64, 64, 8),
     CS_VS2(63306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 64, 8),
     CS_VS2(63311, 3, TrF16, 3, SHAPE_STAR, 1, 64, 64, 8),
+    CS_VS2(61401, 4, TrF16W3, 1, SHAPE_BOX, 1, 64, 64, 8),
+    CS_VS2(61402, 4, TrF16W3, 1, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2(61404, 4, TrF16W3, 1, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63401, 4, TrF16W3, 3, SHAPE_BOX, 1, 64, 64, 8),
+    CS_VS2(63402, 4, TrF16W3, 3, SHAPE_BOX, 2, 64, 64, 8),
+    CS_VS2(63403, 4, TrF16W3, 3, SHAPE_BOX, 3, 64, 64, 8),
+    CS_VS2(63404, 4, TrF16W3, 3, SHAPE_BOX, 4, 64, 64, 8),
+    CS_VS2(63405, 4, TrF16W3, 3, SHAPE_BOX, 5, 64, 64, 8),
+    CS_VS2(63406, 4, TrF16W3, 3, SHAPE_BOX, 6, 64, 64, 8),
+    CS_VS2(63411, 4, TrF16W3, 3, SHAPE_STAR, 1, 64, 64, 8),
+    CS_VS2(63412, 4, TrF16W3, 3, SHAPE_STAR, 2, 64, 64, 8),
+    CS_VS2(63413, 4, TrF16W3, 3, SHAPE_STAR, 3, 64, 64, 8),
+    CS_VS2(63414, 4, TrF16W3, 3, SHAPE_STAR, 4, 64, 64, 8),
     CS_VS2(63312, 3, TrF16, 3, SHAPE_STAR, 2, 64, 64, 8),
     // staged FP64 output of the last step (ids 700000 + the id of the same configuration)
     CS_VS2O(761102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8, 1),
@@ -211,9 +231,19 @@ static const Variant kVariantsSplit[] = {
     CS_VS2O(763106, 1, TrBF16, 3, SHAPE_BOX, 6, 64, 64, 8, 1),
     // 128 x 64 tiles (one workgroup per CU, less halo work per output)
     CS_VS2O(863104, 1, TrBF16, 3, SHAPE_BOX, 4, 128, 64, 8, 0),
+    CS_VS2O(863406, 4, TrF16W3, 3, SHAPE_BOX, 6, 128, 64, 8, 0),
+    // 128 x 128 tiles, 16 waves (one workgroup per CU; two-slice state leaves room for
+    // deep fusion)
+    CS_VS2O(963104, 1, TrBF16, 3, SHAPE_BOX, 4, 128, 128, 16, 0),
+    CS_VS2O(963404, 4, TrF16W3, 3, SHAPE_BOX, 4, 128, 128, 16, 0),
+    CS_VS2O(963406, 4, TrF16W3, 3, SHAPE_BOX, 6, 128, 128, 16, 0),
+    CS_VS2O(963408, 4, TrF16W3, 3, SHAPE_BOX, 8, 128, 128, 16, 0),
+    CS_VS2O(963414, 4, TrF16W3, 3, SHAPE_STAR, 4, 128, 128, 16, 0),
     CS_VS2O(763112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8, 1),
     CS_VS2O(763113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
     CS_VS2O(763304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8, 1),
+    CS_VS2O(763404, 4, TrF16W3, 3, SHAPE_BOX, 4, 64, 64, 8, 1),
+    CS_VS2O(763406, 4, TrF16W3, 3, SHAPE_BOX, 6, 64, 64, 8, 1),
     // persistent dilated form (ids 500000 + the id of the same configuration)
     CS_VS2P(561102, 1, TrBF16, 1, SHAPE_BOX, 2, 64, 64, 8),
     CS_VS2P(561104, 1, TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8),
@@ -274,6 +304,26 @@ static size_t build_split(int R, const double *w, unsigned char *out, size_t cap
 
 // A fragments of the dilated form: weight slice q, instruction i, lane l, element j is
 // W[t - v_m] with m = l & 15, chunk 4 i + (l >> 4) and t = (ty, tx0 + j).
+// The weights' slices of the dilated form: as the state (split_value) with NSW slices,
+// except for TrF16W3, whose third slice is the remainder after two slices stored at
+// the second slice's scale (its products join the level-1 accumulator).
+template <class Tr>
+static void split_weight(double v, typename Tr::T *sl) {
+  using T = typename Tr::T;
+  if constexpr (Tr::NSU == 2 && Tr::NSW == 3) {
+    const T h = (T)(float)v;
+    const double r = (v - (double)(float)h) * Tr::SCALE;
+    const T m = (T)(float)r;
+    sl[0] = h;
+    sl[1] = m;
+    sl[2] = (T)(float)(r - (double)(float)m);
+  } else {
+    T s[Tr::NSU];
+    split_value<Tr>(v, s);
+    for (int k = 0; k < Tr::NSW; ++k) sl[k] = s[k];
+  }
+}
+
 template <class Tr>
 static size_t build_split2(int R, int shape, const double *w, unsigned char *out, size_t cap) {
   using T = typename Tr::T;
@@ -291,8 +341,8 @@ static size_t build_split2(int R, int shape, const double *w, unsigned char *out
           const int dy = ct.ty[c] - (m >> 3), dx = ct.tx[c] + j - (m & 7);
           double v = 0.0;
           if (ct.live[c] && dy >= -R && dy <= R && dx >= -R && dx <= R) v = w[(dy + R) * D + (dx + R)];
-          T sl[Tr::NSU];
-          split_value<Tr>(v, sl);
+          T sl[3];
+          split_weight<Tr>(v, sl);
           o[((size_t)(q * nins + i) * 64 + lane) * 8 + j] = sl[q];
         }
   return bytes;
@@ -302,6 +352,7 @@ size_t build_bfrag_split2(int prec, int R, int shape, const double *w, void *out
   unsigned char *o = reinterpret_cast<unsigned char *>(out);
   if (prec == 1) return build_split2<TrBF16>(R, shape, w, o, cap);
   if (prec == 3) return build_split2<TrF16>(R, shape, w, o, cap);
+  if (prec == 4) return build_split2<TrF16W3>(R, shape, w, o, cap);
   return 0;
 }
 

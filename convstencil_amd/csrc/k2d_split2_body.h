@@ -37,6 +37,10 @@ template <>
 struct Quad<TrF16> {
   using V = f16x4;
 };
+template <>
+struct Quad<TrF16W3> {
+  using V = f16x4;
+};
 
 // OS = 1: the last step's FP64 results are staged in LDS (over the slice planes, which
 // are no longer read) and written out row by row, coalesced.
@@ -269,7 +273,9 @@ template <class C>
 struct SplitRegs {
   static constexpr int NT = C::NW * 64;
   static constexpr int NIT = (C::PLANE + NT - 1) / NT;
+  static_assert(NIT <= 64, "one validity bit per load");
   double v[NIT];
+  uint64_t ok;  // bit i: v[i] is a cell of the array (else it is stored as 0)
 };
 
 template <class C>
@@ -278,6 +284,8 @@ __device__ __forceinline__ void issue_tile_split(const double *__restrict__ in, 
   constexpr int NT = SplitRegs<C>::NT, NIT = SplitRegs<C>::NIT;
   const bool all_in = t.ay0 >= 0 && t.ay0 + C::EY <= g.ext[1] && t.ax0 >= 0 && t.ax0 + C::EX <= t.pitch;
   const double *base = in + t.ay0 * t.pitch + t.ax0;
+  // the loaded values are not touched until store_tile_split (a select here would wait)
+  rg.ok = 0;
 #pragma unroll
   for (int i = 0; i < NIT; ++i) {
     const int idx = t.tid + i * NT;
@@ -285,8 +293,8 @@ __device__ __forceinline__ void issue_tile_split(const double *__restrict__ in, 
     const int64_t ay = t.ay0 + r, ax = t.ax0 + c;
     const bool ok = idx < C::PLANE && c >= 0 && c < C::EX &&
                     (all_in || (ay >= 0 && ay < g.ext[1] && ax >= 0 && ax < t.pitch));
-    const double x = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
-    rg.v[i] = ok ? x : 0.0;
+    rg.v[i] = *(ok ? base + ((int64_t)r * t.pitch + c) : in);
+    rg.ok |= (uint64_t)ok << i;
   }
 }
 
@@ -300,7 +308,7 @@ __device__ __forceinline__ void store_tile_split(const SplitCtx &t, const SplitR
     const int idx = t.tid + i * NT;
     const bool in_plane = i < NIT - 1 || idx < C::PLANE;
     T sl[C::NSU];
-    split_f32<typename C::Tr>((float)rg.v[i], sl);
+    split_f32<typename C::Tr>((rg.ok >> i) & 1 ? (float)rg.v[i] : 0.0f, sl);
 #pragma unroll
     for (int k = 0; k < C::NSU; ++k) lds[in_plane ? k * C::PLANE + idx : C::SPARE] = sl[k];
   }
@@ -409,7 +417,7 @@ __device__ __forceinline__ void block_split(const char *p, const int *off, const
     for (int s = 0; s < C::NSU; ++s)
 #pragma unroll
       for (int q = 0; q < C::NSW; ++q)
-        if (s + q >= 1 && s + q < Tr::NL) d1 = Tr::mma(w[q][i], u[s], d1);
+        if (keep_product<Tr>(s, q)) d1 = Tr::mma(w[q][i], u[s], d1);
   }
 }
 
@@ -436,7 +444,7 @@ __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::
   f4 d0[PERS], d1[PERS];
 #pragma unroll
   for (int j = 0; j < PERS; ++j) block_split<C>(lds + ot[j], off, w, d0[j], d1[j]);
-  if constexpr (!LAST || C::OS == 1) __syncthreads();  // every wave has read this step's input
+  if constexpr (!LAST || C::OS == 1) lds_barrier();  // every wave has read this step's input
 #pragma unroll
   for (int j = 0; j < PERS; ++j) {
     if constexpr (!LAST) {
@@ -483,9 +491,9 @@ __device__ __forceinline__ void split_step(const SplitCtx &t, const typename C::
       }
     }
   }
-  if constexpr (!LAST) __syncthreads();  // this step's output is complete before it is read
+  if constexpr (!LAST) lds_barrier();  // this step's output is complete before it is read
   if constexpr (LAST && C::OS == 1) {
-    __syncthreads();  // the staged block is complete
+    lds_barrier();  // the staged block is complete
     constexpr int NT = NW * 64, RP = NT / C::BX, OFF = C::KF * C::R;
     static_assert(NT % C::BX == 0, "a pass covers whole rows");
     const double *ob = reinterpret_cast<const double *>(lds);
@@ -536,11 +544,11 @@ __device__ __forceinline__ void tile_split2(const double *__restrict__ in, doubl
   split_tile_setup<C>(g, tile, t, out);
   if (t.full) {
     load_tile_split_full<C>(in, t, lds);
-    __syncthreads();
+    lds_barrier();
     split_steps<C, 0, false>(t, w, off, reinterpret_cast<char *>(smem));
   } else {
     load_tile_split<C, LG>(in, g, t, lds);
-    __syncthreads();
+    lds_barrier();
     split_steps<C, 0, true>(t, w, off, reinterpret_cast<char *>(smem));
   }
 }

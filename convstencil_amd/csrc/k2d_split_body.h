@@ -43,6 +43,18 @@ struct TrF16 {
   static constexpr double SCALE = 2048.0;
   __device__ static f4 mma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
 };
+// FP16 with three weight slices and two state slices: the weights carry 33 bits, the
+// state 22, and four products are kept (w0 u0; w0 u1, w1 u0; w2 u0) -- the state's
+// rounding error stays random from step to step, while a two-slice weight would bias
+// every step the same way.  The third weight slice is stored at the second's scale
+// (the host's split), so its product shares the level-1 accumulator.
+struct TrF16W3 {
+  using T = _Float16;
+  using V = f16x8;
+  static constexpr int CK = 8, KI = 32, NSU = 2, NSW = 3, NL = 3;
+  static constexpr double SCALE = 2048.0;
+  __device__ static f4 mma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
 struct TrF32 {
   using T = float;
   using V = float;
@@ -50,6 +62,15 @@ struct TrF32 {
   static constexpr double SCALE = 1.0;
   __device__ static f4 mma(V a, V b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 };
+
+// Slice products the dilated form keeps below level 0 (state slice s times weight
+// slice q, level s + q): every level below NL, except w1 u1 for TrF16W3.
+template <class Tr>
+constexpr bool keep_product(int s, int q) {
+  if (s + q < 1 || s + q >= Tr::NL) return false;
+  if constexpr (Tr::NSU == 2 && Tr::NSW == 3) return s == 0 || q == 0;
+  return true;
+}
 
 // Splits v into NS slices of Tr::T; slice s is scaled by SCALE^s before rounding.
 template <class Tr>

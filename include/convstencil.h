@@ -49,8 +49,10 @@ typedef enum cs_precision {
   CS_PREC_BF16X3 = 1, /* 2-D: state/weights split into 3 BF16 slices, BF16 MFMA, FP32 accumulate per level
                          (6 of 9 slice products), FP64 recombine; state FP64 in HBM */
   CS_PREC_FP32X2 = 2, /* 2-D: FP32 hi/lo slices, FP32 MFMA (3 products), FP64 recombine */
-  CS_PREC_FP16X2 = 3  /* 2-D: 2 FP16 slices rescaled by 2^11 (3 products), FP32 accumulate, FP64
+  CS_PREC_FP16X2 = 3, /* 2-D: 2 FP16 slices rescaled by 2^11 (3 products), FP32 accumulate, FP64
                          recombine; values must stay below 2^15 in magnitude */
+  CS_PREC_FP16W3 = 4  /* 2-D: weights in 3 FP16 slices, state in 2 (4 products: the weights' error no
+                         longer biases every step), FP32 accumulate, FP64 recombine; values below 2^15 */
 } cs_precision;
 
 /* Padded array layout.  Dimensions are listed slowest first; a 2-D grid has

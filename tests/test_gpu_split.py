@@ -2,8 +2,9 @@
 
 The bar is the stated error, not bitwise: relative L2 <= 1e-6 and relative max
 <= 5e-6 after the stated number of steps (north_star: box2d3r relative L2 <= 1e-6
-after 64 steps).  BF16x3 and FP16x2 are the fast forms, FP32x2 the hi/lo
-reference form; all keep the state in FP64 between launches.
+after 64 steps).  BF16x3, FP16x2 and FP16 with three weight slices (fp16w3) are
+the fast forms, FP32x2 the hi/lo reference form; all keep the state in FP64 between
+launches.
 """
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ from oracle import oracle
 from refimpl import rel_errors
 
 pytestmark = pytest.mark.gpu
-PRECS = ["bf16x3", "fp16x2", "fp32x2"]
+PRECS = ["bf16x3", "fp16x2", "fp16w3", "fp32x2"]
 
 
 def _setup(name, sizes, seed):
@@ -44,13 +45,14 @@ def test_split_error(gpu, prec, name, sizes, steps):
 
 
 # Bar after 64 steps of box2d3r.  BF16x3 carries 24 significand bits and meets 1e-6
-# with margin (measured 7.5e-8).  FP16x2 carries 22: dropping the 2^-22 remainder every
-# step is a random walk of about sqrt(64) * 1.4e-7 over 64 steps, and it measures
-# 1.17e-6, so its stated bound is 2e-6, not 1e-6.
-BOUND_64 = {"bf16x3": 1e-6, "fp16x2": 2e-6}
+# with margin (measured 3.7e-8).  FP16x2 measures 1.17e-6: its two-slice weights are
+# wrong by up to 2^-23 relative, the same way in every step, so the error grows with
+# the step count; its stated bound is 2e-6.  FP16 with three weight slices removes that
+# bias (the state's 22-bit rounding is random from step to step) and meets 1e-6.
+BOUND_64 = {"bf16x3": 1e-6, "fp16x2": 2e-6, "fp16w3": 1e-6}
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2", "fp16w3"])
 def test_box2d3r_64_steps(gpu, prec):
     # box2d3r, halo 3, T = 64 on a 1024^2 grid, full oracle comparison
     L, r, offs, w, dense, _ = _setup("box2d3r", [8, 8], 64)
@@ -91,25 +93,28 @@ def _split_variants():
         info = (ctypes.c_int64 * 12)()
         name = ctypes.create_string_buffer(128)
         lib.cs_debug_variant_info(4, i, info, name, 128)
-        out.append((int(info[0]), int(info[1]), int(info[2]), int(info[3]), name.value.decode()))
+        out.append((int(info[0]), int(info[1]), int(info[2]), int(info[3]), int(info[5]), int(info[6]),
+                    name.value.decode()))
     return out
 
 
 SPLIT_VARIANTS = _split_variants()
-PREC_NAME = {1: "bf16x3", 2: "fp32x2", 3: "fp16x2"}
+PREC_NAME = {1: "bf16x3", 2: "fp32x2", 3: "fp16x2", 4: "fp16w3"}
 
 
-@pytest.mark.parametrize("vid,prec,R,shape,name", SPLIT_VARIANTS, ids=[v[4] for v in SPLIT_VARIANTS])
-def test_every_split_variant(gpu, vid, prec, R, shape, name):
-    # each compiled split variant, forced by id, on a grid with partial tiles, 9 steps
+@pytest.mark.parametrize("vid,prec,R,shape,by,bx,name", SPLIT_VARIANTS, ids=[v[-1] for v in SPLIT_VARIANTS])
+def test_every_split_variant(gpu, vid, prec, R, shape, by, bx, name):
+    # each compiled split variant, forced by id, 9 steps: on a grid with partial tiles and
+    # on 3 x 3 tiles whose centre tile lies inside the interior
     print(f"case {name}", flush=True)
     stencil = {(1, 0): "box2d1r", (1, 1): "star2d1r", (3, 0): "box2d3r", (3, 1): "star2d3r"}[(R, shape)]
-    L, r, offs, w, dense, x = _setup(stencil, [150, 211], vid)
-    with Plan(L, r, dense, precision=PREC_NAME[prec], variant=vid) as p:
-        p.upload(x)
-        p.run(9)
-        got = p.download()
-        assert f"variant={vid}" in p.describe()
-    want = oracle.run(L, offs, w, x, 9)
-    rm, rl2 = rel_errors(L.interior(got), L.interior(want))
-    assert rl2 <= 1e-6 and rm <= 5e-6, (rm, rl2)
+    for sizes in ([150, 211], [3 * by - 3, 3 * bx - 5]):
+        L, r, offs, w, dense, x = _setup(stencil, sizes, vid)
+        with Plan(L, r, dense, precision=PREC_NAME[prec], variant=vid) as p:
+            p.upload(x)
+            p.run(9)
+            got = p.download()
+            assert f"variant={vid}" in p.describe()
+        want = oracle.run(L, offs, w, x, 9)
+        rm, rl2 = rel_errors(L.interior(got), L.interior(want))
+        assert rl2 <= 1e-6 and rm <= 5e-6, (sizes, rm, rl2)

@@ -771,7 +771,17 @@ def test_variant_emulation_3d(v):
 
 # ---------------------------------------------------------------- 2-D split precision
 
-SPLIT = {1: ("bf16", 8, 32, 3, 3, 3, 1.0), 2: ("f32", 1, 4, 2, 2, 2, 1.0), 3: ("f16", 8, 32, 2, 2, 2, 2048.0)}
+SPLIT = {1: ("bf16", 8, 32, 3, 3, 3, 1.0), 2: ("f32", 1, 4, 2, 2, 2, 1.0), 3: ("f16", 8, 32, 2, 2, 2, 2048.0),
+         4: ("f16", 8, 32, 2, 3, 3, 2048.0)}
+
+
+def keep_product(prec, s, q):
+    """Slice products (state slice s, weight slice q) the dilated split form keeps below
+    level 0: every level below NL, except w1 u1 for FP16 with three weight slices."""
+    NSU, NSW, NL = SPLIT[prec][3], SPLIT[prec][4], SPLIT[prec][5]
+    if not 1 <= s + q < NL:
+        return False
+    return s == 0 or q == 0 if (NSU, NSW) == (2, 3) else True
 
 
 def _to_f32(raw, kind):
@@ -1055,7 +1065,7 @@ def emulate_split2(v, layout, dense, x):
                     d0 = (d0 + (A[0, i] @ B[0]).astype(np.float32)).astype(np.float32)
                     for s in range(NSU):
                         for q in range(NSW):
-                            if 1 <= s + q < NL:
+                            if keep_product(prec, s, q):
                                 d1 = (d1 + (A[q, i] @ B[s]).astype(np.float32)).astype(np.float32)
                 for ln in range(64):
                     ty = y + orow[ln]
