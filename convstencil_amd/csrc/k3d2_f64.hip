@@ -182,16 +182,20 @@ __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ i
   const int orow = 2 * ((lane >> 2) & 3) + ((lane & 3) >> 1), ocol = 2 * (lane >> 4) + (lane & 1);
 
   // plane p (interior z) of the tile's column, cell idx of the LDS plane layout
+  // (the loaded values are not touched until store(): a select here would wait for them)
+  static_assert(C::PF <= 64, "one validity bit per load");
+  uint64_t okm = 0;
   auto issue = [&](int64_t p, double *v) {
     const double *base = in + ((g.org[0] + p) * ext1 + ay0) * ext2 + ax0;
+    okm = 0;
 #pragma unroll
     for (int i = 0; i < C::PF; ++i) {
       const int idx = tid + i * NT;
       const int r = idx / PX, c = idx - r * PX - C::CS;
       const bool ok = idx < C::PLANE && c >= 0 && c < C::EX &&
                       (all_in || (ay0 + r >= 0 && ay0 + r < ext1 && ax0 + c >= 0 && ax0 + c < ext2));
-      const double x = *(ok ? base + ((int64_t)r * ext2 + c) : in);
-      v[i] = ok ? x : 0.0;
+      v[i] = *(ok ? base + ((int64_t)r * ext2 + c) : in);
+      okm |= (uint64_t)ok << i;
     }
   };
   auto store = [&](int s, const double *v) {
@@ -199,7 +203,7 @@ __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ i
 #pragma unroll
     for (int i = 0; i < C::PF; ++i) {
       const int idx = tid + i * NT;
-      dst[(i < C::PF - 1 || idx < C::PLANE) ? idx : C::SPARE - s * C::PLANE] = v[i];
+      dst[(i < C::PF - 1 || idx < C::PLANE) ? idx : C::SPARE - s * C::PLANE] = (okm >> i) & 1 ? v[i] : 0.0;
     }
   };
   auto slot_of = [&](int64_t p) -> int { return (int)((p - (Z0 - R)) % C::NP); };
@@ -214,7 +218,7 @@ __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ i
 
   for (int64_t z = Z0; z < Z1; ++z) {
     store(slot_of(z + R), pf);
-    __syncthreads();  // planes z - R .. z + R are in the ring
+    lds_barrier();  // planes z - R .. z + R are in the ring
     if (z + 1 < Z1) issue(z + R + 1, pf);
     int slot[C::NP];
 #pragma unroll
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ i
           oplane[(ay0 + ty) * ext2 + ax0 + tx] = acc[j];
       }
     }
-    __syncthreads();  // every wave is done with plane z - R before its slot is refilled
+    lds_barrier();  // every wave is done with plane z - R before its slot is refilled
   }
 }
 
@@ -357,7 +361,7 @@ __device__ __forceinline__ void fused_level(const FusedCtx &t, int64_t p, double
         }
       }
     }
-    __syncthreads();  // level L of plane q is complete before level L + 1 reads it
+    lds_barrier();  // level L of plane q is complete before level L + 1 reads it
     fused_level<C, L + 1>(t, p, lds, out, off, b);
   }
 }
@@ -417,8 +421,12 @@ __global__ __launch_bounds__(NW * 64) void k3f_f64(const double *__restrict__ in
   for (int idx = t.tid; idx < (KF - 1) * C::RING; idx += NT) lds[C::RING + idx] = 0.0;
 
   // input plane p (interior z), zero outside the array
+  // (the loaded values are not touched until store(): a select here would wait for them)
+  static_assert(C::PF <= 64, "one validity bit per load");
+  uint64_t okm = 0;
   auto issue = [&](int64_t p, double *v) {
     const bool plane_ok = t.org0 + p >= 0 && t.org0 + p < g.ext[0];
+    okm = 0;
 #pragma unroll
     for (int i = 0; i < C::PF; ++i) {
       const int idx = t.tid + i * NT;
@@ -426,8 +434,8 @@ __global__ __launch_bounds__(NW * 64) void k3f_f64(const double *__restrict__ in
       const bool ok = plane_ok && idx < C::PLANE && c >= 0 && c < C::EX &&
                       (all_in || (t.ay0 + r >= 0 && t.ay0 + r < t.ext1 && t.ax0 + c >= 0 && t.ax0 + c < t.ext2));
       const int64_t o = ((t.org0 + p) * t.ext1 + t.ay0 + r) * t.ext2 + t.ax0 + c;
-      const double x = *(ok ? in + o : in);
-      v[i] = ok ? x : 0.0;
+      v[i] = *(ok ? in + o : in);
+      okm |= (uint64_t)ok << i;
     }
   };
   auto store = [&](int64_t p, const double *v) {
@@ -435,7 +443,7 @@ __global__ __launch_bounds__(NW * 64) void k3f_f64(const double *__restrict__ in
 #pragma unroll
     for (int i = 0; i < C::PF; ++i) {
       const int idx = t.tid + i * NT;
-      dst[(i < C::PF - 1 || idx < C::PLANE) ? idx : C::SPARE - (int)(dst - lds)] = v[i];
+      dst[(i < C::PF - 1 || idx < C::PLANE) ? idx : C::SPARE - (int)(dst - lds)] = (okm >> i) & 1 ? v[i] : 0.0;
     }
   };
 
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(NW * 64) void k3f_f64(const double *__restrict__ in
   issue(p0, pf);
   for (int64_t p = p0; p <= p1; ++p) {
     store(p, pf);
-    __syncthreads();  // input plane p is in ring 0 (and, at p0, the zeroed rings are visible)
+    lds_barrier();  // input plane p is in ring 0 (and, at p0, the zeroed rings are visible)
     if (p < p1) issue(p + 1, pf);
     fused_level<C, 1>(t, p, lds, out, off, b);
   }
