@@ -94,21 +94,23 @@ def unfused_hbm(L, r, dense, T, seed, graph):
             "hbm_gbs_alg": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
-def split_comparison(seed, graph, n=8192, T=64):
-    """box2d3r n x n, halo 3, T steps: the FP64 path and the split forms (BF16x3; FP16
-    with three weight slices, fp16w3), each with the autotuner's choice, on the same
-    seeded grid; each split result's relative L2 and max error against the FP64 result
-    (the FP64 path matches the CPU oracle to 1e-12 relative max in the test suite)."""
+def split_comparison(seed, graph, shape="box2d3r", sizes=(8192, 8192), T=64):
+    """One workload (default box2d3r 8192^2, halo 3, T = 64) on the FP64 path and the
+    split forms (BF16x3; FP16 with three weight slices, fp16w3), each with the
+    autotuner's choice, on the same seeded grid; each split result's relative L2 and max
+    error against the FP64 result (the FP64 path matches the CPU oracle to 1e-12
+    relative max in the test suite)."""
     import convstencil_amd as cs
-    from convstencil_amd import Layout, Plan, shape_offsets, dense_weights
-    L = Layout.with_halo([n, n], 3)
-    offs = shape_offsets("box2d3r")
+    from convstencil_amd import Layout, Plan, shape_offsets, dense_weights, SHAPES
+    ndim, r, kind = SHAPES[shape]
+    L = Layout.with_halo(list(sizes), r)
+    offs = shape_offsets(shape)
     w = cs.fill_weights_host(len(offs), seed)
-    dense = dense_weights(2, 3, offs, w)
+    dense = dense_weights(ndim, r, offs, w)
     res, out = {}, {}
     splits = ("bf16x3", "fp16w3")
     for prec in ("fp64",) + splits:
-        with Plan(L, 3, dense, precision=prec, use_graph=graph, autotune=1) as p:
+        with Plan(L, r, dense, precision=prec, use_graph=graph, autotune=1) as p:
             p.fill_random(seed)
             rep = p.time(T, warmup=1, reps=3)
             p.fill_random(seed)
@@ -121,7 +123,7 @@ def split_comparison(seed, graph, n=8192, T=64):
         res[prec]["rel_l2_vs_fp64"] = float(np.sqrt(np.sum(d * d) / np.sum(out["fp64"] ** 2)))
         res[prec]["rel_max_vs_fp64"] = float(np.max(np.abs(d)) / np.max(np.abs(out["fp64"])))
         res[f"speedup_{prec}_over_fp64"] = round(res[prec]["gstencil_per_s"] / res["fp64"]["gstencil_per_s"], 3)
-    res["workload"] = f"box2d3r {n}x{n} halo 3 T={T}"
+    res["workload"] = f"{shape} {'x'.join(map(str, sizes))} halo {r} T={T}"
     return res
 
 
@@ -274,6 +276,8 @@ def main():
     if not args.no_extras and args.workload == "box2d1r" and args.precision == "fp64":
         extras["unfused"] = unfused_hbm(L, r, dense, T, args.seed, not args.no_graph)
         extras["split_box2d3r"] = split_comparison(args.seed, not args.no_graph)
+    if not args.no_extras and ndim == 3 and args.precision == "fp64":
+        extras[f"split_{shape}"] = split_comparison(args.seed, not args.no_graph, shape, sizes, T)
 
     cpu = None
     if not args.no_cpu:
@@ -290,9 +294,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else f"f64 state, {args.precision} products, f32 accumulation",
         "data": "synthetic (SplitMix64 seeded grid and weights)",
-        "config": {"workload": f"{shape} {'x'.join(map(str, sizes))} halo {halo} T={T} fp64", "time_steps": T,
+        "config": {"workload": f"{shape} {'x'.join(map(str, sizes))} halo {halo} T={T} {args.precision}",
+                   "time_steps": T,
                    "fusion_depth": kf, "plan": desc, "wall_s": round(wall, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -14,7 +14,8 @@
 //   --graph=0|1      capture the time loop in a hipGraph (default 1)
 //   --autotune=0|1|2 physical mode: 1 = the autotuner's cached choice (timed on a miss),
 //                    2 = time the variants again (default 0: library defaults)
-//   --precision=P    physical mode, 2-D: fp64 (default), bf16x3, fp32x2, fp16x2 or fp16w3
+//   --precision=P    physical mode: fp64 (default), bf16x3, fp32x2, fp16x2 or fp16w3 (2-D);
+//                    fp64, bf16x3 or fp16w3 (3-D)
 //   --json           also print a JSON record (physical mode)
 //   --exact-fusion   2-D reference mode: compose the 3x3 weights exactly instead of
 //                    using the reference's 49-entry table

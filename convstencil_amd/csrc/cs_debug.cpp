@@ -11,6 +11,7 @@ extern "C" {
 static const cs::Variant *family(int32_t ndim, int *n) {
   *n = 0;
   if (ndim == 4) return cs::variants_2d_split(n);
+  if (ndim == 5) return cs::variants_3d_split(n);
   if (ndim == 1) return cs::variants_1d_f64(n);
   if (ndim == 2) return cs::variants_2d_f64(n);
   if (ndim == 3) return cs::variants_3d_f64(n);
@@ -135,6 +136,26 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
     live[i] = t.live[i];
   }
   return t.n;
+}
+
+int32_t cs_debug_ctable3(int32_t radius, int32_t shape, int32_t dz, int32_t *c0, int32_t *ty, int32_t *tx,
+                         int32_t *live, int32_t cap) {
+  if (radius < 1 || radius > 3 || dz < -radius || dz > radius || !c0) return -1;
+  const cs::CTable3 t3 = cs::make_ctable3(radius, shape);
+  const cs::CTable &t = t3.t[dz + radius];
+  if (t.n > cap) return -1;
+  *c0 = t3.c0;
+  for (int i = 0; i < t.n; ++i) {
+    ty[i] = t.ty[i];
+    tx[i] = t.tx[i];
+    live[i] = t.live[i];
+  }
+  return t.n;
+}
+
+int64_t cs_debug_bfrag_split3(int32_t precision, int32_t radius, int32_t shape, const double *w, void *out,
+                              int64_t cap) {
+  return (int64_t)cs::build_bfrag_split3(precision, radius, shape, w, out, (size_t)cap);
 }
 
 int32_t cs_debug_dtable3(int32_t radius, int32_t shape, int32_t dz, int32_t *hdr, int32_t *ty, int32_t *tx,

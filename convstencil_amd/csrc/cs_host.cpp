@@ -72,7 +72,8 @@ int default_kf(int ndim, int R, int top) {
 
 const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   *cnt = 0;
-  if (prec != CS_PREC_FP64) return ndim == 2 ? cs::variants_2d_split(cnt) : nullptr;
+  if (prec != CS_PREC_FP64)
+    return ndim == 2 ? cs::variants_2d_split(cnt) : ndim == 3 ? cs::variants_3d_split(cnt) : nullptr;
   if (ndim == 1) return cs::variants_1d_f64(cnt);
   if (ndim == 2) return cs::variants_2d_f64(cnt);
   if (ndim == 3) return cs::variants_3d_f64(cnt);
@@ -108,7 +109,7 @@ cs_status select_variants(cs_plan *p) {
   const int prec = p->opt.precision;
   if (nd < 1 || nd > 3) return CS_ERR_UNSUPPORTED;
   if (prec != CS_PREC_FP64 && (prec < CS_PREC_BF16X3 || prec > CS_PREC_FP16W3)) return CS_ERR_INVALID;
-  if (prec != CS_PREC_FP64 && nd != 2) return CS_ERR_UNSUPPORTED;
+  if (prec != CS_PREC_FP64 && nd == 1) return CS_ERR_UNSUPPORTED;
   // Split precision (Variant::mf holds the precision): the dilated form (st 2) for the
   // stencil's shape class, else its box variants (zero weights off the stencil), else the
   // Toeplitz form (st 0, box only).  The remainder launches come from the same form.
@@ -149,8 +150,9 @@ cs_status select_variants(cs_plan *p) {
     if (u) CS_HIP(u->prepare());
   if (prec != CS_PREC_FP64) {
     std::vector<unsigned char> hb(1 << 20);
-    const size_t bytes = v->st == 2 ? cs::build_bfrag_split2(prec, p->R, p->shape, p->w.data(), hb.data(), hb.size())
-                                    : cs::build_bfrag_split(prec, p->R, p->w.data(), hb.data(), hb.size());
+    const size_t bytes = nd == 3      ? cs::build_bfrag_split3(prec, p->R, p->shape, p->w.data(), hb.data(), hb.size())
+                         : v->st == 2 ? cs::build_bfrag_split2(prec, p->R, p->shape, p->w.data(), hb.data(), hb.size())
+                                      : cs::build_bfrag_split(prec, p->R, p->w.data(), hb.data(), hb.size());
     if (bytes == 0) return CS_ERR_UNSUPPORTED;
     if (p->bfrag) (void)hipFree(p->bfrag);
     p->bfrag = nullptr;

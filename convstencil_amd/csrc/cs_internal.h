@@ -80,6 +80,7 @@ const Variant *variants_2d_f64(int *count);
 const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
 const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only
 const Variant *variants_2d_split(int *count);
+const Variant *variants_3d_split(int *count);  // split-precision 3-D (k3d_split.hip)
 
 // Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].
 void build_bfrag_2d_f64(int mf, int R, int shape, const double *w_dense, double *bfrag);
@@ -94,6 +95,8 @@ size_t build_bfrag_split(int prec, int R, const double *w_dense, void *out, size
 // A fragments of the dilated split form (Variant::st == 2; prec 1 BF16x3 or 3 FP16x2).
 size_t build_bfrag_split2(int prec, int R, int shape, const double *w_dense, void *out, size_t cap);
 int bfrag_ksteps_3d_f64(int R, int shape);
+// A fragments of the split-precision 3-D kernels (prec 1 BF16x3 or 4 FP16 w3).
+size_t build_bfrag_split3(int prec, int R, int shape, const double *w_dense, void *out, size_t cap);
 
 // Device-side synthetic input fill (same bits as the host generator).
 hipError_t launch_fill_random(double *a, const Geo &g, uint64_t stream_seed, hipStream_t s);

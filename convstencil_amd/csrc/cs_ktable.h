@@ -419,52 +419,53 @@ struct CTable {
   int cmin, cmax;    // first and last chunk start column
 };
 
-constexpr CTable make_ctable(int R, int shape) {
-  CTable best{};
-  best.n = 1 << 20;
-  for (int c0 = 0; c0 <= 4; c0 += 4) {
-    CTable t{};
-    t.c0 = c0;
-    int n = 0;
-    for (int ty = -R; ty <= R + 1; ++ty) {
-      int lo = 1 << 20, hi = -(1 << 20);
-      for (int vy = 0; vy <= 1; ++vy) {
-        const int dy = ty - vy;
-        if (dy < -R || dy > R) continue;
-        int a = 0, b = -1;
-        row_span(R, shape, dy, a, b);
-        if (a > b) continue;
-        lo = a < lo ? a : lo;
-        hi = b + 7 > hi ? b + 7 : hi;
-      }
-      if (lo > hi) continue;
-      for (int a = lo - (((lo - c0) % 8) + 8) % 8; a <= hi; a += 8) {
-        t.ty[n] = ty;
-        t.tx[n] = a;
-        t.live[n] = 1;
-        ++n;
-      }
+// Chunk table of the footprint `sp` with chunk starts c0 (mod 8).
+constexpr CTable make_ctable_spans(const Spans &sp, int c0) {
+  const int R = sp.R;
+  CTable t{};
+  t.c0 = c0;
+  int n = 0;
+  for (int ty = -R; ty <= R + 1; ++ty) {
+    int lo = 1 << 20, hi = -(1 << 20);
+    for (int vy = 0; vy <= 1; ++vy) {
+      const int dy = ty - vy;
+      if (dy < -R || dy > R) continue;
+      const int a = sp.lo[dy + R], b = sp.hi[dy + R];
+      if (a > b) continue;
+      lo = a < lo ? a : lo;
+      hi = b + 7 > hi ? b + 7 : hi;
     }
-    const int n4 = (n + 3) / 4 * 4;
-    for (int e = n; e < n4; ++e) {  // padding chunks repeat the first chunk with zero weight
-      t.ty[e] = t.ty[0];
-      t.tx[e] = t.tx[0];
-      t.live[e] = 0;
+    if (lo > hi) continue;
+    for (int a = lo - (((lo - c0) % 8) + 8) % 8; a <= hi; a += 8) {
+      t.ty[n] = ty;
+      t.tx[n] = a;
+      t.live[n] = 1;
+      ++n;
     }
-    t.n = n4;
-    t.tmin = 1 << 20;
-    t.tmax = -(1 << 20);
-    t.cmin = 1 << 20;
-    t.cmax = -(1 << 20);
-    for (int e = 0; e < n4; ++e) {
-      t.tmin = t.ty[e] < t.tmin ? t.ty[e] : t.tmin;
-      t.tmax = t.ty[e] > t.tmax ? t.ty[e] : t.tmax;
-      t.cmin = t.tx[e] < t.cmin ? t.tx[e] : t.cmin;
-      t.cmax = t.tx[e] > t.cmax ? t.tx[e] : t.cmax;
-    }
-    if (t.n < best.n) best = t;
   }
-  return best;
+  const int n4 = (n + 3) / 4 * 4;
+  for (int e = n; e < n4; ++e) {  // padding chunks repeat the first chunk with zero weight
+    t.ty[e] = t.ty[0];
+    t.tx[e] = t.tx[0];
+    t.live[e] = 0;
+  }
+  t.n = n4;
+  t.tmin = 1 << 20;
+  t.tmax = -(1 << 20);
+  t.cmin = 1 << 20;
+  t.cmax = -(1 << 20);
+  for (int e = 0; e < n4; ++e) {
+    t.tmin = t.ty[e] < t.tmin ? t.ty[e] : t.tmin;
+    t.tmax = t.ty[e] > t.tmax ? t.ty[e] : t.tmax;
+    t.cmin = t.tx[e] < t.cmin ? t.tx[e] : t.cmin;
+    t.cmax = t.tx[e] > t.cmax ? t.tx[e] : t.cmax;
+  }
+  return t;
+}
+
+constexpr CTable make_ctable(int R, int shape) {
+  const CTable a = make_ctable_spans(spans_2d(R, shape), 0), b = make_ctable_spans(spans_2d(R, shape), 4);
+  return b.n < a.n ? b : a;
 }
 
 // Address plan of a split table's B reads at row pitch px (elements of esz bytes): read i
@@ -660,6 +661,52 @@ constexpr int best_pitch_ctable(const CTable &t, int px0, int esz, long bpx, lon
   for (int px = p0; px < p0 + 64; px += 8) {
     if (lds_wgs(bpx * px + fixed) < wg0) break;
     const int c = ctable_lds_cost(t, px, esz);
+    if (c < best_cost) {
+      best_cost = c;
+      best = px;
+    }
+  }
+  return best;
+}
+
+// ---- chunk tables of the split-precision 3-D kernels: one CTable per plane dz ----
+// Every plane's chunks start at the same residue c0 (a block's column alignment then
+// holds for each dz); the c0 with the fewest chunks over all planes is chosen.  Each
+// plane's chunks are padded to whole MFMAs on their own: an MFMA reads one plane.
+struct CTable3 {
+  int R, c0;
+  CTable t[7];  // plane dz at t[dz + R]
+};
+
+constexpr CTable3 make_ctable3(int R, int shape) {
+  CTable3 best{};
+  int best_n = 1 << 30;
+  for (int c0 = 0; c0 <= 4; c0 += 4) {
+    CTable3 c{};
+    c.R = R;
+    c.c0 = c0;
+    int n = 0;
+    for (int dz = -R; dz <= R; ++dz) {
+      c.t[dz + R] = make_ctable_spans(spans_3d(R, shape, dz), c0);
+      n += c.t[dz + R].n;
+    }
+    if (n < best_n) {
+      best = c;
+      best_n = n;
+    }
+  }
+  return best;
+}
+
+// As best_pitch_ctable, with the reads of every plane counted.
+constexpr int best_pitch_ctable3(const CTable3 &t, int px0, int esz, long bpx, long fixed) {
+  const int p0 = (px0 + 7) / 8 * 8;
+  const int wg0 = lds_wgs(bpx * p0 + fixed);
+  int best = p0, best_cost = 1 << 30;
+  for (int px = p0; px < p0 + 64; px += 8) {
+    if (lds_wgs(bpx * px + fixed) < wg0) break;
+    int c = 0;
+    for (int dz = -t.R; dz <= t.R; ++dz) c += ctable_lds_cost(t.t[dz + t.R], px, esz);
     if (c < best_cost) {
       best_cost = c;
       best = px;

@@ -304,26 +304,6 @@ static size_t build_split(int R, const double *w, unsigned char *out, size_t cap
 
 // A fragments of the dilated form: weight slice q, instruction i, lane l, element j is
 // W[t - v_m] with m = l & 15, chunk 4 i + (l >> 4) and t = (ty, tx0 + j).
-// The weights' slices of the dilated form: as the state (split_value) with NSW slices,
-// except for TrF16W3, whose third slice is the remainder after two slices stored at
-// the second slice's scale (its products join the level-1 accumulator).
-template <class Tr>
-static void split_weight(double v, typename Tr::T *sl) {
-  using T = typename Tr::T;
-  if constexpr (Tr::NSU == 2 && Tr::NSW == 3) {
-    const T h = (T)(float)v;
-    const double r = (v - (double)(float)h) * Tr::SCALE;
-    const T m = (T)(float)r;
-    sl[0] = h;
-    sl[1] = m;
-    sl[2] = (T)(float)(r - (double)(float)m);
-  } else {
-    T s[Tr::NSU];
-    split_value<Tr>(v, s);
-    for (int k = 0; k < Tr::NSW; ++k) sl[k] = s[k];
-  }
-}
-
 template <class Tr>
 static size_t build_split2(int R, int shape, const double *w, unsigned char *out, size_t cap) {
   using T = typename Tr::T;

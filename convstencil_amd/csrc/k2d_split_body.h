@@ -85,6 +85,26 @@ __host__ __device__ inline void split_value(double v, typename Tr::T *s) {
   }
 }
 
+// The weights' slices of the dilated forms (host side): as the state (split_value) with
+// NSW slices, except for TrF16W3, whose third slice is the remainder after two slices
+// stored at the second slice's scale (its products join the level-1 accumulator).
+template <class Tr>
+inline void split_weight(double v, typename Tr::T *sl) {
+  using T = typename Tr::T;
+  if constexpr (Tr::NSU == 2 && Tr::NSW == 3) {
+    const T h = (T)(float)v;
+    const double r = (v - (double)(float)h) * Tr::SCALE;
+    const T m = (T)(float)r;
+    sl[0] = h;
+    sl[1] = m;
+    sl[2] = (T)(float)(r - (double)(float)m);
+  } else {
+    T s[Tr::NSU];
+    split_value<Tr>(v, s);
+    for (int k = 0; k < Tr::NSW; ++k) sl[k] = s[k];
+  }
+}
+
 template <class Tr, int R, int KF, int BY, int BX, int NW>
 struct CfgS {
   using T = typename Tr::T;

@@ -16,7 +16,8 @@ extern "C" {
 #endif
 
 /* Number of compiled kernel variants of a family: 1, 2, 3 = FP64 kernels for 1-, 2-, 3-D grids;
- * 4 = 2-D split-precision kernels (there info[1] holds the cs_precision). */
+ * 4 = 2-D and 5 = 3-D split-precision kernels (there info[1] holds the cs_precision; the 3-D
+ * split kernels' z-chunk length is in their name, "zc<n>"). */
 int32_t cs_debug_variant_count(int32_t ndim);
 /* info[0..11] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
  * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
@@ -72,6 +73,17 @@ int32_t cs_debug_dtable3(int32_t radius, int32_t shape, int32_t dz, int32_t *hdr
                          int32_t *live, int32_t cap);
 /* B fragments of the 3-D 8 x 8 block kernels, planes dz = -r .. r in order; returns the k-step count. */
 int32_t cs_debug_bfrag_3d2(int32_t radius, int32_t shape, const double *w, double *bfrag, int32_t cap);
+
+/* Chunk table of plane dz of the split-precision 3-D kernels (family 5 of cs_debug_variant_info),
+ * in the layout of cs_debug_ctable; every plane shares *c0.  Returns the plane's chunk count (4 per
+ * MFMA) or -1. */
+int32_t cs_debug_ctable3(int32_t radius, int32_t shape, int32_t dz, int32_t *c0, int32_t *ty, int32_t *tx,
+                         int32_t *live, int32_t cap);
+/* A fragments of the split-precision 3-D kernels, layout [weight slice][instruction][lane][8
+ * elements] with the instructions of planes dz = -r .. r in order; precision 1 (BF16x3) or 4 (FP16
+ * with three weight slices).  Returns the byte count, or 0. */
+int64_t cs_debug_bfrag_split3(int32_t precision, int32_t radius, int32_t shape, const double *w, void *out,
+                              int64_t cap);
 
 /* Times `reps` plain device copies of `count` doubles (needs a GPU); the kernel is the
  * reference for HBM byte counters.  Returns 0 on success. */

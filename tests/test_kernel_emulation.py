@@ -1140,3 +1140,214 @@ def test_ctables_cover_every_tap():
             for (tdy, tdx) in taps:
                 for m in range(16):
                     assert (tdy + (m >> 3), tdx + (m & 7)) in slots, (R, shape, tdy, tdx, m)
+
+
+# ------------------------------------------------------------- 3-D split precision
+
+def ctable3s(R, shape, dz):
+    lib = _dbg()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib.cs_debug_ctable3.restype = ctypes.c_int32
+    lib.cs_debug_ctable3.argtypes = [ctypes.c_int32] * 3 + [ip, ip, ip, ip, ctypes.c_int32]
+    cap = 128
+    c0 = ctypes.c_int32()
+    ty, tx, live = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+    n = lib.cs_debug_ctable3(R, shape, dz, ctypes.byref(c0), ty, tx, live, cap)
+    assert n > 0 and n % 4 == 0
+    return c0.value, np.array(ty[:n]), np.array(tx[:n]), np.array(live[:n])
+
+
+def emulate_split3(v, layout, dense, x):
+    """The split-precision 3-D kernel (k3d_split.hip): KF levels streamed through z, a ring
+    of 2R+1 planes of slices per level, 16 x 16 blocks with one chunk table per plane."""
+    import re
+    prec, R, SH, KF, BY, BX = v["mf"], v["R"], v["shape"], v["kf"], v["by"], v["bx"]
+    ZC = int(re.search(r"zc(\d+)", v["name"]).group(1))
+    kind, CK, KI, NSU, NSW, NL, SCALE = SPLIT[prec]
+    tabs = {dz: ctable3s(R, SH, dz) for dz in range(-R, R + 1)}
+    c0 = tabs[0][0]
+    assert all(t[0] == c0 for t in tabs.values())
+    nins = {dz: len(tabs[dz][1]) // 4 for dz in tabs}
+    ib = {dz: sum(nins[d] for d in range(-R, dz)) for dz in tabs}
+    NINS = sum(nins.values())
+    lib = _dbg()
+    lib.cs_debug_bfrag_split3.restype = ctypes.c_int64
+    lib.cs_debug_bfrag_split3.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _lib.c_double_p,
+                                          ctypes.c_void_p, ctypes.c_int64]
+    raw = np.zeros(1 << 20, dtype=np.uint8)
+    wd = np.ascontiguousarray(dense, dtype=np.float64)
+    nbytes = lib.cs_debug_bfrag_split3(prec, R, SH, wd.ctypes.data_as(_lib.c_double_p), raw.ctypes.data, raw.size)
+    assert nbytes == NSW * NINS * 64 * 8 * 2
+    wf = _to_f32(raw[:nbytes].copy().view(np.uint16), kind).reshape(NSW, NINS, 64, 8)
+    EY, EX, PX, CS = BY + 2 * KF * R, BX + 2 * KF * R, v["px"], v["cs"]
+    assert PX % 8 == 0
+    PLANE, NP = EY * PX, 2 * R + 1
+    assert v["lds"] == (KF * NP * NSU * PLANE + 8) * 2 and v["lds"] <= 160 * 1024
+    lanes = np.arange(64)
+    grp, nn = lanes >> 4, lanes & 15
+    A = np.zeros((NSW, NINS, 16, 32), dtype=np.float64)
+    for q in range(NSW):
+        for i in range(NINS):
+            A[q, i][nn[:, None], 8 * grp[:, None] + np.arange(8)[None, :]] = wf[q, i]
+    off = np.zeros((NINS, 64), dtype=np.int64)
+    for dz in tabs:
+        _, cty, ctx, _ = tabs[dz]
+        for li in range(nins[dz]):
+            off[ib[dz] + li] = (2 * (nn & 7) + cty[4 * li + grp]) * PX + 8 * (nn >> 3) + ctx[4 * li + grp] + CS
+    orow = 2 * (nn & 7) + (grp >> 1)
+    ocol = 8 * (nn >> 3) + 4 * (grp & 1)
+    INV = np.float32(1.0 / SCALE)
+
+    def geo(l):
+        LO, H = R * l, BY + 2 * R * (KF - l)
+        HI = LO + BX + 2 * R * (KF - l)
+        LOX = LO - ((LO + CS + c0) % 8)
+        return LO, H, HI, LOX, -(-H // 16), -(-(HI - LOX) // 16)
+    ext, org, n = layout.ext, layout.org, layout.n
+    x3 = x.reshape(ext)
+    out = np.zeros(layout.cells)
+    o3 = out.reshape(ext)
+    tiles = (-(-n[0] // ZC), -(-n[1] // BY), -(-n[2] // BX))
+    rr, cc = np.divmod(np.arange(PLANE), PX)
+    cc = cc - CS
+    for zc in range(tiles[0]):
+        for ty_ in range(tiles[1]):
+            for tx_ in range(tiles[2]):
+                Y0, X0 = ty_ * BY - KF * R, tx_ * BX - KF * R
+                ay0, ax0 = org[1] + Y0, org[2] + X0
+                Z0, Z1 = zc * ZC, min(zc * ZC + ZC, n[0])
+                base = Z0 - KF * R
+                full = Y0 >= 0 and X0 >= 0 and Y0 + EY <= n[1] and X0 + EX <= n[2]
+                rings = np.full((KF, NP, NSU, PLANE), np.nan, dtype=np.float32)
+                rings[1:] = 0.0
+                slot = lambda p: (p - base) % NP
+
+                def load(p):
+                    pl = np.zeros(PLANE)
+                    az = org[0] + p
+                    if 0 <= az < ext[0]:
+                        ay, ax = ay0 + rr, ax0 + cc
+                        ok = (cc >= 0) & (cc < EX) & (ay >= 0) & (ay < ext[1]) & (ax >= 0) & (ax < ext[2])
+                        pl[ok] = x3[az, ay[ok], ax[ok]]
+                    return np.stack(split_f32_np(pl.astype(np.float32), kind, NSU, SCALE))
+
+                def inside(ty, tx):
+                    return full or (0 <= Y0 + ty < n[1] and 0 <= X0 + tx < n[2])
+                for p in range(base, Z1 - 1 + KF * R + 1):
+                    rings[0, slot(p)] = load(p)
+                    for l in range(1, KF + 1):
+                        qz = p - l * R
+                        if not (Z0 - (KF - l) * R <= qz <= Z1 - 1 + (KF - l) * R):
+                            continue
+                        if l < KF and (qz < 0 or qz >= n[0]):
+                            rings[l, slot(qz)] = 0.0
+                            continue
+                        LO, H, HI, LOX, NBY, NBX = geo(l)
+                        writes = []
+                        for blk in range(NBY * NBX):
+                            ba, bb = divmod(blk, NBX)
+                            y, xx = LO + min(16 * ba, H - 16), LOX + 16 * bb
+                            d0 = np.zeros((16, 16), dtype=np.float32)
+                            d1 = np.zeros((16, 16), dtype=np.float32)
+                            for dz in range(-R, R + 1):
+                                src = rings[l - 1, slot(qz + dz)]
+                                for li in range(nins[dz]):
+                                    i = ib[dz] + li
+                                    b0 = y * PX + xx + off[i]
+                                    assert np.all(b0 % 8 == 0), (v["name"], "16-byte LDS read misaligned")
+                                    assert b0.min() >= 0 and b0.max() + 8 <= PLANE
+                                    assert np.all(b0 % PX + 8 <= PX), (v["name"], "read wraps into the next row")
+                                    B = []
+                                    for s in range(NSU):
+                                        g = src[s][b0[:, None] + np.arange(8)[None, :]]
+                                        assert np.isfinite(g).all(), (v["name"], "reads an unset LDS cell")
+                                        Bs = np.zeros((32, 16), dtype=np.float64)
+                                        Bs[8 * grp[:, None] + np.arange(8)[None, :], nn[:, None]] = g
+                                        B.append(Bs)
+                                    d0 = (d0 + (A[0, i] @ B[0]).astype(np.float32)).astype(np.float32)
+                                    for s in range(NSU):
+                                        for q in range(NSW):
+                                            if keep_product(prec, s, q):
+                                                d1 = (d1 + (A[q, i] @ B[s]).astype(np.float32)).astype(np.float32)
+                            for ln in range(64):
+                                tyy = y + orow[ln]
+                                for vv in range(4):
+                                    m = 4 * grp[ln] + vv
+                                    txx = xx + ocol[ln] + vv
+                                    a0, a1 = d0[m, nn[ln]], d1[m, nn[ln]]
+                                    if l == KF:
+                                        if LO <= txx < HI and inside(tyy, txx):
+                                            gi = (org[0] + qz, ay0 + tyy, ax0 + txx)
+                                            assert all(org[d] <= gi[d] < org[d] + n[d] for d in range(3))
+                                            o3[gi] = float(a0) + float(a1) * float(INV)
+                                    elif inside(tyy, txx):  # full tiles: every cell, overshoot included
+                                        li_ = tyy * PX + txx + CS
+                                        assert 0 <= li_ < PLANE and 0 <= txx + CS < PX
+                                        writes.append((li_, np.float32(a0 + a1 * INV)))
+                        if l < KF:
+                            for li_, val in writes:
+                                for k, s_ in enumerate(split_f32_np(np.array([val], dtype=np.float32), kind, NSU,
+                                                                    SCALE)):
+                                    rings[l, slot(qz), k, li_] = s_[0]
+    return out
+
+
+VARIANTS_SPLIT3 = variants(5)
+
+
+def _dense3(R, sh, seed):
+    D = 2 * R + 1
+    rng = np.random.default_rng(seed)
+    w = rng.uniform(0.05, 1.0, (D, D, D))
+    zz, yy, xx = np.meshgrid(*[np.arange(-R, R + 1)] * 3, indexing="ij")
+    if sh == 1:
+        w[((zz != 0).astype(int) + (yy != 0) + (xx != 0)) > 1] = 0.0
+    w /= w.sum()
+    return w.reshape(-1)
+
+
+def _check_split3(v, layout):
+    R = v["R"]
+    dense = _dense3(R, v["shape"], v["id"])
+    x = oracle.fill_data(layout, v["id"])
+    x = np.where(layout.interior_mask(), x, 0.0)  # fused kernels run with a zero halo
+    got = emulate_split3(v, layout, dense, x)
+    offs, ww = oracle.dense_to_taps(3, R, dense)
+    want = oracle.run(layout, offs, ww, x, v["kf"])
+    gi, wi = layout.interior(got), layout.interior(want)
+    err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+    assert err < 2e-6, (v["name"], err)
+    assert np.all(got[~layout.interior_mask()] == 0.0)
+
+
+@pytest.mark.parametrize("v", VARIANTS_SPLIT3, ids=[v["name"] for v in VARIANTS_SPLIT3])
+def test_variant_emulation_split3(v):
+    # 3 x 3 tiles per plane: the centre tile lies inside the interior (unmasked path), the
+    # others are cut by the domain edge (partial tiles); 7 planes, one z-chunk
+    _check_split3(v, Layout.with_halo([7, 3 * v["by"] - 3, 3 * v["bx"] - 5], v["R"]))
+
+
+def test_split3_zchunks():
+    # two z-chunks (the second partial) for an unfused and a fused variant
+    import re
+    for kf in (1, 2):
+        v = [u for u in VARIANTS_SPLIT3 if u["kf"] == kf][0]
+        zc = int(re.search(r"zc(\d+)", v["name"]).group(1))
+        _check_split3(v, Layout.with_halo([zc + 3, v["by"] + 5, v["bx"] // 2 + 7], v["R"]))
+
+
+def test_ctable3s_cover_every_tap():
+    # per plane: every (tap, output offset v) pair lands in exactly one live chunk slot
+    for shape in (0, 1):
+        R = 1
+        for dz in range(-R, R + 1):
+            c0, ty, tx, live = ctable3s(R, shape, dz)
+            assert c0 in (0, 4) and np.all((tx - c0) % 8 == 0)
+            slots = [(ty[e], tx[e] + j) for e in range(len(ty)) if live[e] for j in range(8)]
+            assert len(slots) == len(set(slots))
+            taps = {(a, b) for a in range(-R, R + 1) for b in range(-R, R + 1)
+                    if shape == 0 or (dz == 0 and (a == 0 or b == 0)) or (a == 0 and b == 0)}
+            for (tdy, tdx) in taps:
+                for m in range(16):
+                    vy, vx = m >> 3, m & 7
+                    assert (tdy + vy, tdx + vx) in slots, (shape, dz, tdy, tdx, m)
