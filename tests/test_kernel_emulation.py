@@ -1351,3 +1351,52 @@ def test_ctable3s_cover_every_tap():
                 for m in range(16):
                     vy, vx = m >> 3, m & 7
                     assert (tdy + vy, tdx + vx) in slots, (shape, dz, tdy, tdx, m)
+
+
+def test_fp16x2_weights_compensated():
+    # the two-slice FP16 weights of the dilated form: every tap within 2 FP16 ulps of its
+    # plain split, and the errors d = w - (h + l / 2048) cancelling in the sum and the
+    # first moments (a plain split leaves a sum of order 1e-9 that compounds over steps)
+    lib = _dbg()
+    lib.cs_debug_bfrag_split2.restype = ctypes.c_int64
+    lib.cs_debug_bfrag_split2.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _lib.c_double_p,
+                                          ctypes.c_void_p, ctypes.c_int64]
+    for R, shape in ((3, 0), (3, 1), (1, 0)):
+        D = 2 * R + 1
+        dense = _dense(R, shape, 77 + R + shape).reshape(D, D)
+        c0, cty, ctx, clive = ctable(R, shape)
+        nins = len(cty) // 4
+        raw = np.zeros(1 << 20, dtype=np.uint8)
+        nbytes = lib.cs_debug_bfrag_split2(3, R, shape, np.ascontiguousarray(dense.reshape(-1)).ctypes.data_as(
+            _lib.c_double_p), raw.ctypes.data, raw.size)
+        assert nbytes == 2 * nins * 64 * 8 * 2
+        wf = raw[:nbytes].copy().view(np.float16).astype(np.float64).reshape(2, nins, 64, 8)
+        h, l = np.full((D, D), np.nan), np.full((D, D), np.nan)
+        for i in range(nins):
+            for lane in range(64):
+                m, c = lane & 15, 4 * i + (lane >> 4)
+                for j in range(8):
+                    dy, dx = cty[c] - (m >> 3), ctx[c] + j - (m & 7)
+                    if clive[c] and -R <= dy <= R and -R <= dx <= R:
+                        for arr, q in ((h, 0), (l, 1)):
+                            v = wf[q, i, lane, j]
+                            assert np.isnan(arr[dy + R, dx + R]) or arr[dy + R, dx + R] == v
+                            arr[dy + R, dx + R] = v
+        assert not np.isnan(h).any()
+        hp = dense.astype(np.float32).astype(np.float16).astype(np.float64)
+        lp = ((dense - hp) * 2048).astype(np.float32).astype(np.float16).astype(np.float64)
+        assert np.array_equal(h, hp)
+        ulp = np.spacing(np.abs(lp).astype(np.float16)).astype(np.float64)
+        assert np.all(np.abs(l - lp) <= 2 * ulp + 1e-30)
+        assert np.all((dense != 0) | ((h == 0) & (l == 0)))
+        yy, xx = np.meshgrid(np.arange(-R, R + 1), np.arange(-R, R + 1), indexing="ij")
+        d, dp = dense - (h + l / 2048), dense - (hp + lp / 2048)
+        got = [abs(d.sum()), abs((d * yy).sum()), abs((d * xx).sum())]
+        plain = [abs(dp.sum()), abs((dp * yy).sum()), abs((dp * xx).sum())]
+        # the sum (the mass added or removed per step) cancels; the moments shrink (on the
+        # 13-tap star only six taps can move the y moment, by two ulps each)
+        # (the greedy search stops at the granularity of the slices' ulps: about 1e-10 for
+        # the 49-tap box2d3r, which models to about 5e-8 relative L2 after 64 steps, coarser
+        # for 9 taps of about 1/9)
+        assert sum(plain) > 5e-9, (R, shape, plain)
+        assert got[0] <= max(0.1 * plain[0], 1e-10) and sum(got) <= 0.25 * sum(plain), (R, shape, got, plain)
