@@ -463,7 +463,9 @@ constexpr CTable make_ctable_spans(const Spans &sp, int c0) {
   return t;
 }
 
-constexpr CTable make_ctable(int R, int shape) {
+// Row-major chunk table of a 2-D stencil (the c0 with fewer chunks); make_ctable below
+// orders its chunks for the LDS reads.
+constexpr CTable make_ctable_rows(int R, int shape) {
   const CTable a = make_ctable_spans(spans_2d(R, shape), 0), b = make_ctable_spans(spans_2d(R, shape), 4);
   return b.n < a.n ? b : a;
 }
@@ -669,6 +671,74 @@ constexpr int best_pitch_ctable(const CTable &t, int px0, int esz, long bpx, lon
   return best;
 }
 
+// ---- chunk order of the split tables ----
+//
+// The lanes of an MFMA's four k groups read chunks 4 i .. 4 i + 3; a 16-byte read is served
+// in four passes of 16 lanes (lds_cost), and two lanes of one pass whose chunks lie in the
+// same row 8 columns apart read the same 16 bytes from the same banks.  Row-major order puts
+// such chunks in k groups 0 and 1 (the same passes); ordered as (ty, a), (ty + 1, a),
+// (ty, a + 8), (ty + 1, a + 8) they fall in different passes, and at pitches of 16 (mod 32)
+// elements every read of box2d3r, star2d3r and box2d1r is conflict-free in the model
+// (4 cycles instead of 8).  order_ctable finds such orders: instruction by instruction,
+// the first unused chunk takes k group 0 and every choice of three unused chunks for groups
+// 1..3 is tried at pitch px, the cheapest kept (stopping at a conflict-free choice).
+constexpr int kOrderPitch = 112;  // 2-byte slices: 16 (mod 32) elements
+
+constexpr int cinstr_cost(const int (&ty)[4], const int (&tx)[4], int px, int esz) {
+  int a[64] = {};
+  for (int l = 0; l < 64; ++l) {
+    const int g = l >> 4, n = l & 15;
+    a[l] = ((2 * (n & 7) + ty[g] + 16) * px + 8 * (n >> 3) + tx[g] + 16) * esz;
+  }
+  return lds_cost(a, 16, false);
+}
+
+constexpr CTable order_ctable(const CTable &t, int px, int esz) {
+  CTable o = t;
+  bool used[64] = {};
+  for (int i = 0; i < t.n / 4; ++i) {
+    int c0 = 0;
+    while (used[c0]) ++c0;
+    used[c0] = true;
+    int best = 1 << 30, b1 = -1, b2 = -1, b3 = -1;
+    for (int x1 = 0; x1 < t.n && best > 4; ++x1) {
+      if (used[x1]) continue;
+      for (int x2 = 0; x2 < t.n && best > 4; ++x2) {
+        if (used[x2] || x2 == x1) continue;
+        for (int x3 = 0; x3 < t.n && best > 4; ++x3) {
+          if (used[x3] || x3 == x1 || x3 == x2) continue;
+          const int ty[4] = {t.ty[c0], t.ty[x1], t.ty[x2], t.ty[x3]};
+          const int tx[4] = {t.tx[c0], t.tx[x1], t.tx[x2], t.tx[x3]};
+          const int c = cinstr_cost(ty, tx, px, esz);
+          if (c < best) {
+            best = c;
+            b1 = x1;
+            b2 = x2;
+            b3 = x3;
+          }
+        }
+      }
+    }
+    used[b1] = used[b2] = used[b3] = true;
+    const int sel[4] = {c0, b1, b2, b3};
+    for (int g = 0; g < 4; ++g) {
+      o.ty[4 * i + g] = t.ty[sel[g]];
+      o.tx[4 * i + g] = t.tx[sel[g]];
+      o.live[4 * i + g] = t.live[sel[g]];
+    }
+  }
+  return o;
+}
+
+// Chunk table of the split-precision 2-D kernels (and their host-side A fragments).
+constexpr CTable make_ctable(int R, int shape) { return order_ctable(make_ctable_rows(R, shape), kOrderPitch, 2); }
+
+// One evaluation per stencil class for the kernels' compile-time tables.
+template <int R, int SH>
+struct CTab2 {
+  static constexpr CTable T = make_ctable(R, SH);
+};
+
 // ---- chunk tables of the split-precision 3-D kernels: one CTable per plane dz ----
 // Every plane's chunks start at the same residue c0 (a block's column alignment then
 // holds for each dz); the c0 with the fewest chunks over all planes is chosen.  Each
@@ -695,8 +765,14 @@ constexpr CTable3 make_ctable3(int R, int shape) {
       best_n = n;
     }
   }
+  for (int dz = -R; dz <= R; ++dz) best.t[dz + R] = order_ctable(best.t[dz + R], kOrderPitch, 2);
   return best;
 }
+
+template <int R, int SH>
+struct CTab3 {
+  static constexpr CTable3 T = make_ctable3(R, SH);
+};
 
 // As best_pitch_ctable, with the reads of every plane counted.
 constexpr int best_pitch_ctable3(const CTable3 &t, int px0, int esz, long bpx, long fixed) {

@@ -209,6 +209,10 @@ static const Variant kVariantsSplit[] = {
     CS_VS2(63305, 3, TrF16, 3, SHAPE_BOX, 5, 64, 64, 8),
     CS_VS2(63306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 64, 8),
     CS_VS2(63311, 3, TrF16, 3, SHAPE_STAR, 1, 64, 64, 8),
+    CS_VS2(63307, 3, TrF16, 3, SHAPE_BOX, 7, 64, 64, 8),
+    CS_VS2(63308, 3, TrF16, 3, SHAPE_BOX, 8, 64, 64, 8),
+    CS_VS2(63313, 3, TrF16, 3, SHAPE_STAR, 3, 64, 64, 8),
+    CS_VS2(63314, 3, TrF16, 3, SHAPE_STAR, 4, 64, 64, 8),
     CS_VS2(61401, 4, TrF16W3, 1, SHAPE_BOX, 1, 64, 64, 8),
     CS_VS2(61402, 4, TrF16W3, 1, SHAPE_BOX, 2, 64, 64, 8),
     CS_VS2(61404, 4, TrF16W3, 1, SHAPE_BOX, 4, 64, 64, 8),
@@ -239,6 +243,22 @@ static const Variant kVariantsSplit[] = {
     CS_VS2O(963406, 4, TrF16W3, 3, SHAPE_BOX, 6, 128, 128, 16, 0),
     CS_VS2O(963408, 4, TrF16W3, 3, SHAPE_BOX, 8, 128, 128, 16, 0),
     CS_VS2O(963414, 4, TrF16W3, 3, SHAPE_STAR, 4, 128, 128, 16, 0),
+    CS_VS2O(863306, 3, TrF16, 3, SHAPE_BOX, 6, 128, 64, 8, 0),
+    CS_VS2O(963306, 3, TrF16, 3, SHAPE_BOX, 6, 128, 128, 16, 0),
+    CS_VS2O(963308, 3, TrF16, 3, SHAPE_BOX, 8, 128, 128, 16, 0),
+    // FP16x2 kf6 launch shapes (ids 1000000 + shape index * 100000 + the base id)
+    CS_VS2O(1063306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 64, 4, 0),
+    CS_VS2O(1163306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 64, 16, 0),
+    CS_VS2O(1263306, 3, TrF16, 3, SHAPE_BOX, 6, 48, 64, 8, 0),
+    CS_VS2O(1363306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 48, 8, 0),
+    CS_VS2O(1463306, 3, TrF16, 3, SHAPE_BOX, 6, 80, 64, 8, 0),
+    CS_VS2O(1563306, 3, TrF16, 3, SHAPE_BOX, 6, 64, 80, 8, 0),
+    CS_VS2O(1463305, 3, TrF16, 3, SHAPE_BOX, 5, 80, 64, 8, 0),
+    CS_VS2O(1663306, 3, TrF16, 3, SHAPE_BOX, 6, 80, 64, 10, 0),
+    CS_VS2O(1763306, 3, TrF16, 3, SHAPE_BOX, 6, 80, 64, 12, 0),
+    CS_VS2O(1463406, 4, TrF16W3, 3, SHAPE_BOX, 6, 80, 64, 8, 0),
+    CS_VS2O(1463314, 3, TrF16, 3, SHAPE_STAR, 4, 80, 64, 8, 0),
+    CS_VS2O(1463316, 3, TrF16, 3, SHAPE_STAR, 6, 80, 64, 8, 0),
     CS_VS2O(763112, 1, TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8, 1),
     CS_VS2O(763113, 1, TrBF16, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
     CS_VS2O(763304, 3, TrF16, 3, SHAPE_BOX, 4, 64, 64, 8, 1),

@@ -51,7 +51,7 @@ struct CfgS2 {
   static constexpr int R = R_, SH = SH_, KF = KF_, BY = BY_, BX = BX_, NW = NW_, OS = OS_;
   static constexpr int EY = BY + 2 * KF * R;
   static constexpr int EX = BX + 2 * KF * R;
-  static constexpr CTable CT = make_ctable(R, SH);
+  static constexpr CTable CT = CTab2<R, SH>::T;
   static constexpr int NINS = CT.n / 4;  // MFMAs per slice product per block
   static constexpr int NSU = Tr::NSU, NSW = Tr::NSW;
   // region of fused step s: rows [LO, LO + H), columns [LO, HI)

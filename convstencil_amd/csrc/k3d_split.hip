@@ -28,7 +28,7 @@ struct CfgS3 {
   using T = typename Tr::T;
   static constexpr int R = R_, SH = SH_, BY = BY_, BX = BX_, KF = KF_, ZC = ZC_, NW = NW_;
   static constexpr int NSU = Tr::NSU, NSW = Tr::NSW, NP = 2 * R + 1, ESZ = (int)sizeof(T);
-  static constexpr CTable3 CT = make_ctable3(R, SH);
+  static constexpr CTable3 CT = CTab3<R, SH>::T;
   // MFMAs of plane dz, and the index of its first one in the per-lane arrays
   static constexpr int nins(int dz) { return CT.t[dz + R].n / 4; }
   static constexpr int ibase(int dz) {
