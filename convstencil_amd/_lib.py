@@ -71,6 +71,9 @@ SIGNATURES = {
                                    ctypes.c_char_p, ctypes.c_int32]),
     "cs_fill_random_host": (None, [c_double_p, ctypes.POINTER(cs_layout), ctypes.c_uint64]),
     "cs_fill_weights_host": (None, [c_double_p, ctypes.c_int32, ctypes.c_uint64]),
+    "cs_grid_save_txt": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cs_layout), c_double_p]),
+    "cs_grid_save_f64": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cs_layout), c_double_p]),
+    "cs_grid_load_f64": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cs_layout), c_double_p, ctypes.c_int64]),
     "gpu_1d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int]),
     "gpu_star_1d2r_step2": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int]),
     "gpu_box_2d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),

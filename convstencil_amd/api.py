@@ -6,6 +6,7 @@ reference's host launchers (src/2d/2d_utils.h:77-79, src/1d/1d_utils.h:75,85,
 src/3d/3d_utils.h:75-77) with numpy arrays for the host pointers.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -216,6 +217,33 @@ def fill_weights_host(count, seed):
     w = np.empty(int(count), dtype=np.float64)
     _lib.load().cs_fill_weights_host(_dptr(w), int(count), ctypes.c_uint64(seed))
     return w
+
+
+def save_txt(path, layout: Layout, arr):
+    """The reference's text dump of a padded array (save_to_txt, src/2d/main.cu:36-55 and
+    the 1-D / 3-D variants): "%.0f" values, as its drivers write output_gpu.txt."""
+    a = _contig(arr, layout.cells)
+    L = layout.to_c()
+    check(_lib.load().cs_grid_save_txt(os.fsencode(path), ctypes.byref(L), _dptr(a)), "cs_grid_save_txt")
+
+
+def save_f64(path, layout: Layout, arr):
+    """Exact binary grid file: "CSGRID64" header with the layout, then the padded array."""
+    a = _contig(arr, layout.cells)
+    L = layout.to_c()
+    check(_lib.load().cs_grid_save_f64(os.fsencode(path), ctypes.byref(L), _dptr(a)), "cs_grid_save_f64")
+
+
+def load_f64(path):
+    """Reads a file written by save_f64 (or cs_grid_save_f64); returns (Layout, flat array)."""
+    lib = _lib.load()
+    L = _lib.cs_layout()
+    check(lib.cs_grid_load_f64(os.fsencode(path), ctypes.byref(L), None, 0), "cs_grid_load_f64")
+    layout = Layout(3, list(L.ext), list(L.org), list(L.n))
+    layout.ndim = int(L.ndim)
+    out = np.empty(layout.cells, dtype=np.float64)
+    check(lib.cs_grid_load_f64(os.fsencode(path), ctypes.byref(L), _dptr(out), out.size), "cs_grid_load_f64")
+    return layout, out
 
 
 def device_count():

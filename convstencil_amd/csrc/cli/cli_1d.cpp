@@ -76,13 +76,17 @@ int main(int argc, char *argv[]) {
   printf("INFO: shape = %s,  n = %d, times = %d\n", kShapeStr[shape], n, times);
   const size_t cols = (size_t)n + 2 * halo;
   double *input = (double *)malloc((cols + 1) * sizeof(double));
-  double *output = (double *)malloc((cols + 1) * sizeof(double));
+  double *output = (double *)calloc(cols + 1, sizeof(double));
   if (!input || !output) {
     fprintf(stderr, "convstencil: host allocation failed\n");
     return 1;
   }
   for (size_t i = 0; i < cols + 1; ++i) input[i] = (double)(rand() % 10000);
-  const int rc = shape == 0 ? gpu_1d1r(input, output, p1, times, n) : gpu_star_1d2r_step2(input, output, p2, times, n);
+  int rc = shape == 0 ? gpu_1d1r(input, output, p1, times, n) : gpu_star_1d2r_step2(input, output, p2, times, n);
+  if (rc == 0 && cli::wants_output(e)) {  // the reference dumps the first n + 2 halo cells (src/1d/main.cu:275)
+    const long long ext[1] = {(long long)cols}, org[1] = {halo}, nn[1] = {n};
+    if (!cli::write_grid(e, cli::ref_layout(1, ext, org, nn), output)) rc = 1;
+  }
   free(output);
   free(input);
   return rc ? 1 : 0;

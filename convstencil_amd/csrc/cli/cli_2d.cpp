@@ -109,6 +109,10 @@ int main(int argc, char *argv[]) {
   for (size_t i = 0; i < rows * cols; ++i) matrix[i] = (double)(rand() % 100);
   int rc = (shape == box_2d1r || shape == star_2d1r) ? gpu_box_2d1r(matrix, output, param, times, m, n)
                                                      : gpu_box_2d3r(matrix, output, param, times, m, n);
+  if (rc == 0 && cli::wants_output(e)) {
+    const long long ext[2] = {(long long)rows, (long long)cols}, org[2] = {3, 4}, nn[2] = {m, n};
+    if (!cli::write_grid(e, cli::ref_layout(2, ext, org, nn), output)) rc = 1;
+  }
   free(output);
   free(matrix);
   return rc ? 1 : 0;

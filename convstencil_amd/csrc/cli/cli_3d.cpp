@@ -85,8 +85,13 @@ int main(int argc, char *argv[]) {
     return 1;
   }
   for (size_t i = 0; i < cells; ++i) matrix[i] = (double)(rand() % 100);
-  const int rc = shape == 0 ? gpu_box_3d1r(matrix, output, box, times, h, m, n)
-                            : gpu_star_3d1r(matrix, output, star, times, h, m, n);
+  int rc = shape == 0 ? gpu_box_3d1r(matrix, output, box, times, h, m, n)
+                      : gpu_star_3d1r(matrix, output, star, times, h, m, n);
+  if (rc == 0 && cli::wants_output(e)) {
+    const long long ext[3] = {(long long)h + 6, (long long)m + 6, (long long)n + 6}, org[3] = {3, 3, 3},
+                    nn[3] = {h, m, n};
+    if (!cli::write_grid(e, cli::ref_layout(3, ext, org, nn), output)) rc = 1;
+  }
   free(output);
   free(matrix);
   return rc ? 1 : 0;

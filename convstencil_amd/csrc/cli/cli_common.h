@@ -19,6 +19,9 @@
 //   --json           also print a JSON record (physical mode)
 //   --exact-fusion   2-D reference mode: compose the 3x3 weights exactly instead of
 //                    using the reference's 49-entry table
+//   --write-output[=PATH]  write the final padded array in the reference's text format
+//                    (its WRITE_OUTPUT build, src/2d/main.cu:410-415; default output_gpu.txt)
+//   --save-f64=PATH  write the final padded array as an exact binary grid (cs_grid_save_f64)
 #pragma once
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +44,8 @@ struct Ext {
   int precision = CS_PREC_FP64;
   bool json = false;
   bool exact_fusion = false;
+  std::string write_output;  // text dump path ("" = none)
+  std::string save_f64;      // binary grid path ("" = none)
 };
 
 // Removes the additive flags from argv; returns false (after printing a message)
@@ -79,6 +84,12 @@ inline bool strip_flags(int &argc, char **argv, Ext &e) {
         e.json = true;
       } else if (a == "--exact-fusion") {
         e.exact_fusion = true;
+      } else if (a == "--write-output") {
+        e.write_output = "output_gpu.txt";
+      } else if (const char *v7 = val("--write-output=")) {
+        e.write_output = v7;
+      } else if (const char *v8 = val("--save-f64=")) {
+        e.save_f64 = v8;
       } else {
         argv[w++] = argv[i];
       }
@@ -90,6 +101,43 @@ inline bool strip_flags(int &argc, char **argv, Ext &e) {
   argc = w;
   argv[argc] = nullptr;
   return true;
+}
+
+// Writes the files requested by --write-output / --save-f64 for a final padded array;
+// false (after a message) if one cannot be written.
+inline bool write_grid(const Ext &e, const cs_layout &L, const double *host) {
+  if (!e.write_output.empty()) {
+    printf("Writing %s\n", e.write_output.c_str());
+    fflush(stdout);
+    if (cs_grid_save_txt(e.write_output.c_str(), &L, host) != CS_OK) {
+      fprintf(stderr, "convstencil: cannot write %s\n", e.write_output.c_str());
+      return false;
+    }
+  }
+  if (!e.save_f64.empty() && cs_grid_save_f64(e.save_f64.c_str(), &L, host) != CS_OK) {
+    fprintf(stderr, "convstencil: cannot write %s\n", e.save_f64.c_str());
+    return false;
+  }
+  return true;
+}
+
+inline bool wants_output(const Ext &e) { return !e.write_output.empty() || !e.save_f64.empty(); }
+
+// Layout of the reference's padded host arrays (slowest first): ext, interior origin, n.
+inline cs_layout ref_layout(int ndim, const long long *ext, const long long *org, const long long *n) {
+  cs_layout L{};
+  L.ndim = ndim;
+  for (int d = 0; d < 3; ++d) {
+    L.ext[d] = 1;
+    L.org[d] = 0;
+    L.n[d] = 1;
+  }
+  for (int k = 0; k < ndim; ++k) {
+    L.ext[3 - ndim + k] = ext[k];
+    L.org[3 - ndim + k] = org[k];
+    L.n[3 - ndim + k] = n[k];
+  }
+  return L;
 }
 
 // Reads n doubles from stdin after the reference's prompt; false if one is missing.
@@ -135,6 +183,14 @@ inline int run_physical(const char *tag, int ndim, int r, const std::vector<doub
   if (s == CS_OK) s = cs_plan_time(p, e.steps, 1, 1, &rep);
   char desc[512] = {0};
   cs_plan_describe(p, desc, sizeof desc);
+  std::vector<double> final_grid;
+  if (s == CS_OK && wants_output(e)) {  // the grid after exactly `steps` steps from the seed
+    final_grid.resize((size_t)(L.ext[0] * L.ext[1] * L.ext[2]));
+    s = cs_plan_fill_random(p, e.seed);
+    cs_report r2{};
+    if (s == CS_OK) s = cs_plan_run(p, e.steps, &r2);
+    if (s == CS_OK) s = cs_plan_download(p, final_grid.data());
+  }
   cs_plan_destroy(p);
   if (s != CS_OK) {
     fprintf(stderr, "convstencil: %s\n", cs_status_string(s));
@@ -148,6 +204,7 @@ inline int run_physical(const char *tag, int ndim, int r, const std::vector<doub
            "\"launches\": %lld, \"fusion_depth\": %d, \"plan\": \"%s\"}\n",
            e.steps, e.seed, rep.elapsed_ms, rep.gstencil_per_s, (long long)rep.launches, rep.fusion_depth, desc);
   fflush(stdout);
+  if (wants_output(e) && !write_grid(e, L, final_grid.data())) return 1;
   return 0;
 }
 

@@ -228,7 +228,6 @@ __device__ __forceinline__ void load_tile_split(const double *__restrict__ in, c
 template <class C>
 __device__ __forceinline__ void load_tile_split_full(const double *__restrict__ in, const SplitCtx &t,
                                                      typename C::T *__restrict__ lds) {
-  using T = typename C::T;
   using Q = typename Quad<typename C::Tr>::V;
   constexpr int NT = C::NW * 64, QPR = C::PX / 4, RPP = NT / QPR;
   static_assert(C::PX % 4 == 0 && RPP >= 1, "quads tile the rows");

@@ -169,6 +169,19 @@ cs_status cs_run(const cs_layout *layout, const cs_stencil *stencil, const doubl
 void cs_fill_random_host(double *a, const cs_layout *layout, uint64_t seed);
 void cs_fill_weights_host(double *w, int32_t count, uint64_t seed);
 
+/* ---- grid files (host only; no device needed) ---- */
+/* The reference's text dump of a whole padded array (save_to_txt, src/1d/main.cu:21-36,
+ * src/2d/main.cu:36-55, src/3d/main.cu:22-40; written as output_gpu.txt by its drivers
+ * built with WRITE_OUTPUT, src/2d/main.cu:410-415): "%.0f" values, 1-D "index value"
+ * lines, 2-D tab-terminated rows, 3-D planes of rows each followed by "\nNext\n". */
+cs_status cs_grid_save_txt(const char *path, const cs_layout *layout, const double *host);
+/* Binary FP64 grid (no counterpart in the reference): a 96-byte header ("CSGRID64", version
+ * 1, ndim, ext[3], org[3], n[3]) and the padded array, exact.  cs_grid_load_f64 fills
+ * *layout and, if host is not NULL, reads the array into host (capacity in doubles);
+ * CS_ERR_INVALID for a missing, foreign, truncated or too large file. */
+cs_status cs_grid_save_f64(const char *path, const cs_layout *layout, const double *host);
+cs_status cs_grid_load_f64(const char *path, cs_layout *layout, double *host, int64_t capacity);
+
 /* ---- drop-in launchers (reference names; see the header comment) ---- */
 /* 1d1r: host arrays of input_n + 7 doubles; interior at [4, input_n + 4) of `in`;
  * 7 fused weights; out[j] = final[j + 1] for j < input_n + 6  (src/1d/gpu_1r.cu:227-301). */
