@@ -78,6 +78,14 @@ SIGNATURES = {
     "gpu_star_1d2r_step2": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int]),
     "gpu_box_2d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "gpu_box_2d3r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "gpu_box_2d1r_breakdown1": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int]),
+    "gpu_box_2d1r_breakdown2": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int]),
+    "gpu_box_2d1r_breakdown3": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int]),
+    "gpu_box_2d1r_breakdown4": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                                ctypes.c_int]),
     "gpu_box_3d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int]),
     "gpu_star_3d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -71,7 +71,7 @@ int main(int argc, char *argv[]) {
     else
       cs::fused_1d2r(v, p2);
   }
-  if (breakdown) printf("note: --breakdown ablation kernels are not part of this build; running the main kernel\n");
+  if (breakdown) printf("note: the --breakdown ladder of this build is the 2-D one; running the main kernel\n");
   const int halo = shape == 0 ? 3 : 4;
   printf("INFO: shape = %s,  n = %d, times = %d\n", kShapeStr[shape], n, times);
   const size_t cols = (size_t)n + 2 * halo;

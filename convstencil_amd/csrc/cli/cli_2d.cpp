@@ -89,7 +89,8 @@ int main(int argc, char *argv[]) {
       p1r[7] = v[4];
     }
   }
-  if (breakdown) printf("note: --breakdown ablation kernels are not part of this build; running the main kernel\n");
+  const bool ladder = breakdown && (shape == box_2d1r || shape == star_2d1r);
+  if (breakdown && !ladder) printf("note: the --breakdown ladder exists for box2d1r and star2d1r; running the main kernel\n");
   double box[49], star[49] = {0.0};
   if (e.exact_fusion)
     cs::fused_2d_exact(p1r, box);
@@ -107,8 +108,15 @@ int main(int argc, char *argv[]) {
     return 1;
   }
   for (size_t i = 0; i < rows * cols; ++i) matrix[i] = (double)(rand() % 100);
-  int rc = (shape == box_2d1r || shape == star_2d1r) ? gpu_box_2d1r(matrix, output, param, times, m, n)
-                                                     : gpu_box_2d3r(matrix, output, param, times, m, n);
+  int rc = 0;
+  if (ladder) {  // the reference's order (src/2d/main.cu:349-357): rungs 1-4, then the main launcher
+    int (*const rungs[4])(const double *, double *, const double *, int, int, int) = {
+        gpu_box_2d1r_breakdown1, gpu_box_2d1r_breakdown2, gpu_box_2d1r_breakdown3, gpu_box_2d1r_breakdown4};
+    for (int k = 0; k < 4 && rc == 0; ++k) rc = rungs[k](matrix, output, param, times, m, n);
+  }
+  if (rc == 0)
+    rc = (shape == box_2d1r || shape == star_2d1r) ? gpu_box_2d1r(matrix, output, param, times, m, n)
+                                                   : gpu_box_2d3r(matrix, output, param, times, m, n);
   if (rc == 0 && cli::wants_output(e)) {
     const long long ext[2] = {(long long)rows, (long long)cols}, org[2] = {3, 4}, nn[2] = {m, n};
     if (!cli::write_grid(e, cli::ref_layout(2, ext, org, nn), output)) rc = 1;

@@ -75,7 +75,7 @@ int main(int argc, char *argv[]) {
     double v[27];
     if (!cli::read_params(shape == 0 ? 27 : 7, v)) return 1;
   }
-  if (breakdown) printf("note: --breakdown ablation kernels are not part of this build; running the main kernel\n");
+  if (breakdown) printf("note: the --breakdown ladder of this build is the 2-D one; running the main kernel\n");
   printf("INFO: shape = %s, h = %d, m = %d, n = %d, times = %d\n", kShapeStr[shape], h, m, n, times);
   const size_t cells = ((size_t)h + 6) * ((size_t)m + 6) * ((size_t)n + 6);
   double *matrix = (double *)malloc(cells * sizeof(double));

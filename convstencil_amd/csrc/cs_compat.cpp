@@ -39,10 +39,11 @@ int fail(const char *what, cs_status s) {
 // Runs `times` single-launch steps on `dev_in` (padded, `cells` doubles) and copies
 // the final padded array to `dev_out`.  Prints nothing; returns elapsed times.
 int run_compat(const cs_layout &L, int radius, const std::vector<double> &dense, const double *in, double *out,
-               int times, long long *ms, double *secs) {
+               int times, long long *ms, double *secs, int variant = 0) {
   cs_stencil st{L.ndim, radius, dense.data()};
   cs_options opt{};
   opt.precision = CS_PREC_FP64;
+  opt.tile_variant = variant;
   cs_plan *p = nullptr;
   cs_status s = cs_plan_create(&p, &L, &st, &opt);
   if (s != CS_OK) return fail("plan creation", s);
@@ -170,6 +171,42 @@ int gpu_box_2d1r(const double *in, double *out, const double *params, int times,
 // src/2d/gpu.cu:336-425 (1 step per launch, :420)
 int gpu_box_2d3r(const double *in, double *out, const double *params, int times, int input_m, int input_n) {
   return run_2d(in, out, params, times, input_m, input_n, 1.0);
+}
+
+// The --breakdown ladder (src/2d/gpu.cu:434-801, run by src/2d/main.cu:349-357): the same
+// launch loop as gpu_box_2d1r on one rung's kernel, printed as the reference prints its
+// rungs.  Rung kernels: 1 FP64 FMA from HBM/L2, 2 FP64 FMA from an LDS tile, 3 the dilated
+// MFMA tables at the smallest LDS pitch, 4 the production single-step kernel (k2d_ablation.hip).
+static int run_breakdown(int rung, int variant, const double *in, double *out, const double *params, int times, int m,
+                         int n) {
+  if (!in || !out || !params || bad_sizes(times, {m, n})) return fail("gpu_box_2d1r_breakdown", CS_ERR_INVALID);
+  const cs_layout L = layout_2d((int64_t)m + 6, (int64_t)n + 8, 3, 4, m, n);
+  std::vector<double> w(params, params + 49);
+  long long ms = 0;
+  double secs = 0;
+  int rc = run_compat(L, 3, w, in, out, times, &ms, &secs, variant);
+  if (rc) return rc;
+  printf("Experiment - Breakdown(2D) %d: \nTime = %lld[ms]\n", rung, ms);
+  printf("GStencil/s = %f\n\n", ((double)m * n * times * 3) / secs / 1e9);
+  fflush(stdout);
+  return 0;
+}
+
+int gpu_box_2d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_m,
+                            int input_n) {
+  return run_breakdown(1, 913000, in, out, params, times, input_m, input_n);
+}
+int gpu_box_2d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_m,
+                            int input_n) {
+  return run_breakdown(2, 923000, in, out, params, times, input_m, input_n);
+}
+int gpu_box_2d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_m,
+                            int input_n) {
+  return run_breakdown(3, 933101, in, out, params, times, input_m, input_n);
+}
+int gpu_box_2d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_m,
+                            int input_n) {
+  return run_breakdown(4, 13101, in, out, params, times, input_m, input_n);
 }
 
 // 196 box weights: plane k = 3 - |dz| holds the 7x7 (dy, dx) slice (src/3d/main.cu:115-467).

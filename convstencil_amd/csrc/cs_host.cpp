@@ -75,7 +75,17 @@ const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   if (prec != CS_PREC_FP64)
     return ndim == 2 ? cs::variants_2d_split(cnt) : ndim == 3 ? cs::variants_3d_split(cnt) : nullptr;
   if (ndim == 1) return cs::variants_1d_f64(cnt);
-  if (ndim == 2) return cs::variants_2d_f64(cnt);
+  if (ndim == 2) {  // the MFMA kernels, then the ladder's FMA kernels
+    static const std::vector<Variant> all = [] {
+      int a = 0, b = 0;
+      const Variant *x = cs::variants_2d_f64(&a), *y = cs::variants_2d_ablation(&b);
+      std::vector<Variant> v(x, x + a);
+      v.insert(v.end(), y, y + b);
+      return v;
+    }();
+    *cnt = (int)all.size();
+    return all.data();
+  }
   if (ndim == 3) return cs::variants_3d_f64(cnt);
   return nullptr;
 }
@@ -116,8 +126,10 @@ cs_status select_variants(cs_plan *p) {
   const int mf = prec != CS_PREC_FP64 ? prec : 4;
   const Variant *v = nullptr;
   if (p->opt.tile_variant > 0) {
+    // a forced variant of the stencil's shape class, or a box variant (it computes any
+    // stencil of its radius: the taps outside the stencil get zero weights)
     v = find_variant(nd, 0, p->R, p->shape, 0, p->opt.tile_variant, prec);
-    if (!v && prec != CS_PREC_FP64) v = find_variant(nd, 0, p->R, cs::SHAPE_BOX, 0, p->opt.tile_variant, prec);
+    if (!v) v = find_variant(nd, 0, p->R, cs::SHAPE_BOX, 0, p->opt.tile_variant, prec);
     if (!v) return CS_ERR_INVALID;
   } else if (prec == CS_PREC_FP64) {
     // 3-D: the 8 x 8 block family (mf 8) where it exists, else the 4 x 16 strips
@@ -291,6 +303,7 @@ std::vector<const Variant *> tune_candidates(const cs_plan *p, int classified_sh
     if (v.R != p->R) continue;
     if (v.shape != classified_shape && !(p->opt.precision != CS_PREC_FP64 && v.shape == cs::SHAPE_BOX)) continue;
     if (p->opt.precision != CS_PREC_FP64 && v.mf != p->opt.precision) continue;
+    if (v.mf == 1 || v.mf == 2) continue;  // the --breakdown ladder's FMA kernels
     if (p->opt.fusion_depth > 0 && v.kf != p->opt.fusion_depth) continue;
     out.push_back(&v);
   }

@@ -63,7 +63,8 @@ typedef hipError_t (*LaunchFn)(const double *in, double *out, const double *bfra
 // One kernel variant of the FP64 stencil2row families.
 struct Variant {
   int id;
-  int mf;      // MFMA shape: 16 = v_mfma_f64_16x16x4, 4 = v_mfma_f64_4x4x4
+  int mf;      // MFMA shape: 16 = v_mfma_f64_16x16x4, 4 = v_mfma_f64_4x4x4; 1, 2 = the
+               // --breakdown ladder's FP64 FMA kernels (global, LDS)
   int R, shape, kf, by, bx, nw;
   size_t lds_bytes;
   int px;  // LDS row pitch in doubles
@@ -80,6 +81,7 @@ const Variant *variants_2d_f64(int *count);
 const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
 const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only
 const Variant *variants_2d_split(int *count);
+const Variant *variants_2d_ablation(int *count);  // --breakdown rungs 1-2 (mf 1, 2; forced by id only)
 const Variant *variants_3d_split(int *count);  // split-precision 3-D (k3d_split.hip)
 
 // Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].

@@ -13,6 +13,34 @@ __global__ __launch_bounds__(NW * 64) void k2d_f64(const double *__restrict__ in
   tile_f64<MF, R, SH, KF, BY, BX, NW, ST>(in, out, bfrag, g, tile, lds, threadIdx.x);
 }
 
+// The same kernel with 128-byte aligned LDS rows (the --breakdown ladder's rung 3).
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+__global__ __launch_bounds__(NW * 64) void k2d_f64u(const double *__restrict__ in, double *__restrict__ out,
+                                                    const double *__restrict__ bfrag, Geo g) {
+  extern __shared__ __align__(16) double lds[];
+  const int64_t ntiles = g.tiles[1] * g.tiles[2];
+  const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+  if (tile >= ntiles) return;  // uniform over the workgroup
+  tile_f64<MF, R, SH, KF, BY, BX, NW, ST, 64, kAligned>(in, out, bfrag, g, tile, lds, threadIdx.x);
+}
+
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+static hipError_t prepare_k2du() {
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST, kAligned>;
+  return hipFuncSetAttribute((const void *)k2d_f64u<MF, R, SH, KF, BY, BX, NW, ST>,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
+}
+
+template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
+static hipError_t launch_k2du(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s) {
+  using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST, kAligned>;
+  const int64_t ntiles = g.tiles[1] * g.tiles[2];
+  const int64_t grid = ((ntiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((k2d_f64u<MF, R, SH, KF, BY, BX, NW, ST>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in,
+                     out, bfrag, g);
+  return hipGetLastError();
+}
+
 template <int MF, int R, int SH, int KF, int BY, int BX, int NW, int ST>
 static hipError_t prepare_k2d() {
   using C = Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>;
@@ -137,6 +165,15 @@ static hipError_t launch_k2dp(const double *in, double *out, const double *bfrag
         Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>::CS                                                \
   }
 
+#define CS_V2DU(ID, MF, R, SH, KF, BY, BX, NW, ST)                                               \
+  Variant {                                                                                        \
+    ID, MF, R, SH, KF, BY, BX, NW, Cfg2D<MF, R, SH, KF, BY, BX, NW, ST, kAligned>::LDS,            \
+        Cfg2D<MF, R, SH, KF, BY, BX, NW, ST, kAligned>::PX, ST, &launch_k2du<MF, R, SH, KF, BY, BX, NW, ST>, \
+        &prepare_k2du<MF, R, SH, KF, BY, BX, NW, ST>,                                               \
+        "k2d_f64<mf" #MF ",r" #R "," #SH ",kf" #KF "," #BY "x" #BX ",w" #NW ",st" #ST ",aligned128>", \
+        Cfg2D<MF, R, SH, KF, BY, BX, NW, ST, kAligned>::CS                                         \
+  }
+
 #define CS_V2D(ID, MF, R, SH, KF, BY, BX, NW, ST)                                            \
   Variant {                                                                                  \
     ID, MF, R, SH, KF, BY, BX, NW, Cfg2D<MF, R, SH, KF, BY, BX, NW, ST>::LDS,                 \
@@ -227,6 +264,11 @@ static const Variant kVariants[] = {
     CS_V2DP(513112, 4, 3, SHAPE_STAR, 2, 64, 64, 8, 1),
     CS_V2DP(513113, 4, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
     CS_V2DP(513114, 4, 3, SHAPE_STAR, 4, 64, 64, 8, 1),
+    // the --breakdown ladder's rung 3: single-step kernels with 128-byte aligned LDS rows
+    // (ids 930000 + R * 1000 + 100 + shape * 10 + KF; listed last so that they are never
+    // the first match)
+    CS_V2DU(933101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 1), CS_V2DU(931101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 1),
+    CS_V2DU(933111, 4, 3, SHAPE_STAR, 1, 64, 64, 8, 1), CS_V2DU(931111, 4, 1, SHAPE_STAR, 1, 64, 64, 8, 1),
 };
 
 const Variant *variants_2d_f64(int *count) {
@@ -236,6 +278,11 @@ const Variant *variants_2d_f64(int *count) {
 
 void build_bfrag_2d_f64(int mf, int R, int shape, const double *w, double *bfrag) {
   const int D = 2 * R + 1;
+  if (mf == 1 || mf == 2) {  // the ladder's FMA kernels read the dense weights
+    const int n = bfrag_ksteps_2d_f64(mf, R, shape) * 64;
+    for (int i = 0; i < n; ++i) bfrag[i] = i < D * D ? w[i] : 0.0;
+    return;
+  }
   if (mf == 16) {
     const KTable kt = make_ktable(R, shape);
     for (int ks = 0; ks < kt.n; ++ks)
@@ -268,6 +315,7 @@ void build_bfrag_2d_f64(int mf, int R, int shape, const double *w, double *bfrag
 }
 
 int bfrag_ksteps_2d_f64(int mf, int R, int shape) {
+  if (mf == 1 || mf == 2) return ((2 * R + 1) * (2 * R + 1) + 63) / 64;
   return mf == 16 ? make_ktable(R, shape).n : dtable_ksteps(make_dtable(R, shape));
 }
 

@@ -30,6 +30,8 @@
 
 namespace cs {
 
+constexpr int kAligned = 99;  // Cfg2D pitch override: rows aligned to 128 bytes
+
 template <int MF_, int R_, int SH_, int KF_, int BY_, int BX_, int NW_, int ST_, int PXM_ = -1>
 struct Cfg2D {
   static constexpr int MF = MF_, R = R_, SH = SH_, KF = KF_, BY = BY_, BX = BX_, NW = NW_, ST = ST_;
@@ -82,8 +84,11 @@ struct Cfg2D {
   static constexpr int NBUF = (KF > 1 && ST == 0) ? 2 : 1;
   // 16x16x4: odd pitch; 4x4x4: the even pitch with the fewest LDS bank conflicts for
   // this table's reads and the result writes that keeps the most workgroups per CU
-  // (cs_ktable.h, lds_cost); PXM_ >= 0 forces the smallest pitch >= PX0 that is PXM_ mod 16
+  // (cs_ktable.h, lds_cost); PXM_ >= 0 forces the smallest pitch >= PX0 that is PXM_ mod 16,
+  // PXM_ = kAligned the smallest multiple of 16 (rows aligned to 128 bytes, no bank-aware
+  // choice: the --breakdown ladder's rung 3)
   static constexpr int PX = MF == 16 ? (PX0 | 1)
+                            : PXM_ == kAligned ? (PX0 + 15) / 16 * 16
                             : PXM_ >= 0 ? PX0 + ((PXM_ - PX0 % 16) + 16) % 16
                                         : best_pitch_dtables(&DT, 1, PX0, (long)NBUF * EY * 8, 16);
   static constexpr int BUF = EY * PX;

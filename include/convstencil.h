@@ -195,6 +195,14 @@ int gpu_star_1d2r_step2(const double *in, double *out, const double *params, int
 int gpu_box_2d1r(const double *in, double *out, const double *params, int times, int input_m, int input_n);
 /* Same computation, 1 step per launch in the GStencil/s line (src/2d/gpu.cu:336-425). */
 int gpu_box_2d3r(const double *in, double *out, const double *params, int times, int input_m, int input_n);
+/* The --breakdown ladder (src/2d/2d_utils.h:81-87, src/2d/gpu.cu:434-801): gpu_box_2d1r's
+ * arguments and result on one rung's kernel each -- 1 FP64 FMA from HBM/L2, 2 FP64 FMA from an
+ * LDS tile, 3 the dilated MFMA tables with 128-byte aligned LDS rows, 4 the production
+ * single-step kernel at the bank model's pitch -- printing "Experiment - Breakdown(2D) K: ", "Time = N[ms]", "GStencil/s = X". */
+int gpu_box_2d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_m, int input_n);
+int gpu_box_2d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_m, int input_n);
+int gpu_box_2d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_m, int input_n);
+int gpu_box_2d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_m, int input_n);
 /* 3D box: (h+6)(m+6)(n+6) arrays, interior [3, .+3); 196 weights = 4 planes of 7x7 for
  * |dz| = 3, 2, 1, 0 (z-symmetric 343-point stencil)  (src/3d/gpu_box.cu:446-536). */
 int gpu_box_3d1r(const double *in, double *out, const double *params, int times, int input_h, int input_m,
