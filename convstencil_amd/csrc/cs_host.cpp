@@ -303,7 +303,8 @@ std::vector<const Variant *> tune_candidates(const cs_plan *p, int classified_sh
     if (v.R != p->R) continue;
     if (v.shape != classified_shape && !(p->opt.precision != CS_PREC_FP64 && v.shape == cs::SHAPE_BOX)) continue;
     if (p->opt.precision != CS_PREC_FP64 && v.mf != p->opt.precision) continue;
-    if (v.mf == 1 || v.mf == 2) continue;  // the --breakdown ladder's FMA kernels
+    // the --breakdown ladder's FMA kernels (for split plans mf holds the precision instead)
+    if (p->opt.precision == CS_PREC_FP64 && (v.mf == 1 || v.mf == 2)) continue;
     if (p->opt.fusion_depth > 0 && v.kf != p->opt.fusion_depth) continue;
     out.push_back(&v);
   }
