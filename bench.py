@@ -154,7 +154,8 @@ def batch_workload(args):
     seeds = [args.seed + 1000 + i for i in range(G)]
     res = {}
     finals = {}
-    for prec in ("fp64", "bf16x3"):
+    precs = ("fp64", "bf16x3", "fp16x2")
+    for prec in precs:
         with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids], precision=prec,
                    use_graph=True) as b:
             b.fill_random(seeds)
@@ -165,19 +166,20 @@ def batch_workload(args):
             res[prec] = {"gstencil_per_s": round(pts * T / (rep.elapsed_ms * 1e-3) / 1e9, 2),
                          "ms_per_run": round(rep.elapsed_ms, 3), "describe": b.describe()}
     threads = oracle.max_threads()
-    errs = {"fp64": [], "bf16x3": []}
+    errs = {p: [] for p in precs}
     t0 = time.perf_counter()
     for i, (L, r, offs, w, dense) in enumerate(grids):
         want = L.interior(oracle.run(L, offs, w, oracle.fill_data(L, seeds[i]), T, threads=threads))
-        for prec in ("fp64", "bf16x3"):
+        for prec in precs:
             got = L.interior(finals[prec][i])
             d = got - want
             errs[prec].append((float(np.max(np.abs(d)) / np.max(np.abs(want))),
                                float(np.sqrt(np.sum(d * d) / np.sum(want * want)))))
     oracle_s = time.perf_counter() - t0
     res["fp64"]["max_rel_max_err"] = max(e[0] for e in errs["fp64"])
-    res["bf16x3"]["max_rel_max_err"] = max(e[0] for e in errs["bf16x3"])
-    res["bf16x3"]["max_rel_l2_err"] = max(e[1] for e in errs["bf16x3"])
+    for prec in precs[1:]:
+        res[prec]["max_rel_max_err"] = max(e[0] for e in errs[prec])
+        res[prec]["max_rel_l2_err"] = max(e[1] for e in errs[prec])
     out = {
         "metric": "GStencil/s at 1 GPU: FP64 MFMA vs split-precision at stated error; % roofline",
         "value": res["fp64"]["gstencil_per_s"], "unit": "GStencil/s", "n_gpus": 1, "steps": args.steps,

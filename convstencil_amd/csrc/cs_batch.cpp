@@ -30,7 +30,7 @@ struct Seq {
 
 struct cs_batch {
   int count = 0;
-  int prec = 0;  // 0 FP64, 1 BF16x3
+  int prec = 0;  // 0 FP64, 1 BF16x3, 2 FP16x2 (compensated weights)
   cs_options opt{};
   int device = 0;
   hipStream_t stream = nullptr;
@@ -196,7 +196,7 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
   if (!out || count <= 0 || !layouts || !stencils) return CS_ERR_INVALID;
   *out = nullptr;
   const int prec = opt ? opt->precision : CS_PREC_FP64;
-  if (prec != CS_PREC_FP64 && prec != CS_PREC_BF16X3) return CS_ERR_UNSUPPORTED;
+  if (prec != CS_PREC_FP64 && prec != CS_PREC_BF16X3 && prec != CS_PREC_FP16X2) return CS_ERR_UNSUPPORTED;
   for (int i = 0; i < count; ++i) {
     const cs_layout &L = layouts[i];
     const cs_stencil &st = stencils[i];
@@ -207,7 +207,7 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
   CS_TRY(check_device(&dev));
   cs_batch *b = new cs_batch();
   b->count = count;
-  b->prec = prec == CS_PREC_FP64 ? 0 : 1;
+  b->prec = prec == CS_PREC_FP64 ? 0 : prec == CS_PREC_BF16X3 ? 1 : 2;
   if (opt) b->opt = *opt;
   b->device = dev;
   b->layouts.assign(layouts, layouts + count);
@@ -241,7 +241,8 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
         s = CS_ERR_ALLOC;
     } else {
       std::vector<unsigned char> h(1 << 20);
-      const size_t bytes = cs::build_bfrag_split2(CS_PREC_BF16X3, R, shape, w.data(), h.data(), h.size());
+      const size_t bytes = cs::build_bfrag_split2(b->prec == 1 ? CS_PREC_BF16X3 : CS_PREC_FP16X2, R, shape, w.data(),
+                                                   h.data(), h.size());
       if (bytes == 0 || hipMalloc(&d, bytes) != hipSuccess ||
           hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
         s = CS_ERR_ALLOC;
@@ -374,7 +375,8 @@ cs_status cs_batch_describe(cs_batch *b, char *buf, int32_t len) {
   int64_t tiles = 0;
   for (const cs_layout &L : b->layouts) tiles += ((L.n[1] + 63) / 64) * ((L.n[2] + 63) / 64);
   snprintf(buf, (size_t)len, "batch grids=%d precision=%s tiles=%lld workgroups=%d lds=%zu", b->count,
-           b->prec == 0 ? "fp64" : "bf16x3", (long long)tiles, b->max_wg, cs::batch_lds_bytes(b->prec));
+           b->prec == 0 ? "fp64" : b->prec == 1 ? "bf16x3" : "fp16x2", (long long)tiles, b->max_wg,
+           cs::batch_lds_bytes(b->prec));
   return CS_OK;
 }
 

@@ -16,6 +16,8 @@
 //     always terminates; the host reports the flag as an error;
 //   * tiles are computed by the same tile bodies as the single-grid kernels
 //     (k2d_f64_body.h, k2d_split_body.h).
+#include <type_traits>
+
 #include "cs_batch.h"
 #include "k2d_f64_body.h"
 #include "k2d_split2_body.h"
@@ -53,7 +55,7 @@ __global__ void batch_reset(unsigned *sync, int n) {
 // batched workload's two stencils (box2d1r, star2d3r), fused and single-step.
 // Every extra body adds to the one kernel's register allocation.
 //   FP64:  0 box r1 kf4, 1 box r1 kf1, 2 star r3 kf2, 3 star r3 kf1
-//   BF16x3 (dilated split form): the same four kinds
+//   BF16x3 and FP16x2 with compensated weights (dilated split forms): the same four kinds
 template <int PREC>
 __device__ __forceinline__ void run_item(const BatchGrid &G, const BatchItem &m, unsigned char *smem, int tid) {
   const double *in = G.buf[m.src];
@@ -68,12 +70,13 @@ __device__ __forceinline__ void run_item(const BatchGrid &G, const BatchItem &m,
       default: break;
     }
   } else {
-    const TrBF16::T *bf = reinterpret_cast<const TrBF16::T *>(G.bfrag);
+    using Tr = typename std::conditional<PREC == 1, TrBF16, TrF16>::type;
+    const typename Tr::T *bf = reinterpret_cast<const typename Tr::T *>(G.bfrag);
     switch (m.kind) {
-      case 0: tile_split2<TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
-      case 1: tile_split2<TrBF16, 1, SHAPE_BOX, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
-      case 2: tile_split2<TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
-      case 3: tile_split2<TrBF16, 3, SHAPE_STAR, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 0: tile_split2<Tr, 1, SHAPE_BOX, 4, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 1: tile_split2<Tr, 1, SHAPE_BOX, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 2: tile_split2<Tr, 3, SHAPE_STAR, 2, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
+      case 3: tile_split2<Tr, 3, SHAPE_STAR, 1, 64, 64, 8, 8>(in, out, bf, G.geo, m.tile, smem, tid); break;
       default: break;
     }
   }
@@ -122,15 +125,21 @@ static size_t batch_lds() {
     m = std::max(m, Cfg2D<4, 3, SHAPE_STAR, 2, 64, 64, 8, 1>::LDS);
     return m;
   } else {
-    return std::max(CfgS2<TrBF16, 1, SHAPE_BOX, 4, 64, 64, 8>::LDS, CfgS2<TrBF16, 3, SHAPE_STAR, 2, 64, 64, 8>::LDS);
+    using Tr = typename std::conditional<PREC == 1, TrBF16, TrF16>::type;
+    return std::max(CfgS2<Tr, 1, SHAPE_BOX, 4, 64, 64, 8>::LDS, CfgS2<Tr, 3, SHAPE_STAR, 2, 64, 64, 8>::LDS);
   }
 }
 
-size_t batch_lds_bytes(int prec) { return prec == 0 ? batch_lds<0>() : batch_lds<1>(); }
+// prec: 0 FP64, 1 BF16x3, 2 FP16x2
+size_t batch_lds_bytes(int prec) { return prec == 0 ? batch_lds<0>() : prec == 1 ? batch_lds<1>() : batch_lds<2>(); }
+
+static const void *batch_fn(int prec) {
+  return prec == 0 ? (const void *)k2d_batch<0> : prec == 1 ? (const void *)k2d_batch<1> : (const void *)k2d_batch<2>;
+}
 
 hipError_t batch_prepare(int prec, int *max_wg) {
   const size_t lds = batch_lds_bytes(prec);
-  const void *fn = prec == 0 ? (const void *)k2d_batch<0> : (const void *)k2d_batch<1>;
+  const void *fn = batch_fn(prec);
   hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   int per_cu = 0, dev = 0;
@@ -153,8 +162,10 @@ hipError_t batch_launch(int prec, const BatchGrid *grids, const BatchItem *items
   const size_t lds = batch_lds_bytes(prec);
   if (prec == 0)
     hipLaunchKernelGGL(k2d_batch<0>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, nwg);
-  else
+  else if (prec == 1)
     hipLaunchKernelGGL(k2d_batch<1>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, nwg);
+  else
+    hipLaunchKernelGGL(k2d_batch<2>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, nwg);
   return hipGetLastError();
 }
 

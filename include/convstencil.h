@@ -129,7 +129,7 @@ cs_status cs_plan_describe(cs_plan *plan, char *buf, int32_t len);
 /* ---- batches: many independent 2-D grids advanced by one persistent launch ----
  * No counterpart in the reference (one grid per process).  Every grid has its own
  * layout and stencil; supported stencils are the radius-1 box (box2d1r) and the
- * radius-3 star (star2d3r), in FP64 and in CS_PREC_BF16X3.  cs_batch_run advances every grid by
+ * radius-3 star (star2d3r), in FP64, CS_PREC_BF16X3 and CS_PREC_FP16X2.  cs_batch_run advances every grid by
  * `steps` steps with a single persistent kernel launch (plus a small reset kernel),
  * optionally captured in a hipGraph (opt->use_graph). */
 typedef struct cs_batch cs_batch;

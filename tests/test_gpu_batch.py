@@ -50,7 +50,7 @@ def _check(batch, grids, steps, prec):
         assert np.array_equal(got[~L.interior_mask()], want[~L.interior_mask()]), i
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3", "fp16x2"])
 @pytest.mark.parametrize("steps", [1, 7, 12])
 def test_batch_matches_oracle(gpu, prec, steps):
     print(f"case batch {prec} T={steps}", flush=True)
@@ -107,3 +107,12 @@ def test_batch_rejects_unsupported(gpu):
     dense = np.full(25, 1.0 / 25)
     with pytest.raises(RuntimeError):
         Batch([L], [2], [dense])
+
+
+def test_batch_rejects_split_forms_without_bodies(gpu):
+    # FP16 with three weight slices and FP32x2 have no batch tile bodies
+    L = Layout.with_halo([40, 40], 1)
+    dense = np.full(9, 1.0 / 9)
+    for prec in ("fp16w3", "fp32x2"):
+        with pytest.raises(RuntimeError):
+            Batch([L], [1], [dense], precision=prec)
