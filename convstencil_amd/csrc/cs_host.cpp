@@ -153,7 +153,9 @@ cs_status select_variants(cs_plan *p) {
   p->shape = v->shape;
   const int fam = prec != CS_PREC_FP64 ? v->st : -1;
   p->vmain = v;
-  p->vone = find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec, fam);
+  // single-step launches: the chosen variant itself when it is a single-step one (a variant
+  // forced by id must be the one that runs), else the family's first single-step variant
+  p->vone = v->kf == 1 ? v : find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec, fam);
   if (!p->vone) return CS_ERR_UNSUPPORTED;
   p->vrem.assign(v->kf, nullptr);
   for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0, prec, fam);
