@@ -175,8 +175,9 @@ int gpu_box_2d3r(const double *in, double *out, const double *params, int times,
 
 // The --breakdown ladder (src/2d/gpu.cu:434-801, run by src/2d/main.cu:349-357): the same
 // launch loop as gpu_box_2d1r on one rung's kernel, printed as the reference prints its
-// rungs.  Rung kernels: 1 FP64 FMA from HBM/L2, 2 FP64 FMA from an LDS tile, 3 the dilated
-// MFMA tables at the smallest LDS pitch, 4 the production single-step kernel (k2d_ablation.hip).
+// rungs.  Rung kernels (k2d_ablation.hip, k2d_f64.hip): 1 FP64 FMA from HBM/L2, 2 FP64 FMA
+// from an LDS tile, 3 the dilated MFMA tables with 128-byte aligned LDS rows, 4 the
+// production single-step kernel (the bank model's pitch, the output staged in LDS).
 static int run_breakdown(int rung, int variant, const double *in, double *out, const double *params, int times, int m,
                          int n) {
   if (!in || !out || !params || bad_sizes(times, {m, n})) return fail("gpu_box_2d1r_breakdown", CS_ERR_INVALID);
@@ -206,7 +207,7 @@ int gpu_box_2d1r_breakdown3(const double *in, double *out, const double *params,
 }
 int gpu_box_2d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_m,
                             int input_n) {
-  return run_breakdown(4, 13101, in, out, params, times, input_m, input_n);
+  return run_breakdown(4, 713101, in, out, params, times, input_m, input_n);
 }
 
 // 196 box weights: plane k = 3 - |dz| holds the 7x7 (dy, dx) slice (src/3d/main.cu:115-467).

@@ -8,7 +8,8 @@
 //   2  the tile and its halo staged in LDS once, FP64 FMAs from LDS (k2d_abl_lds, here);
 //   3  the dilated v_mfma_f64_4x4x4 tables on the tile in LDS with rows aligned to 128
 //      bytes, the layout a straightforward port uses (k2d_f64 with that pitch, k2d_f64.hip);
-//   4  the same at the pitch the LDS bank model picks (the production single-step kernel);
+//   4  the production single-step kernel: the pitch the LDS bank model picks, and the
+//      output staged in LDS and written row by row (k2d_f64 with ST 2);
 // and the production launcher last.  These kernels compute the same stencil as every
 // other variant (halo never written); they run only when a plan is forced to them by id
 // (Variant::mf 1 and 2 mark them), never by the plan's choice or the autotuner.

@@ -189,6 +189,12 @@ static hipError_t launch_k2dp(const double *in, double *out, const double *bfrag
 // takes the first entry that matches (MFMA shape, radius, shape class, fusion depth), so
 // preferred variants come first.
 static const Variant kVariants[] = {
+    // single-step launches (a halo that is not zero, and the reference's pre-fused
+    // weights): the output staged in LDS and written row by row (ST 2) wins at every radius
+    CS_V2D(711101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 2),
+    CS_V2D(711111, 4, 1, SHAPE_STAR, 1, 64, 64, 8, 2),
+    CS_V2D(713101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 2),
+    CS_V2D(713111, 4, 3, SHAPE_STAR, 1, 64, 64, 8, 2),
     // register-staged, v_mfma_f64_4x4x4 on 8 x 8 blocks (default family)
     CS_V2D(11101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 1),
     CS_V2D(11102, 4, 1, SHAPE_BOX, 2, 64, 64, 8, 1),
@@ -235,14 +241,12 @@ static const Variant kVariants[] = {
     CS_V2D(111208, 4, 1, SHAPE_BOX, 8, 64, 64, 4, 1),
     CS_V2D(113202, 4, 3, SHAPE_BOX, 2, 64, 64, 4, 1),
     // last step staged through LDS, coalesced output rows (ids 700000 + R * 1000 + ...)
-    CS_V2D(711101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 2),
     CS_V2D(711102, 4, 1, SHAPE_BOX, 2, 64, 64, 8, 2),
     CS_V2D(711104, 4, 1, SHAPE_BOX, 4, 64, 64, 8, 2),
     CS_V2D(711106, 4, 1, SHAPE_BOX, 6, 64, 64, 8, 2),
     CS_V2D(711108, 4, 1, SHAPE_BOX, 8, 64, 64, 8, 2),
     CS_V2D(711114, 4, 1, SHAPE_STAR, 4, 64, 64, 8, 2),
     CS_V2D(711118, 4, 1, SHAPE_STAR, 8, 64, 64, 8, 2),
-    CS_V2D(713101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 2),
     CS_V2D(713102, 4, 3, SHAPE_BOX, 2, 64, 64, 8, 2),
     CS_V2D(713112, 4, 3, SHAPE_STAR, 2, 64, 64, 8, 2),
     CS_V2D(713113, 4, 3, SHAPE_STAR, 3, 64, 64, 8, 2),
@@ -269,6 +273,13 @@ static const Variant kVariants[] = {
     // the first match)
     CS_V2DU(933101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 1), CS_V2DU(931101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 1),
     CS_V2DU(933111, 4, 3, SHAPE_STAR, 1, 64, 64, 8, 1), CS_V2DU(931111, 4, 1, SHAPE_STAR, 1, 64, 64, 8, 1),
+    // the production depths with 128-byte aligned rows (id + 9000000), for the autotuner
+    CS_V2DU(9711106, 4, 1, SHAPE_BOX, 6, 64, 64, 8, 2), CS_V2DU(9711108, 4, 1, SHAPE_BOX, 8, 64, 64, 8, 2),
+    CS_V2DU(9111106, 4, 1, SHAPE_BOX, 6, 64, 64, 8, 1), CS_V2DU(9013103, 4, 3, SHAPE_BOX, 3, 64, 64, 8, 1),
+    CS_V2DU(9013102, 4, 3, SHAPE_BOX, 2, 64, 64, 8, 1), CS_V2DU(9013113, 4, 3, SHAPE_STAR, 3, 64, 64, 8, 1),
+    CS_V2DU(9713113, 4, 3, SHAPE_STAR, 3, 64, 64, 8, 2), CS_V2DU(9713101, 4, 3, SHAPE_BOX, 1, 64, 64, 8, 2),
+    CS_V2DU(9713111, 4, 3, SHAPE_STAR, 1, 64, 64, 8, 2), CS_V2DU(9711101, 4, 1, SHAPE_BOX, 1, 64, 64, 8, 2),
+    CS_V2DU(9711111, 4, 1, SHAPE_STAR, 1, 64, 64, 8, 2),
 };
 
 const Variant *variants_2d_f64(int *count) {

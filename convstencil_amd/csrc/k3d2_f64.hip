@@ -138,10 +138,14 @@ __device__ __forceinline__ double all_planes(const double *lds, const int *slot,
   }
 }
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+// OS = 1: each output plane is staged in LDS (in the ring slot of plane z - R, which the
+// next plane replaces) and written row by row, coalesced, instead of from the 8 x 8 blocks.
+template <int R, int SH, int BY, int BX, int ZC, int NW, int OS = 0>
 __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ in, double *__restrict__ out,
                                                     const double *__restrict__ bfrag, Geo g) {
   using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
+  constexpr int SP = BX + 2;  // staging pitch: the 8 rows of a block's writes hit distinct banks
+  static_assert(OS == 0 || BY * SP <= C::PLANE, "the staged plane fits a ring slot");
   constexpr int PX = C::PX, NT = NW * 64;
   extern __shared__ __align__(16) double lds[];
 
@@ -231,15 +235,36 @@ __global__ __launch_bounds__(NW * 64) void k3d2_f64(const double *__restrict__ i
       acc[j] = all_planes<C, -R>(lds, slot, y * PX + x, off, b, 0.0);
     }
     double *oplane = out + (g.org[0] + z) * ext1 * ext2;
+    if constexpr (OS == 0) {
 #pragma unroll
-    for (int j = 0; j < C::PER; ++j) {
-      const int blk = j * NW + wave;
-      if ((C::NB % NW == 0 || j < C::PER - 1) || blk < C::NB) {
-        int y, x;
-        block_origin3<C>(blk, y, x);
-        const int ty = y + orow, tx = x + ocol;
-        if (tx >= R && tx < R + BX && (full || (Y0 + ty < ny && X0 + tx < nx)))
-          oplane[(ay0 + ty) * ext2 + ax0 + tx] = acc[j];
+      for (int j = 0; j < C::PER; ++j) {
+        const int blk = j * NW + wave;
+        if ((C::NB % NW == 0 || j < C::PER - 1) || blk < C::NB) {
+          int y, x;
+          block_origin3<C>(blk, y, x);
+          const int ty = y + orow, tx = x + ocol;
+          if (tx >= R && tx < R + BX && (full || (Y0 + ty < ny && X0 + tx < nx)))
+            oplane[(ay0 + ty) * ext2 + ax0 + tx] = acc[j];
+        }
+      }
+    } else {
+      lds_barrier();  // every wave has read planes z - R .. z + R
+      double *stg = lds + slot[0] * C::PLANE;  // plane z - R's slot: BY rows of pitch SP
+#pragma unroll
+      for (int j = 0; j < C::PER; ++j) {
+        const int blk = j * NW + wave;
+        if ((C::NB % NW == 0 || j < C::PER - 1) || blk < C::NB) {
+          int y, x;
+          block_origin3<C>(blk, y, x);
+          const int ty = y + orow, tx = x + ocol;
+          if (tx >= R && tx < R + BX) stg[(ty - R) * SP + tx - R] = acc[j];
+        }
+      }
+      lds_barrier();  // the staged plane is complete
+      constexpr int NT2 = NW * 64;
+      for (int idx = tid; idx < BY * BX; idx += NT2) {
+        const int r = idx / BX, c = idx - r * BX;
+        if (full || (Y0 + R + r < ny && X0 + R + c < nx)) oplane[(ay0 + R + r) * ext2 + ax0 + R + c] = stg[r * SP + c];
       }
     }
     lds_barrier();  // every wave is done with plane z - R before its slot is refilled
@@ -477,32 +502,33 @@ static hipError_t launch_k3f(const double *in, double *out, const double *bfrag,
   return hipGetLastError();
 }
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+template <int R, int SH, int BY, int BX, int ZC, int NW, int OS>
 static hipError_t prepare_k3d2() {
   using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
-  return hipFuncSetAttribute((const void *)k3d2_f64<R, SH, BY, BX, ZC, NW>,
+  return hipFuncSetAttribute((const void *)k3d2_f64<R, SH, BY, BX, ZC, NW, OS>,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)C::LDS);
 }
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+template <int R, int SH, int BY, int BX, int ZC, int NW, int OS>
 static hipError_t launch_k3d2(const double *in, double *out, const double *bfrag, const Geo &g0, hipStream_t s) {
   using C = Cfg3D2<R, SH, BY, BX, ZC, NW>;
   Geo g = g0;
   g.tiles[0] = (g.n[0] + ZC - 1) / ZC;
   const int64_t ntiles = g.tiles[0] * g.tiles[1] * g.tiles[2];
   const int64_t grid = ((ntiles + 7) / 8) * 8;
-  hipLaunchKernelGGL((k3d2_f64<R, SH, BY, BX, ZC, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in, out,
-                     bfrag, g);
+  hipLaunchKernelGGL((k3d2_f64<R, SH, BY, BX, ZC, NW, OS>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in,
+                     out, bfrag, g);
   return hipGetLastError();
 }
 
 // Variant::mf = 8 marks this family (8 x 8 blocks, dilated tables); st = z-chunk length.
-#define CS_V3D2(ID, R, SH, BY, BX, ZC, NW)                                                                     \
+#define CS_V3D2O(ID, R, SH, BY, BX, ZC, NW, OS, OSN)                                                          \
   Variant {                                                                                                    \
     ID, 8, R, SH, 1, BY, BX, NW, Cfg3D2<R, SH, BY, BX, ZC, NW>::LDS, Cfg3D2<R, SH, BY, BX, ZC, NW>::PX, ZC,    \
-        &launch_k3d2<R, SH, BY, BX, ZC, NW>, &prepare_k3d2<R, SH, BY, BX, ZC, NW>,                             \
-        "k3d2_f64<r" #R "," #SH "," #BY "x" #BX ",zc" #ZC ",w" #NW ">", Cfg3D2<R, SH, BY, BX, ZC, NW>::CS    \
+        &launch_k3d2<R, SH, BY, BX, ZC, NW, OS>, &prepare_k3d2<R, SH, BY, BX, ZC, NW, OS>,                     \
+        "k3d2_f64<r" #R "," #SH "," #BY "x" #BX ",zc" #ZC ",w" #NW OSN ">", Cfg3D2<R, SH, BY, BX, ZC, NW>::CS \
   }
+#define CS_V3D2(ID, R, SH, BY, BX, ZC, NW) CS_V3D2O(ID, R, SH, BY, BX, ZC, NW, 0, "")
 
 // Fused family: same tables and B fragments (mf 8), fusion depth KF; st = z-chunk length.
 #define CS_V3F(ID, R, SH, BY, BX, KF, ZC, NW)                                                             \
@@ -515,6 +541,11 @@ static hipError_t launch_k3d2(const double *in, double *out, const double *bfrag
   }
 
 static const Variant kVariants3D2[] = {
+    // single-step: the output planes staged in LDS and written row by row (ids + 300)
+    // are the first choice for the box and star classes
+    CS_V3D2O(41404, 1, SHAPE3_BOX, 16, 128, 32, 8, 1, ",os1"), CS_V3D2O(41414, 1, SHAPE3_STAR, 16, 128, 32, 8, 1, ",os1"),
+    CS_V3D2O(41401, 1, SHAPE3_BOX, 32, 64, 32, 8, 1, ",os1"),  CS_V3D2O(41411, 1, SHAPE3_STAR, 32, 64, 32, 8, 1, ",os1"),
+    CS_V3D2O(41402, 1, SHAPE3_BOX, 16, 64, 32, 4, 1, ",os1"),  CS_V3D2O(41412, 1, SHAPE3_STAR, 16, 64, 32, 4, 1, ",os1"),
     CS_V3D2(41101, 1, SHAPE3_BOX, 32, 64, 32, 8),   CS_V3D2(41111, 1, SHAPE3_STAR, 32, 64, 32, 8),
     CS_V3D2(41121, 1, SHAPE3_DIAMOND, 32, 64, 32, 8), CS_V3D2(41102, 1, SHAPE3_BOX, 16, 64, 32, 4),
     CS_V3D2(41112, 1, SHAPE3_STAR, 16, 64, 32, 4),  CS_V3D2(41103, 1, SHAPE3_BOX, 32, 64, 64, 8),

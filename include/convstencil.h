@@ -198,7 +198,7 @@ int gpu_box_2d3r(const double *in, double *out, const double *params, int times,
 /* The --breakdown ladder (src/2d/2d_utils.h:81-87, src/2d/gpu.cu:434-801): gpu_box_2d1r's
  * arguments and result on one rung's kernel each -- 1 FP64 FMA from HBM/L2, 2 FP64 FMA from an
  * LDS tile, 3 the dilated MFMA tables with 128-byte aligned LDS rows, 4 the production
- * single-step kernel at the bank model's pitch -- printing "Experiment - Breakdown(2D) K: ", "Time = N[ms]", "GStencil/s = X". */
+ * single-step kernel (the bank model's pitch, the output staged in LDS) -- printing "Experiment - Breakdown(2D) K: ", "Time = N[ms]", "GStencil/s = X". */
 int gpu_box_2d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_m, int input_n);
 int gpu_box_2d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_m, int input_n);
 int gpu_box_2d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_m, int input_n);
