@@ -88,6 +88,10 @@ SIGNATURES = {
                                                 ctypes.c_int]),
     "gpu_box_3d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int]),
+    **{f"gpu_1d1r_breakdown{k}": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int])
+       for k in range(1, 5)},
+    **{f"gpu_box_3d1r_breakdown{k}": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int, ctypes.c_int]) for k in range(1, 5)},
     "gpu_star_3d1r": (ctypes.c_int, [c_double_p, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int]),
 }

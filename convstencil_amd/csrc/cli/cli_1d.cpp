@@ -71,7 +71,9 @@ int main(int argc, char *argv[]) {
     else
       cs::fused_1d2r(v, p2);
   }
-  if (breakdown) printf("note: the --breakdown ladder of this build is the 2-D one; running the main kernel\n");
+  // the ladder runs for 1d1r only, as in the reference (src/1d/main.cu:221-233)
+  const bool ladder = breakdown && shape == 0;
+  if (breakdown && !ladder) printf("note: the --breakdown ladder exists for 1d1r; running the main kernel\n");
   const int halo = shape == 0 ? 3 : 4;
   printf("INFO: shape = %s,  n = %d, times = %d\n", kShapeStr[shape], n, times);
   const size_t cols = (size_t)n + 2 * halo;
@@ -82,7 +84,13 @@ int main(int argc, char *argv[]) {
     return 1;
   }
   for (size_t i = 0; i < cols + 1; ++i) input[i] = (double)(rand() % 10000);
-  int rc = shape == 0 ? gpu_1d1r(input, output, p1, times, n) : gpu_star_1d2r_step2(input, output, p2, times, n);
+  int rc = 0;
+  if (ladder) {
+    int (*const rungs[4])(const double *, double *, const double *, int, int) = {
+        gpu_1d1r_breakdown1, gpu_1d1r_breakdown2, gpu_1d1r_breakdown3, gpu_1d1r_breakdown4};
+    for (int k = 0; k < 4 && rc == 0; ++k) rc = rungs[k](input, output, p1, times, n);
+  }
+  if (rc == 0) rc = shape == 0 ? gpu_1d1r(input, output, p1, times, n) : gpu_star_1d2r_step2(input, output, p2, times, n);
   if (rc == 0 && cli::wants_output(e)) {  // the reference dumps the first n + 2 halo cells (src/1d/main.cu:275)
     const long long ext[1] = {(long long)cols}, org[1] = {halo}, nn[1] = {n};
     if (!cli::write_grid(e, cli::ref_layout(1, ext, org, nn), output)) rc = 1;

@@ -75,7 +75,9 @@ int main(int argc, char *argv[]) {
     double v[27];
     if (!cli::read_params(shape == 0 ? 27 : 7, v)) return 1;
   }
-  if (breakdown) printf("note: the --breakdown ladder of this build is the 2-D one; running the main kernel\n");
+  // the ladder runs for box3d1r only, as in the reference (src/3d/main.cu:620-637)
+  const bool ladder = breakdown && shape == 0;
+  if (breakdown && !ladder) printf("note: the --breakdown ladder exists for box3d1r; running the main kernel\n");
   printf("INFO: shape = %s, h = %d, m = %d, n = %d, times = %d\n", kShapeStr[shape], h, m, n, times);
   const size_t cells = ((size_t)h + 6) * ((size_t)m + 6) * ((size_t)n + 6);
   double *matrix = (double *)malloc(cells * sizeof(double));
@@ -85,8 +87,15 @@ int main(int argc, char *argv[]) {
     return 1;
   }
   for (size_t i = 0; i < cells; ++i) matrix[i] = (double)(rand() % 100);
-  int rc = shape == 0 ? gpu_box_3d1r(matrix, output, box, times, h, m, n)
-                      : gpu_star_3d1r(matrix, output, star, times, h, m, n);
+  int rc = 0;
+  if (ladder) {
+    int (*const rungs[4])(const double *, double *, const double *, int, int, int, int) = {
+        gpu_box_3d1r_breakdown1, gpu_box_3d1r_breakdown2, gpu_box_3d1r_breakdown3, gpu_box_3d1r_breakdown4};
+    for (int k = 0; k < 4 && rc == 0; ++k) rc = rungs[k](matrix, output, box, times, h, m, n);
+  }
+  if (rc == 0)
+    rc = shape == 0 ? gpu_box_3d1r(matrix, output, box, times, h, m, n)
+                    : gpu_star_3d1r(matrix, output, star, times, h, m, n);
   if (rc == 0 && cli::wants_output(e)) {
     const long long ext[3] = {(long long)h + 6, (long long)m + 6, (long long)n + 6}, org[3] = {3, 3, 3},
                     nn[3] = {h, m, n};

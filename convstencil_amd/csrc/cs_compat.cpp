@@ -130,6 +130,40 @@ int gpu_1d1r(const double *in, double *out, const double *params, int times, int
   return 0;
 }
 
+// The 1-D --breakdown ladder (src/1d/gpu_1r.cu:305-651, run by src/1d/main.cu:224-228):
+// gpu_1d1r's launch loop and result on one rung's kernel (k1d_ablation.hip, k1d_f64.hip):
+// 1 FP64 FMA from HBM/L2, 2 FP64 FMA from an LDS tile, 3 the MFMA kernel on the reference's
+// 1024-output tiling, 4 the production single-step kernel.
+static int run_breakdown_1d(int rung, int variant, const double *in, double *out, const double *params, int times,
+                            int input_n) {
+  if (!in || !out || !params || bad_sizes(times, {input_n})) return fail("gpu_1d1r_breakdown", CS_ERR_INVALID);
+  const int64_t ext = (int64_t)input_n + 7;
+  const cs_layout L = layout_1d(ext, 4, input_n);
+  std::vector<double> w(params, params + 7), fin((size_t)ext);
+  long long ms = 0;
+  double secs = 0;
+  int rc = run_compat(L, 3, w, in, fin.data(), times, &ms, &secs, variant);
+  if (rc) return rc;
+  memcpy(out, fin.data() + 1, (size_t)(ext - 1) * sizeof(double));
+  printf("Experiment - Breakdown(1D) %d: \nTime = %lld[ms]\n", rung, ms);
+  printf("GStencil/s = %f\n\n", ((double)input_n * times * 3) / secs / 1e9);
+  fflush(stdout);
+  return 0;
+}
+
+int gpu_1d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_n) {
+  return run_breakdown_1d(1, 913000, in, out, params, times, input_n);
+}
+int gpu_1d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_n) {
+  return run_breakdown_1d(2, 923000, in, out, params, times, input_n);
+}
+int gpu_1d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_n) {
+  return run_breakdown_1d(3, 933001, in, out, params, times, input_n);
+}
+int gpu_1d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_n) {
+  return run_breakdown_1d(4, 13001, in, out, params, times, input_n);
+}
+
 // src/1d/gpu_2r.cu:85-159: device array of n + 8, interior [4, n + 4), 9 weights,
 // host output = the first n + 7 device elements.
 int gpu_star_1d2r_step2(const double *in, double *out, const double *params, int times, int input_n) {
@@ -259,6 +293,43 @@ int gpu_box_3d1r(const double *in, double *out, const double *params, int times,
   if (!in || !out || !params || bad_sizes(times, {input_h, input_m, input_n}))
     return fail("gpu_box_3d1r", CS_ERR_INVALID);
   return run_3d("box", dense_box3d(params), in, out, times, input_h, input_m, input_n, true);
+}
+
+// The 3-D --breakdown ladder (src/3d/gpu_box.cu:538-904, run by src/3d/main.cu:623-628):
+// gpu_box_3d1r's launch loop and result on one rung's kernel (k3d_ablation.hip,
+// k3d_f64.hip, k3d_fma.hip): 1 FP64 FMA from HBM/L2, 2 FP64 FMA from a ring of LDS planes,
+// 3 the MFMA strip kernel with 128-byte aligned LDS rows, 4 the production kernel
+// (register-blocked FMAs over streamed planes).  The reference prints the elapsed seconds
+// on a line of their own here too (:618-622).
+static int run_breakdown_3d(int rung, int variant, const double *in, double *out, const double *params, int times,
+                            int h, int m, int n) {
+  if (!in || !out || !params || bad_sizes(times, {h, m, n})) return fail("gpu_box_3d1r_breakdown", CS_ERR_INVALID);
+  const cs_layout L = layout_3d(h, m, n, 3);
+  long long ms = 0;
+  double secs = 0;
+  int rc = run_compat(L, 3, dense_box3d(params), in, out, times, &ms, &secs, variant);
+  if (rc) return rc;
+  printf("Experiment - Breakdown(3D) %d: \nTime = %lld[ms]\n%g\n", rung, ms, secs);
+  printf("GStencil/s = %f\n\n", ((double)m * n * h * times * 3) / secs / 1e9);
+  fflush(stdout);
+  return 0;
+}
+
+int gpu_box_3d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n) {
+  return run_breakdown_3d(1, 913000, in, out, params, times, input_h, input_m, input_n);
+}
+int gpu_box_3d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n) {
+  return run_breakdown_3d(2, 923000, in, out, params, times, input_h, input_m, input_n);
+}
+int gpu_box_3d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n) {
+  return run_breakdown_3d(3, 933001, in, out, params, times, input_h, input_m, input_n);
+}
+int gpu_box_3d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n) {
+  return run_breakdown_3d(4, 35302, in, out, params, times, input_h, input_m, input_n);
 }
 
 // src/3d/gpu_star.cu:238-332

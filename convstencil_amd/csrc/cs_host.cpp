@@ -70,24 +70,33 @@ int default_kf(int ndim, int R, int top) {
   return std::min(top, R == 1 ? 4 : 2);
 }
 
+std::vector<Variant> joined(const Variant *(*a)(int *), const Variant *(*b)(int *)) {
+  int na = 0, nb = 0;
+  const Variant *x = a(&na), *y = b(&nb);
+  std::vector<Variant> v(x, x + na);
+  v.insert(v.end(), y, y + nb);
+  return v;
+}
+
+std::vector<Variant> joined3(const Variant *(*a)(int *), const Variant *(*b)(int *), const Variant *(*c)(int *)) {
+  std::vector<Variant> v = joined(a, b);
+  int nc = 0;
+  const Variant *z = c(&nc);
+  v.insert(v.end(), z, z + nc);
+  return v;
+}
+
 const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   *cnt = 0;
   if (prec != CS_PREC_FP64)
     return ndim == 2 ? cs::variants_2d_split(cnt) : ndim == 3 ? cs::variants_3d_split(cnt) : nullptr;
-  if (ndim == 1) return cs::variants_1d_f64(cnt);
-  if (ndim == 2) {  // the MFMA kernels, then the ladder's FMA kernels
-    static const std::vector<Variant> all = [] {
-      int a = 0, b = 0;
-      const Variant *x = cs::variants_2d_f64(&a), *y = cs::variants_2d_ablation(&b);
-      std::vector<Variant> v(x, x + a);
-      v.insert(v.end(), y, y + b);
-      return v;
-    }();
-    *cnt = (int)all.size();
-    return all.data();
-  }
-  if (ndim == 3) return cs::variants_3d_f64(cnt);
-  return nullptr;
+  if (ndim < 1 || ndim > 3) return nullptr;
+  // the MFMA kernels, then the --breakdown ladder's FMA kernels
+  static const std::vector<Variant> all[3] = {
+      joined(cs::variants_1d_f64, cs::variants_1d_ablation), joined(cs::variants_2d_f64, cs::variants_2d_ablation),
+      joined3(cs::variants_3d_f64, cs::variants_3d_fma, cs::variants_3d_ablation)};
+  *cnt = (int)all[ndim - 1].size();
+  return all[ndim - 1].data();
 }
 
 // First variant of the family with this (MFMA shape or precision, radius, shape class,
@@ -132,8 +141,9 @@ cs_status select_variants(cs_plan *p) {
     if (!v) v = find_variant(nd, 0, p->R, cs::SHAPE_BOX, 0, p->opt.tile_variant, prec);
     if (!v) return CS_ERR_INVALID;
   } else if (prec == CS_PREC_FP64) {
-    // 3-D: the 8 x 8 block family (mf 8) where it exists, else the 4 x 16 strips
-    for (int fmf : {nd == 3 ? 8 : mf, mf}) {
+    // 3-D: the 8 x 8 block family (mf 8) where it exists, else the register-blocked FMA
+    // kernels (mf 3, radius 2-3), else the 4 x 16 strips
+    for (int fmf : {nd == 3 ? 8 : mf, nd == 3 ? 3 : mf, mf}) {
       const int top = max_kf(nd, fmf, p->R, p->shape, prec);
       int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, p->R, p->shape, kf, 0, prec);
@@ -172,6 +182,17 @@ cs_status select_variants(cs_plan *p) {
     p->bfrag = nullptr;
     CS_HIP(hipMalloc(&p->bfrag, bytes));
     CS_HIP(hipMemcpy(p->bfrag, hb.data(), bytes, hipMemcpyHostToDevice));
+    return CS_OK;
+  }
+  // the FMA kernels (the ladder's, and the 3-D mf 3 family) read the dense weights
+  // (zero-padded to whole 64-entry rows)
+  if (v->mf == 1 || v->mf == 2 || v->mf == 3) {
+    std::vector<double> h((p->w.size() + 63) / 64 * 64, 0.0);
+    std::copy(p->w.begin(), p->w.end(), h.begin());
+    if (p->bfrag) (void)hipFree(p->bfrag);
+    p->bfrag = nullptr;
+    CS_HIP(hipMalloc(&p->bfrag, h.size() * sizeof(double)));
+    CS_HIP(hipMemcpy(p->bfrag, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
     return CS_OK;
   }
   // B fragments for the family's MFMA shape

@@ -82,6 +82,9 @@ const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf
 const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only
 const Variant *variants_2d_split(int *count);
 const Variant *variants_2d_ablation(int *count);  // --breakdown rungs 1-2 (mf 1, 2; forced by id only)
+const Variant *variants_1d_ablation(int *count);  // the same for 1-D
+const Variant *variants_3d_ablation(int *count);  // the same for 3-D
+const Variant *variants_3d_fma(int *count);       // dense FP64 FMA, radius 2-3 (mf 3)
 const Variant *variants_3d_split(int *count);  // split-precision 3-D (k3d_split.hip)
 
 // Host-side B-fragment construction for the 2-D FP64 family: bfrag[ks * 64 + lane].

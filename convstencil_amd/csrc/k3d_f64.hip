@@ -21,11 +21,13 @@
 
 namespace cs {
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+// AL = 1: rows aligned to 128 bytes (the --breakdown ladder's rung 3) instead of the odd
+// pitch, which puts the 16 rows a strip's lanes read in distinct LDS banks.
+template <int R, int SH, int BY, int BX, int ZC, int NW, int AL = 0>
 struct Cfg3D {
   static constexpr int EY = BY + 2 * R;
   static constexpr int EX = BX + 2 * R;
-  static constexpr int PX = (EX % 2 == 0) ? EX + 1 : EX;  // odd pitch
+  static constexpr int PX = AL ? (EX + 15) / 16 * 16 : (EX % 2 == 0) ? EX + 1 : EX;
   static constexpr int PLANE = EY * PX;
   static constexpr int NP = 2 * R + 1;
   static constexpr size_t LDS = (size_t)NP * PLANE * sizeof(double);
@@ -54,10 +56,10 @@ struct Cfg3D {
   static_assert(LDS <= 160 * 1024, "tile does not fit gfx950 LDS");
 };
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+template <int R, int SH, int BY, int BX, int ZC, int NW, int AL>
 __global__ __launch_bounds__(NW * 64) void k3d_f64(const double *__restrict__ in, double *__restrict__ out,
                                                    const double *__restrict__ bfrag, Geo g) {
-  using C = Cfg3D<R, SH, BY, BX, ZC, NW>;
+  using C = Cfg3D<R, SH, BY, BX, ZC, NW, AL>;
   constexpr KTable3D kt = C::KT;
   constexpr int PX = C::PX;
   constexpr int NBR = C::BREG ? C::NKS : C::max_ks_dz();
@@ -166,31 +168,32 @@ __global__ __launch_bounds__(NW * 64) void k3d_f64(const double *__restrict__ in
   }
 }
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+template <int R, int SH, int BY, int BX, int ZC, int NW, int AL>
 static hipError_t prepare_k3d() {
-  using C = Cfg3D<R, SH, BY, BX, ZC, NW>;
-  return hipFuncSetAttribute((const void *)k3d_f64<R, SH, BY, BX, ZC, NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  using C = Cfg3D<R, SH, BY, BX, ZC, NW, AL>;
+  return hipFuncSetAttribute((const void *)k3d_f64<R, SH, BY, BX, ZC, NW, AL>, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)C::LDS);
 }
 
-template <int R, int SH, int BY, int BX, int ZC, int NW>
+template <int R, int SH, int BY, int BX, int ZC, int NW, int AL>
 static hipError_t launch_k3d(const double *in, double *out, const double *bfrag, const Geo &g0, hipStream_t s) {
-  using C = Cfg3D<R, SH, BY, BX, ZC, NW>;
+  using C = Cfg3D<R, SH, BY, BX, ZC, NW, AL>;
   Geo g = g0;
   g.tiles[0] = (g.n[0] + ZC - 1) / ZC;
   const int64_t ntiles = g.tiles[0] * g.tiles[1] * g.tiles[2];
   const int64_t grid = ((ntiles + 7) / 8) * 8;
-  hipLaunchKernelGGL((k3d_f64<R, SH, BY, BX, ZC, NW>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in, out,
+  hipLaunchKernelGGL((k3d_f64<R, SH, BY, BX, ZC, NW, AL>), dim3((unsigned)grid), dim3(NW * 64), C::LDS, s, in, out,
                      bfrag, g);
   return hipGetLastError();
 }
 
-#define CS_V3D(ID, R, SH, BY, BX, ZC, NW)                                                                         \
+#define CS_V3DA(ID, R, SH, BY, BX, ZC, NW, AL, SUF)                                                            \
   Variant {                                                                                                       \
-    ID, 4, R, SH, 1, BY, BX, NW, Cfg3D<R, SH, BY, BX, ZC, NW>::LDS, Cfg3D<R, SH, BY, BX, ZC, NW>::PX, ZC,         \
-        &launch_k3d<R, SH, BY, BX, ZC, NW>, &prepare_k3d<R, SH, BY, BX, ZC, NW>,                                  \
-        "k3d_f64<r" #R "," #SH "," #BY "x" #BX ",zc" #ZC ",w" #NW ">"                                            \
+    ID, 4, R, SH, 1, BY, BX, NW, Cfg3D<R, SH, BY, BX, ZC, NW, AL>::LDS, Cfg3D<R, SH, BY, BX, ZC, NW, AL>::PX, ZC, \
+        &launch_k3d<R, SH, BY, BX, ZC, NW, AL>, &prepare_k3d<R, SH, BY, BX, ZC, NW, AL>,                          \
+        "k3d_f64<r" #R "," #SH "," #BY "x" #BX ",zc" #ZC ",w" #NW SUF ">"                                        \
   }
+#define CS_V3D(ID, R, SH, BY, BX, ZC, NW) CS_V3DA(ID, R, SH, BY, BX, ZC, NW, 0, "")
 
 // For 3-D variants the Variant::st field carries the z-chunk length.
 static const Variant kVariants3D[] = {
@@ -201,6 +204,9 @@ static const Variant kVariants3D[] = {
     CS_V3D(33001, 3, SHAPE3_BOX, 16, 64, 32, 8),
     CS_V3D(33011, 3, SHAPE3_STAR, 16, 64, 32, 8),
     CS_V3D(33021, 3, SHAPE3_DIAMOND, 16, 64, 32, 8),
+    // the --breakdown ladder's rung 3 (listed last: never the first match)
+    CS_V3DA(933001, 3, SHAPE3_BOX, 16, 64, 32, 8, 1, ",aligned128"),
+    CS_V3DA(933011, 3, SHAPE3_STAR, 16, 64, 32, 8, 1, ",aligned128"),
 };
 
 const Variant *variants_3d_f64(int *count) {

@@ -14,6 +14,8 @@
  *    layouts, same weight layouts, same stdout lines ("ConvStencil(2D): ",
  *    "Time = N[ms]", "GStencil/s = X").  `times` counts launches as in the
  *    reference (each launch applies the given, already time-fused, weights once).
+ *    The --breakdown ladders' launchers (src/1d/1d_utils.h:77-83,
+ *    src/2d/2d_utils.h:81-87, src/3d/3d_utils.h:79-85) are declared beside them.
  *    Where the reference prints a CUDA error and exits, these print the error and
  *    return a non-zero cs_status; the CLIs turn that into exit code 1.
  *
@@ -189,6 +191,14 @@ int gpu_1d1r(const double *in, double *out, const double *params, int times, int
 /* 1d2r: host arrays of input_n + 8 doubles; interior [4, input_n + 4); 9 fused weights;
  * out[j] = final[j] for j < input_n + 7  (src/1d/gpu_2r.cu:85-159). */
 int gpu_star_1d2r_step2(const double *in, double *out, const double *params, int times, int input_n);
+/* The 1-D --breakdown ladder (src/1d/1d_utils.h:77-83, src/1d/gpu_1r.cu:305-651): gpu_1d1r's
+ * arguments and result on one rung's kernel each -- 1 FP64 FMA from HBM/L2, 2 FP64 FMA from an
+ * LDS tile, 3 the MFMA kernel on the reference's 1024-output tiling, 4 the production
+ * single-step kernel -- printing "Experiment - Breakdown(1D) K: ", "Time = N[ms]", "GStencil/s = X". */
+int gpu_1d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_n);
+int gpu_1d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_n);
+int gpu_1d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_n);
+int gpu_1d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_n);
 /* 2D: host arrays of (input_m + 6) x (input_n + 8); interior rows [3, m+3), cols [4, n+4);
  * 49 weights row-major over (dy, dx) in [-3,3]^2  (src/2d/gpu.cu:245-333).  Prints with the
  * 3-steps-per-launch GStencil/s formula of src/2d/gpu.cu:329. */
@@ -207,6 +217,19 @@ int gpu_box_2d1r_breakdown4(const double *in, double *out, const double *params,
  * |dz| = 3, 2, 1, 0 (z-symmetric 343-point stencil)  (src/3d/gpu_box.cu:446-536). */
 int gpu_box_3d1r(const double *in, double *out, const double *params, int times, int input_h, int input_m,
                  int input_n);
+/* The 3-D --breakdown ladder (src/3d/3d_utils.h:79-85, src/3d/gpu_box.cu:538-904):
+ * gpu_box_3d1r's arguments and result on one rung's kernel each -- 1 FP64 FMA from HBM/L2, 2
+ * FP64 FMA from a ring of LDS planes, 3 the MFMA strip kernel with 128-byte aligned LDS rows,
+ * 4 the production kernel (register-blocked FMAs over streamed planes) -- printing "Experiment - Breakdown(3D) K: ", "Time = N[ms]", the
+ * seconds, "GStencil/s = X". */
+int gpu_box_3d1r_breakdown1(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n);
+int gpu_box_3d1r_breakdown2(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n);
+int gpu_box_3d1r_breakdown3(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n);
+int gpu_box_3d1r_breakdown4(const double *in, double *out, const double *params, int times, int input_h, int input_m,
+                            int input_n);
 /* 3D star: same layout; 44 weights for the 63-point octahedron |dz|+|dy|+|dx| <= 3, planes
  * |dz| = 3, 2, 1, 0 with 1, 5, 13, 25 taps, z-symmetric  (src/3d/gpu_star.cu:238-332). */
 int gpu_star_3d1r(const double *in, double *out, const double *params, int times, int input_h, int input_m,

@@ -155,3 +155,44 @@ def test_every_variant(gpu, vid, R, shape, vname):
         p.run(5)
         got = p.download()
     _check(L, got, oracle.run(L, offs, w, x, 5))
+
+
+# the register-blocked FMA kernels (k3d_fma.hip, Variant::mf 3): radius 2 and 3, each
+# shape class, partial tiles and z-chunks
+FMA_3D = [(35302, 3, "box"), (35322, 3, "diamond"), (35312, 3, "star"), (35303, 3, "box"), (35323, 3, "diamond"),
+          (35305, 3, "box"), (35325, 3, "diamond"), (35300, 3, "box"), (35310, 3, "star"), (35320, 3, "diamond"),
+          (35301, 3, "box"), (35321, 3, "diamond"), (35202, 2, "box"), (35212, 2, "star"), (35222, 2, "diamond"),
+          (35200, 2, "box"), (35210, 2, "star"), (35220, 2, "diamond")]
+
+
+def _offsets3(r, kind):
+    rr = range(-r, r + 1)
+    keep = {"box": lambda a, b, c: True,
+            "star": lambda a, b, c: (a != 0) + (b != 0) + (c != 0) <= 1,
+            "diamond": lambda a, b, c: abs(a) + abs(b) + abs(c) <= r}[kind]
+    return [(a, b, c) for a in rr for b in rr for c in rr if keep(a, b, c)]
+
+
+@pytest.mark.parametrize("vid,r,kind", FMA_3D)
+def test_fma_variants(gpu, vid, r, kind):
+    print(f"case k3d_fma {vid} r{r} {kind}", flush=True)
+    offs = _offsets3(r, kind)
+    for sizes in ([11, 13, 70], [40, 33, 65], [3, 70, 129]):
+        L = Layout.with_halo(sizes, r)
+        w = oracle.fill_weights(len(offs), vid + sizes[0])
+        x = oracle.fill_data(L, vid + sizes[1])
+        with Plan(L, r, dense_weights(3, r, offs, w), variant=vid) as p:
+            assert f"variant={vid}" in p.describe()
+            p.upload(x)
+            p.run(3)
+            got = p.download()
+        _check(L, got, oracle.run(L, offs, w, x, 3))
+
+
+def test_fma_is_the_radius3_default(gpu):
+    # radius 3 has no 8 x 8 block kernel: plans take the FMA family, box and octahedron alike
+    for kind, vid in (("box", 35302), ("diamond", 35322), ("star", 35312)):
+        offs = _offsets3(3, kind)
+        L = Layout.with_halo([8, 20, 70], 3)
+        with Plan(L, 3, dense_weights(3, 3, offs, oracle.fill_weights(len(offs), 5))) as p:
+            assert f"variant={vid}" in p.describe(), p.describe()
