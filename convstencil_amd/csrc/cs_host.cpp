@@ -64,10 +64,15 @@ Geo make_geo(const cs_layout &L, const Variant *v) {
 
 // Default fusion depth when the caller does not choose one (the autotuner's
 // cache, when present, overrides this).
-int default_kf(int ndim, int R, int top) {
-  if (ndim == 1) return std::min(top, 32);
-  if (ndim == 3) return std::min(top, 2);
-  return std::min(top, R == 1 ? 4 : 2);
+// FP64 (measured, GStencil/s per physical step): 1d1r 2^20 kf16 878 (kf32 842); box2d1r
+// 10240^2 kf6 925-990 (kf4 861); star2d3r 8192^2 kf3 494 (kf2 454); box3d1r 512^3 kf1
+// 260-265 (kf2 219); star3d1r kf2 303 (kf1 272).  Split precision: kf4 (2-D radius 1),
+// kf2 (2-D radius 2-3 and 3-D).
+int default_kf(int ndim, int R, int top, int shape, bool fp64) {
+  if (ndim == 1) return std::min(top, 16);
+  if (ndim == 3) return std::min(top, fp64 && shape == cs::SHAPE3_BOX ? 1 : 2);
+  if (!fp64) return std::min(top, R == 1 ? 4 : 2);
+  return std::min(top, R == 1 ? 6 : 3);
 }
 
 std::vector<Variant> joined(const Variant *(*a)(int *), const Variant *(*b)(int *)) {
@@ -145,7 +150,8 @@ cs_status select_variants(cs_plan *p) {
     // kernels (mf 3, radius 2-3), else the 4 x 16 strips
     for (int fmf : {nd == 3 ? 8 : mf, nd == 3 ? 3 : mf, mf}) {
       const int top = max_kf(nd, fmf, p->R, p->shape, prec);
-      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
+      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top)
+                                       : default_kf(nd, p->R, top, p->shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, p->R, p->shape, kf, 0, prec);
       if (v) break;
     }
@@ -154,7 +160,8 @@ cs_status select_variants(cs_plan *p) {
     for (const auto &tr : tries) {
       const int top = max_kf(nd, mf, p->R, tr[0], prec, tr[1]);
       if (top == 0) continue;
-      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top) : default_kf(nd, p->R, top);
+      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top)
+                                       : default_kf(nd, p->R, top, tr[0], false);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, mf, p->R, tr[0], kf, 0, prec, tr[1]);
       if (v) break;
     }

@@ -16,7 +16,8 @@
 //     TY (2R+1)^2 TX FMAs (about 34 per value for the box), the weights are staged in LDS
 //     and read as wave-uniform broadcasts;
 //   * the stencil's shape class (box, axes, octahedron) drops zero taps at compile time.
-// Plans use it where the 8 x 8 block family (k3d2_f64.hip) has no kernel for the radius.
+// Plans use it where the 8 x 8 block family (k3d2_f64.hip) has no kernel for the radius
+// (at radius 1 that family is faster: 27 taps are memory-bound, not compute-bound).
 #include "cs_internal.h"
 
 namespace cs {
