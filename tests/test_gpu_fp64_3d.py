@@ -162,7 +162,7 @@ def test_every_variant(gpu, vid, R, shape, vname):
 FMA_3D = [(35302, 3, "box"), (35322, 3, "diamond"), (35312, 3, "star"), (35303, 3, "box"), (35323, 3, "diamond"),
           (35305, 3, "box"), (35325, 3, "diamond"), (35300, 3, "box"), (35310, 3, "star"), (35320, 3, "diamond"),
           (35301, 3, "box"), (35321, 3, "diamond"), (35202, 2, "box"), (35212, 2, "star"), (35222, 2, "diamond"),
-          (35200, 2, "box"), (35210, 2, "star"), (35220, 2, "diamond")]
+          (35200, 2, "box"), (35210, 2, "star"), (35220, 2, "diamond"), (35306, 3, "box")]
 
 
 def _offsets3(r, kind):
