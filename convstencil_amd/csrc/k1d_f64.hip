@@ -139,6 +139,9 @@ static const Variant kVariants1D[] = {
     CS_V1D(13002, 3, 2, 2048, 8),  CS_V1D(13004, 3, 4, 2048, 8),  CS_V1D(13008, 3, 8, 2048, 8),
     CS_V1D(14001, 4, 1, 2048, 8),  CS_V1D(14002, 4, 2, 2048, 8),  CS_V1D(14004, 4, 4, 2048, 8),
     CS_V1D(14008, 4, 8, 2048, 8),
+    // smaller tiles (more workgroups for grids of a few million points): autotuner candidates
+    CS_V1D(511016, 1, 16, 1024, 4), CS_V1D(521016, 1, 16, 1024, 8), CS_V1D(531016, 1, 16, 512, 4),
+    CS_V1D(511008, 1, 8, 1024, 4),  CS_V1D(511032, 1, 32, 1024, 4), CS_V1D(531008, 1, 8, 512, 4),
     // the --breakdown ladder's rung 3: the reference's tiling, 1024 outputs per 256 threads
     // (listed last: never the first match)
     CS_V1D(933001, 3, 1, 1024, 4),

@@ -94,3 +94,21 @@ def test_reference_gpu_1d2r(gpu):
     got[:n + 7] = out[:n + 7]
     _check(L, got, want)
     assert np.array_equal(out[:4], want[:4])
+
+
+@pytest.mark.parametrize("vid", [511016, 521016, 531016, 511008, 511032, 531008, 933001])
+def test_tile_variants(gpu, vid):
+    # every alternative tile, forced: partial tiles, fused launches and remainders
+    print(f"case variant {vid}", flush=True)
+    r = 3 if vid == 933001 else 1
+    offs = [(0, 0, d) for d in range(-r, r + 1)]
+    for n, steps in ((5000, 37), (513, 16), (100003, 9)):
+        L = Layout.with_halo([n], r)
+        w = oracle.fill_weights(len(offs), vid + n)
+        x = oracle.fill_data(L, vid + steps)
+        with Plan(L, r, dense_weights(1, r, offs, w), variant=vid) as p:
+            assert f"variant={vid}" in p.describe()
+            p.upload(x)
+            p.run(steps)
+            got = p.download()
+        _check(L, got, oracle.run(L, offs, w, x, steps))
