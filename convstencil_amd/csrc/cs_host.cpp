@@ -398,9 +398,10 @@ void cs_fill_weights_host(double *w, int32_t count, uint64_t seed) {
 
 namespace {
 
-// Times every candidate variant for kTuneSteps steps (graph as the plan asks) on the
-// plan's own buffers, seeded synthetic data, and keeps the fastest; the buffers are
-// zeroed afterwards (the caller uploads or fills next).
+// Times every candidate variant for at least kTuneSteps steps, a whole number of its
+// fused launches (a remainder launch would come from another variant), with the graph
+// as the plan asks, on the plan's own buffers and seeded synthetic data, and keeps the
+// fastest; the buffers are zeroed afterwards (the caller uploads or fills next).
 constexpr int64_t kTuneSteps = 24;
 
 cs_status autotune_plan(cs_plan *p, int classified) {
@@ -417,8 +418,9 @@ cs_status autotune_plan(cs_plan *p, int classified) {
     if (select_variants(p) != CS_OK) continue;
     CS_TRY(cs_plan_fill_random(p, 1));
     cs_report r{};
-    CS_TRY(cs_plan_time(p, kTuneSteps, 1, 2, &r));
-    const double ms = r.elapsed_ms / (double)kTuneSteps;
+    const int64_t steps = (kTuneSteps + v->kf - 1) / v->kf * v->kf;
+    CS_TRY(cs_plan_time(p, steps, 1, 3, &r));
+    const double ms = r.elapsed_ms / (double)steps;
     if (p->opt.verbose) fprintf(stderr, "convstencil: autotune %s %.4f ms/step\n", v->name, ms);
     if (ms < best_ms) {
       best_ms = ms;
