@@ -272,6 +272,12 @@ int64_t count_launches(const cs_plan *p, int64_t steps) {
 cs_status run_steps(cs_plan *p, int64_t steps) {
   if (steps < 0) return CS_ERR_INVALID;
   if (steps == 0) return CS_OK;
+  if (p->L.n[0] == 0 || p->L.n[1] == 0 || p->L.n[2] == 0) {
+    // no interior cell: every step only swaps the two buffers (no kernel has work)
+    if (steps & 1) p->cur ^= 1;
+    p->steps_done += steps;
+    return CS_OK;
+  }
   CS_HIP(hipSetDevice(p->device));
   if (!p->opt.use_graph) {
     CS_TRY(issue_steps(p, steps));

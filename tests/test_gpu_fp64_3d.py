@@ -196,3 +196,20 @@ def test_fma_is_the_radius3_default(gpu):
         L = Layout.with_halo([8, 20, 70], 3)
         with Plan(L, 3, dense_weights(3, 3, offs, oracle.fill_weights(len(offs), 5))) as p:
             assert f"variant={vid}" in p.describe(), p.describe()
+
+
+@pytest.mark.parametrize("ndim,sizes", [(1, [0]), (2, [0, 7]), (2, [5, 0]), (3, [3, 0, 4]), (3, [0, 2, 2])])
+@pytest.mark.parametrize("steps", [1, 2, 3])
+def test_empty_interior(gpu, ndim, sizes, steps):
+    # no interior cell: the steps only swap the ping-pong buffers, as the oracle's loop does
+    name = {1: "1d1r", 2: "box2d1r", 3: "box3d1r"}[ndim]
+    L = Layout.with_halo(sizes, 1)
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), 9)
+    x = np.random.default_rng(steps).uniform(-1, 1, L.cells)  # a halo that is not zero
+    with Plan(L, 1, dense_weights(ndim, 1, offs, w)) as p:
+        p.upload(x)
+        p.run(steps)
+        got = p.download()
+    want = oracle.run(L, offs, w, x, steps)  # odd steps: the zeroed second buffer
+    assert got.tobytes() == want.tobytes()
