@@ -180,6 +180,9 @@ def batch_workload(args):
     for prec in precs[1:]:
         res[prec]["max_rel_max_err"] = max(e[0] for e in errs[prec])
         res[prec]["max_rel_l2_err"] = max(e[1] for e in errs[prec])
+        # the split bar: relative L2 <= 1e-6 (and relative max <= 5e-6) on every grid
+        res[prec]["within_bar"] = bool(res[prec]["max_rel_l2_err"] <= 1e-6 and res[prec]["max_rel_max_err"] <= 5e-6)
+    res["fp64"]["within_bar"] = bool(res["fp64"]["max_rel_max_err"] <= 1e-12)
     out = {
         "metric": "GStencil/s at 1 GPU: FP64 MFMA vs split-precision at stated error; % roofline",
         "value": res["fp64"]["gstencil_per_s"], "unit": "GStencil/s", "n_gpus": 1, "steps": args.steps,

@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/convstencil_debug.h"
 #include "cs_batch.h"
 #include "cs_host_util.h"
 #include "cs_rng.h"
@@ -39,11 +40,14 @@ struct cs_batch {
   std::vector<double *> buf0, buf1, bfrag;
   std::vector<int> cur;        // buffer holding each grid's state
   std::vector<char> uniform;   // both halos equal -> fused blocks are exact
+  std::vector<double> amax, wabs;  // FP16 range guard: bound on |cell| and sum |w| per grid
   cs::BatchGrid *d_grids = nullptr;
   std::vector<cs::BatchGrid> h_grids;  // host copy; must outlive the asynchronous upload
   int max_wg = 0;
   std::map<int64_t, Seq> seqs;  // keyed by steps (rebuilt when the per-grid state changes)
   int64_t launches = 0;
+  unsigned *d_error = nullptr;  // sticky device error word: set by a barrier that timed out
+  bool failed = false;          // latched once the device word has been seen set
 };
 
 namespace {
@@ -110,7 +114,7 @@ cs_status build_seq(cs_batch *b, int64_t steps, Seq *s) {
   if (flat.size() > (size_t)0x7fffffff) return CS_ERR_UNSUPPORTED;
   s->nblocks = (int)blocks.size();
   s->nitems = (int64_t)flat.size();
-  s->nsync = s->nblocks + 2;
+  s->nsync = s->nblocks + 1;  // barrier counter + one work counter per step block
   if (!flat.empty()) {
     CS_HIP(hipMalloc(&s->items, flat.size() * sizeof(cs::BatchItem)));
     CS_HIP(hipMemcpy(s->items, flat.data(), flat.size() * sizeof(cs::BatchItem), hipMemcpyHostToDevice));
@@ -124,13 +128,26 @@ cs_status build_seq(cs_batch *b, int64_t steps, Seq *s) {
 
 cs_status issue(cs_batch *b, Seq &s) {
   const unsigned nwg = (unsigned)b->max_wg;
-  CS_HIP(cs::batch_launch(b->prec, b->d_grids, s.items, s.block_start, s.nblocks, s.sync, s.nsync, nwg, b->stream));
+  CS_HIP(cs::batch_launch(b->prec, b->d_grids, s.items, s.block_start, s.nblocks, s.sync, s.nsync, b->d_error, nwg,
+                          b->stream));
   return CS_OK;
 }
 
+cs_status check_error_flag(cs_batch *b);
+
 cs_status run(cs_batch *b, int64_t steps) {
   if (steps < 0) return CS_ERR_INVALID;
+  if (b->failed) return CS_ERR_DEVICE;
   if (steps == 0) return CS_OK;
+  std::vector<double> bound((size_t)b->count);
+  const int precision = b->prec == 2 ? CS_PREC_FP16X2 : b->prec == 1 ? CS_PREC_BF16X3 : CS_PREC_FP64;
+  for (int g = 0; g < b->count; ++g)
+    if (!fp16_range_ok(precision, b->amax[(size_t)g], b->wabs[(size_t)g], steps, &bound[(size_t)g])) {
+      fprintf(stderr,
+              "convstencil: FP16 split precision needs |values| < 2^15; grid %d may reach %.3g within %lld step(s)\n",
+              g, bound[(size_t)g], (long long)steps);
+      return CS_ERR_INVALID;
+    }
   CS_HIP(hipSetDevice(b->device));
   auto it = b->seqs.find(steps);
   if (it != b->seqs.end() && it->second.uniform != b->uniform) {
@@ -172,19 +189,24 @@ cs_status run(cs_batch *b, int64_t steps) {
   CS_HIP(hipStreamSynchronize(b->stream));
   b->launches += 1;
   for (int g = 0; g < b->count; ++g) b->cur[(size_t)g] ^= s.parity_flip[(size_t)g];
-  return CS_OK;
+  b->amax = bound;
+  return check_error_flag(b);
 }
 
+// Latches the batch's failure: once a barrier has timed out (the grids' contents are then
+// undefined), every later run, timing and download of this batch returns CS_ERR_DEVICE.
+cs_status latch_error(cs_batch *b, unsigned flag) {
+  if (flag && !b->failed) fprintf(stderr, "convstencil: batch barrier did not complete within its spin limit\n");
+  if (flag) b->failed = true;
+  return b->failed ? CS_ERR_DEVICE : CS_OK;
+}
+
+// Reads the sticky device word (the stream has been synchronised by the caller).
 cs_status check_error_flag(cs_batch *b) {
-  for (auto &kv : b->seqs) {
-    unsigned flag = 0;
-    CS_HIP(hipMemcpy(&flag, kv.second.sync + 1, sizeof(unsigned), hipMemcpyDeviceToHost));
-    if (flag) {
-      fprintf(stderr, "convstencil: batch barrier did not complete within its spin limit\n");
-      return CS_ERR_DEVICE;
-    }
-  }
-  return CS_OK;
+  if (b->failed) return CS_ERR_DEVICE;
+  unsigned flag = 0;
+  CS_HIP(hipMemcpy(&flag, b->d_error, sizeof(unsigned), hipMemcpyDeviceToHost));
+  return latch_error(b, flag);
 }
 
 }  // namespace
@@ -256,10 +278,19 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
     }
     b->buf0.push_back(p0);
     b->buf1.push_back(p1);
+    if (s == CS_OK && (cs::launch_zero(p0, (size_t)cell_count(&layouts[i]), b->stream) != hipSuccess ||
+                       cs::launch_zero(p1, (size_t)cell_count(&layouts[i]), b->stream) != hipSuccess))
+      s = CS_ERR_DEVICE;
     b->cur.push_back(0);
     b->uniform.push_back(1);
+    b->amax.push_back(0.0);  // both buffers are zeroed below
+    b->wabs.push_back(abs_sum(w.data(), w.size()));
   }
   if (s == CS_OK && hipMalloc(&b->d_grids, (size_t)count * sizeof(cs::BatchGrid)) != hipSuccess) s = CS_ERR_ALLOC;
+  if (s == CS_OK && (hipMalloc(&b->d_error, sizeof(unsigned)) != hipSuccess ||
+                     hipMemset(b->d_error, 0, sizeof(unsigned)) != hipSuccess))
+    s = CS_ERR_ALLOC;
+  if (s == CS_OK && hipStreamSynchronize(b->stream) != hipSuccess) s = CS_ERR_DEVICE;
   if (s == CS_OK && cs::batch_prepare(b->prec, &b->max_wg) != hipSuccess) s = CS_ERR_DEVICE;
   if (s == CS_OK && b->max_wg <= 0) s = CS_ERR_UNSUPPORTED;
   if (s != CS_OK) {
@@ -281,6 +312,7 @@ cs_status cs_batch_destroy(cs_batch *b) {
   for (double *p : b->bfrag)
     if (p) (void)hipFree(p);
   if (b->d_grids) (void)hipFree(b->d_grids);
+  if (b->d_error) (void)hipFree(b->d_error);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return CS_OK;
@@ -296,6 +328,7 @@ cs_status cs_batch_upload(cs_batch *b, int32_t grid, const double *host_in) {
   CS_HIP(hipStreamSynchronize(b->stream));
   b->cur[(size_t)grid] = 0;
   b->uniform[(size_t)grid] = halo_is_zero(&L, host_in) ? 1 : 0;
+  b->amax[(size_t)grid] = abs_max(host_in, n);
   return CS_OK;
 }
 
@@ -314,6 +347,7 @@ cs_status cs_batch_fill_random(cs_batch *b, const uint64_t *seeds) {
     CS_HIP(cs::launch_zero(b->buf1[(size_t)i], (size_t)cell_count(&L), b->stream));
     b->cur[(size_t)i] = 0;
     b->uniform[(size_t)i] = 1;
+    b->amax[(size_t)i] = 1.0;  // uniform in [-1, 1) with a zero halo
   }
   CS_HIP(hipStreamSynchronize(b->stream));
   return CS_OK;
@@ -353,7 +387,7 @@ cs_status cs_batch_time(cs_batch *b, int64_t steps, int32_t warmup, int32_t reps
     rep->elapsed_ms = ms / reps;
     double pts = 0;
     for (const cs_layout &L : b->layouts) pts += (double)L.n[1] * (double)L.n[2];
-    rep->gstencil_per_s = pts * (double)steps / (rep->elapsed_ms * 1e-3) / 1e9;
+    rep->gstencil_per_s = rep->elapsed_ms > 0 && pts > 0 ? pts * (double)steps / (rep->elapsed_ms * 1e-3) / 1e9 : 0.0;
     rep->launches = (b->launches - l0) / reps;
   }
   return CS_OK;
@@ -378,6 +412,12 @@ cs_status cs_batch_describe(cs_batch *b, char *buf, int32_t len) {
            b->prec == 0 ? "fp64" : b->prec == 1 ? "bf16x3" : "fp16x2", (long long)tiles, b->max_wg,
            cs::batch_lds_bytes(b->prec));
   return CS_OK;
+}
+
+int32_t cs_debug_batch_latch(const uint32_t *flags, int32_t n, int32_t *status) {
+  cs_batch b;  // host-only: no device state is touched
+  for (int32_t i = 0; i < n; ++i) status[i] = (int32_t)latch_error(&b, flags[i]);
+  return 0;
 }
 
 }  // extern "C"

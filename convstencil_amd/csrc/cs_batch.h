@@ -23,8 +23,11 @@ struct BatchItem {
 
 size_t batch_lds_bytes(int prec);
 hipError_t batch_prepare(int prec, int *max_wg);
+// sync: nsync words (barrier counter, then one work counter per step block), cleared by
+// every launch; error: the batch's sticky error word (set by a barrier that timed out,
+// never cleared by a launch).
 hipError_t batch_launch(int prec, const BatchGrid *grids, const BatchItem *items, const int *block_start,
-                        int nblocks, unsigned *sync, int nsync, unsigned nwg, hipStream_t s);
+                        int nblocks, unsigned *sync, int nsync, unsigned *error, unsigned nwg, hipStream_t s);
 int batch_kind(int prec, int R, int shape, int kf);
 int batch_main_kf(int prec, int R);
 

@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "cs_fused_weights.h"
+#include "cs_host_util.h"
 #include "cs_internal.h"
 
 extern "C" {
@@ -15,6 +16,7 @@ static const cs::Variant *family(int32_t ndim, int *n) {
   if (ndim == 1) return cs::variants_1d_f64(n);
   if (ndim == 2) return cs::variants_2d_f64(n);
   if (ndim == 3) return cs::variants_3d_f64(n);
+  if (ndim == 6) return cs::variants_2d_stream(n);
   return nullptr;
 }
 
@@ -108,6 +110,15 @@ int32_t cs_debug_bfrag_3d(int32_t radius, int32_t shape, const double *w, double
   cs::build_bfrag_3d_f64(radius, shape, w, bfrag);
   return nks;
 }
+
+int32_t cs_debug_fp16_range(int32_t precision, double amax, double wabs, int64_t steps, double *bound) {
+  double b = 0.0;
+  const bool ok = cs::host::fp16_range_ok(precision, amax, wabs, steps, &b);
+  if (bound) *bound = b;
+  return ok ? 1 : 0;
+}
+
+double cs_debug_abs_max(const double *a, int64_t n) { return cs::host::abs_max(a, (size_t)n); }
 
 void cs_debug_fused_2d(int32_t exact, const double *p9, double *out49) {
   if (exact)

@@ -28,13 +28,21 @@ struct Header {
 };
 static_assert(sizeof(Header) == 96, "fixed header size");
 
+// Extents come from file headers too, so every product is checked: the cell count must
+// fit, in bytes, in an int64 (a wrapped product could pass a capacity check while the
+// extents describe far more cells than were read).
 bool layout_ok(const cs_layout *L) {
   if (!L || L->ndim < 1 || L->ndim > 3) return false;
   for (int d = 0; d < 3; ++d) {
-    if (L->ext[d] < 1 || L->n[d] < 0 || L->org[d] < 0 || L->org[d] + L->n[d] > L->ext[d]) return false;
+    if (L->ext[d] < 1 || L->n[d] < 0 || L->org[d] < 0 || L->org[d] > L->ext[d] || L->n[d] > L->ext[d] - L->org[d])
+      return false;
     if (d < 3 - L->ndim && (L->ext[d] != 1 || L->n[d] != 1 || L->org[d] != 0)) return false;
   }
-  return L->ext[0] * L->ext[1] * L->ext[2] > 0;
+  int64_t c = 0, bytes = 0;
+  if (__builtin_mul_overflow(L->ext[0], L->ext[1], &c) || __builtin_mul_overflow(c, L->ext[2], &c) ||
+      __builtin_mul_overflow(c, (int64_t)sizeof(double), &bytes))
+    return false;
+  return c > 0;
 }
 
 int64_t cells(const cs_layout *L) { return L->ext[0] * L->ext[1] * L->ext[2]; }

@@ -45,6 +45,9 @@ struct cs_plan {
   std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
   int64_t launches = 0;
   int tuned = 0;  // 0 defaults or a forced variant, 1 cached autotuner choice, 2 timed now
+  // FP16 range guard: a bound on |cell| of both buffers (halo included) and the weights'
+  // sum of magnitudes, which bounds each step's growth
+  double amax = 0.0, wabs = 0.0;
 };
 
 namespace {
@@ -98,7 +101,8 @@ const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   if (ndim < 1 || ndim > 3) return nullptr;
   // the MFMA kernels, then the --breakdown ladder's FMA kernels
   static const std::vector<Variant> all[3] = {
-      joined(cs::variants_1d_f64, cs::variants_1d_ablation), joined(cs::variants_2d_f64, cs::variants_2d_ablation),
+      joined(cs::variants_1d_f64, cs::variants_1d_ablation),
+      joined3(cs::variants_2d_f64, cs::variants_2d_stream, cs::variants_2d_ablation),
       joined3(cs::variants_3d_f64, cs::variants_3d_fma, cs::variants_3d_ablation)};
   *cnt = (int)all[ndim - 1].size();
   return all[ndim - 1].data();
@@ -272,6 +276,15 @@ int64_t count_launches(const cs_plan *p, int64_t steps) {
 cs_status run_steps(cs_plan *p, int64_t steps) {
   if (steps < 0) return CS_ERR_INVALID;
   if (steps == 0) return CS_OK;
+  double bound = 0.0;
+  if (!fp16_range_ok(p->opt.precision, p->amax, p->wabs, steps, &bound)) {
+    fprintf(stderr,
+            "convstencil: FP16 split precision needs |values| < 2^15; the state may reach %.3g within %lld "
+            "step(s)\n",
+            bound, (long long)steps);
+    return CS_ERR_INVALID;
+  }
+  p->amax = bound;
   if (p->L.n[0] == 0 || p->L.n[1] == 0 || p->L.n[2] == 0) {
     // no interior cell: every step only swaps the two buffers (no kernel has work)
     if (steps & 1) p->cur ^= 1;
@@ -517,6 +530,15 @@ cs_status cs_plan_create(cs_plan **out, const cs_layout *layout, const cs_stenci
     cs_plan_destroy(p);
     return CS_ERR_ALLOC;
   }
+  // both buffers start at zero, so the range guard's bound (0) holds before any upload
+  if (cs::launch_zero(p->buf[0], (size_t)cell_count(layout), p->stream) != hipSuccess ||
+      cs::launch_zero(p->buf[1], (size_t)cell_count(layout), p->stream) != hipSuccess ||
+      hipStreamSynchronize(p->stream) != hipSuccess) {
+    cs_plan_destroy(p);
+    return CS_ERR_DEVICE;
+  }
+  p->amax = 0.0;
+  p->wabs = abs_sum(p->w.data(), p->w.size());
   if (p->opt.autotune > 0 && p->opt.tile_variant == 0) {
     s = autotune_plan(p, classified);
     if (s != CS_OK) {
@@ -550,6 +572,7 @@ cs_status cs_plan_upload(cs_plan *p, const double *host_in) {
   p->cur = 0;
   p->steps_done = 0;
   p->halo_uniform = halo_is_zero(&p->L, host_in);
+  p->amax = abs_max(host_in, n);
   return CS_OK;
 }
 
@@ -568,6 +591,7 @@ cs_status cs_plan_fill_random(cs_plan *p, uint64_t seed) {
   p->cur = 0;
   p->steps_done = 0;
   p->halo_uniform = true;
+  p->amax = 1.0;  // the fill is uniform in [-1, 1) with a zero halo
   return CS_OK;
 }
 
@@ -605,7 +629,8 @@ cs_status cs_plan_time(cs_plan *p, int64_t steps, int32_t warmup, int32_t reps, 
     memset(rep, 0, sizeof(*rep));
     rep->elapsed_ms = ms / reps;
     const double pts = (double)p->L.n[0] * (double)p->L.n[1] * (double)p->L.n[2];
-    rep->gstencil_per_s = pts * (double)steps / (rep->elapsed_ms * 1e-3) / 1e9;
+    // a grid without interior cells launches nothing: no rate
+    rep->gstencil_per_s = rep->elapsed_ms > 0 && pts > 0 ? pts * (double)steps / (rep->elapsed_ms * 1e-3) / 1e9 : 0.0;
     rep->launches = (p->launches - l0) / reps;
     rep->fusion_depth = p->halo_uniform ? p->vmain->kf : 1;
     rep->variant = p->vmain->id;
@@ -654,6 +679,7 @@ cs_status cs_run(const cs_layout *layout, const cs_stencil *st, const double *ho
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (s == CS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) s = CS_ERR_DEVICE;
   if (s == CS_OK && hipEventRecord(e0, p->stream) != hipSuccess) s = CS_ERR_DEVICE;
+  const int64_t l0 = p->launches;  // the autotuner's timing launches are not this run's
   if (s == CS_OK) s = run_steps(p, steps);
   if (s == CS_OK && hipEventRecord(e1, p->stream) != hipSuccess) s = CS_ERR_DEVICE;
   if (s == CS_OK) s = cs_plan_download(p, host_out);
@@ -664,7 +690,7 @@ cs_status cs_run(const cs_layout *layout, const cs_stencil *st, const double *ho
     rep->elapsed_ms = ms;
     const double pts = (double)layout->n[0] * (double)layout->n[1] * (double)layout->n[2];
     rep->gstencil_per_s = ms > 0 ? pts * (double)steps / (ms * 1e-3) / 1e9 : 0.0;
-    rep->launches = p->launches;
+    rep->launches = p->launches - l0;
     rep->fusion_depth = p->halo_uniform ? p->vmain->kf : 1;
     rep->variant = p->vmain->id;
   }

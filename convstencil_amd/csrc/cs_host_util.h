@@ -86,6 +86,34 @@ inline bool halo_is_zero(const cs_layout *L, const double *a) {
   return true;
 }
 
+// Largest magnitude in a[0 .. n); +inf if any value is not finite.
+inline double abs_max(const double *a, size_t n) {
+  double m = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double v = std::fabs(a[i]);
+    if (!(v <= m)) m = std::isfinite(v) ? v : HUGE_VAL;
+  }
+  return m;
+}
+
+inline double abs_sum(const double *w, size_t n) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += std::fabs(w[i]);
+  return s;
+}
+
+// FP16 split forms hold every value as FP16 slices, so the state must stay below 2^15 in
+// magnitude.  One step maps a state bounded by a to one bounded by a * sum|w|, so `steps`
+// steps from a bound amax end below amax * max(1, sum|w|)^steps (*bound_out).  Other
+// precisions always pass.
+inline bool fp16_range_ok(int precision, double amax, double wabs, int64_t steps, double *bound_out) {
+  double b = amax;
+  if (wabs > 1.0 && b > 0.0) b *= std::pow(wabs, (double)steps);  // inf when it overflows
+  *bound_out = b;
+  if (precision != CS_PREC_FP16X2 && precision != CS_PREC_FP16W3) return true;
+  return b < 32768.0;
+}
+
 inline cs_status check_device(int *dev_out) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return CS_ERR_NO_DEVICE;

@@ -78,6 +78,7 @@ struct Variant {
 
 const Variant *variants_1d_f64(int *count);
 const Variant *variants_2d_f64(int *count);
+const Variant *variants_2d_stream(int *count);  // pipelined persistent 2-D FP64 (k2d_stream.hip)
 const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
 const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only
 const Variant *variants_2d_split(int *count);

@@ -45,8 +45,9 @@ __device__ __forceinline__ void grid_barrier(unsigned *counter, unsigned *error,
   __syncthreads();
 }
 
-// Zeroes the barrier counter, the error flag and the per-block work counters
-// (a kernel rather than a memset so that the sequence can be graph-captured).
+// Zeroes the barrier counter and the per-block work counters (a kernel rather than a
+// memset so that the sequence can be graph-captured).  The error flag has a word of its
+// own that this never clears: a barrier that timed out in any launch stays reported.
 __global__ void batch_reset(unsigned *sync, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) sync[i] = 0u;
 }
@@ -86,10 +87,10 @@ template <int PREC>
 __global__ __launch_bounds__(512, 4) void k2d_batch(const BatchGrid *__restrict__ grids,
                                                  const BatchItem *__restrict__ items,
                                                  const int *__restrict__ block_start, int nblocks, unsigned *sync,
-                                                 unsigned nwg) {
+                                                 unsigned *error, unsigned nwg) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int next;
-  unsigned *barrier = sync, *error = sync + 1, *work = sync + 2;
+  unsigned *barrier = sync, *work = sync + 1;
   for (int b = 0; b < nblocks; ++b) {
     const int beg = block_start[b], end = block_start[b + 1];
     for (;;) {
@@ -155,17 +156,17 @@ hipError_t batch_prepare(int prec, int *max_wg) {
 }
 
 hipError_t batch_launch(int prec, const BatchGrid *grids, const BatchItem *items, const int *block_start,
-                        int nblocks, unsigned *sync, int nsync, unsigned nwg, hipStream_t s) {
+                        int nblocks, unsigned *sync, int nsync, unsigned *error, unsigned nwg, hipStream_t s) {
   hipLaunchKernelGGL(batch_reset, dim3(1), dim3(256), 0, s, sync, nsync);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const size_t lds = batch_lds_bytes(prec);
   if (prec == 0)
-    hipLaunchKernelGGL(k2d_batch<0>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, nwg);
+    hipLaunchKernelGGL(k2d_batch<0>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, error, nwg);
   else if (prec == 1)
-    hipLaunchKernelGGL(k2d_batch<1>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, nwg);
+    hipLaunchKernelGGL(k2d_batch<1>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, error, nwg);
   else
-    hipLaunchKernelGGL(k2d_batch<2>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, nwg);
+    hipLaunchKernelGGL(k2d_batch<2>, dim3(nwg), dim3(512), lds, s, grids, items, block_start, nblocks, sync, error, nwg);
   return hipGetLastError();
 }
 

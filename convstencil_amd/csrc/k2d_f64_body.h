@@ -329,7 +329,7 @@ __device__ __forceinline__ bool in_interior(const TileCtx &t, int ty, int tx) {
 template <class C, bool MASK>
 __device__ __forceinline__ void copy_out4(const TileCtx &t, const char *buf) {
   constexpr int NT = C::NW * 64, OFF = C::KF * C::R, RP = NT / C::BX;
-  static_assert(NT % C::BX == 0, "a pass covers whole rows");
+  static_assert(NT % C::BX == 0 && C::BY % RP == 0, "passes cover whole rows and the whole block");
   const int r0 = t.tid / C::BX, c = t.tid - r0 * C::BX;
   const bool col_in = !MASK || (unsigned)(OFF + c - t.cx0) < (unsigned)t.ncx;
   double *o = t.gout + (int64_t)(OFF + r0) * t.pitch + OFF + c;

@@ -1,6 +1,7 @@
 // k2d_split.hip -- split-precision 2-D ConvStencil kernels: the dilated form
 // (k2d_split2_body.h, BF16x3 and FP16x2, preferred) and the Toeplitz form
 // (k2d_split_body.h, also FP32x2).
+#include "cs_wcomp.h"
 #include "k2d_split2_body.h"
 
 namespace cs {
@@ -322,69 +323,6 @@ static size_t build_split(int R, const double *w, unsigned char *out, size_t cap
   return bytes;
 }
 
-// FP16x2 weights whose rounding errors cancel in the low moments.  Rounded to two FP16
-// slices, w = h + l / 2048 is off by up to 2^-23 relative in every tap and in the same
-// direction at every step; over many steps the sum of those errors (the mass the stencil
-// adds or removes) and their first moments (a drift) dominate the result's error.  The
-// low slices are moved by whole FP16 ulps (at most two per tap, greedily, the move that
-// most reduces |sum d| + |sum d dy| + |sum d dx| first, d = w - h - l / 2048) until no
-// move helps; taps with zero weight stay zero.  Modelled on box2d3r over 64 steps, this
-// takes the relative L2 error from up to 1.1e-6 down to about 5e-8.
-static void compensated_f16x2(int R, const double *w, _Float16 *h, _Float16 *l) {
-  const int D = 2 * R + 1, N = D * D;
-  double d[49];
-  int moved[49];
-  double r[3] = {0.0, 0.0, 0.0};
-  for (int t = 0; t < N; ++t) {
-    _Float16 s[2];
-    split_value<TrF16>(w[t], s);
-    h[t] = s[0];
-    l[t] = s[1];
-    d[t] = w[t] - ((double)h[t] + (double)l[t] / 2048.0);
-    moved[t] = 0;
-    r[0] += d[t];
-    r[1] += d[t] * (t / D - R);
-    r[2] += d[t] * (t % D - R);
-  }
-  auto f16_ulp = [](double v) {
-    const double a = v < 0 ? -v : v;
-    if (a < 6.103515625e-05) return 5.9604644775390625e-08;  // 2^-24, subnormal spacing
-    int e = 0;
-    (void)frexp(a, &e);
-    return ldexp(1.0, e - 11);
-  };
-  for (int iter = 0; iter < 4 * N; ++iter) {
-    const double cur = fabs(r[0]) + fabs(r[1]) + fabs(r[2]);
-    double best = cur;
-    int bt = -1, bs = 0;
-    _Float16 bl = 0;
-    for (int t = 0; t < N; ++t) {
-      if (w[t] == 0.0) continue;
-      for (int sgn = -1; sgn <= 1; sgn += 2) {
-        if (moved[t] + sgn > 2 || moved[t] + sgn < -2) continue;
-        const _Float16 nl = (_Float16)((double)l[t] + sgn * f16_ulp((double)l[t]));
-        if (nl == l[t]) continue;
-        const double nd = w[t] - ((double)h[t] + (double)nl / 2048.0), dd = nd - d[t];
-        const double c = fabs(r[0] + dd) + fabs(r[1] + dd * (t / D - R)) + fabs(r[2] + dd * (t % D - R));
-        if (c < best) {
-          best = c;
-          bt = t;
-          bs = sgn;
-          bl = nl;
-        }
-      }
-    }
-    if (bt < 0) break;
-    const double nd = w[bt] - ((double)h[bt] + (double)bl / 2048.0), dd = nd - d[bt];
-    r[0] += dd;
-    r[1] += dd * (bt / D - R);
-    r[2] += dd * (bt % D - R);
-    d[bt] = nd;
-    l[bt] = bl;
-    moved[bt] += bs;
-  }
-}
-
 // A fragments of the dilated form: weight slice q, instruction i, lane l, element j is
 // slice q of W[t - v_m] with m = l & 15, chunk 4 i + (l >> 4) and t = (ty, tx0 + j).
 template <class Tr>
@@ -397,14 +335,25 @@ static size_t build_split2(int R, int shape, const double *w, unsigned char *out
   T *o = reinterpret_cast<T *>(out);
   const int D = 2 * R + 1;
   T tap[3][49] = {};  // slices of every tap
-  if constexpr (Tr::NSU == 2 && Tr::NSW == 2 && Tr::SCALE == 2048.0) {
-    compensated_f16x2(R, w, tap[0], tap[1]);
-  } else {
+  for (int t = 0; t < D * D; ++t) {
+    T sl[3];
+    split_weight<Tr>(w[t], sl);
+    for (int q = 0; q < Tr::NSW; ++q) tap[q][t] = sl[q];
+  }
+  if constexpr (Tr::NSW == Tr::NSU) {
+    // BF16x3 and FP16x2: the last weight slice is moved by ulps until the weights' errors
+    // cancel in the mass and the first moments (cs_wcomp.h); FP16 w3's third slice leaves
+    // errors near 2^-36 and needs none
+    double fixed[49];
+    int off[49][3];
+    const double last_scale = Tr::NSW == 3 ? Tr::SCALE * Tr::SCALE : Tr::SCALE;
     for (int t = 0; t < D * D; ++t) {
-      T sl[3];
-      split_weight<Tr>(w[t], sl);
-      for (int q = 0; q < Tr::NSW; ++q) tap[q][t] = sl[q];
+      fixed[t] = (double)(float)tap[0][t] + (Tr::NSW == 3 ? (double)(float)tap[1][t] / Tr::SCALE : 0.0);
+      off[t][0] = 0;
+      off[t][1] = t / D - R;
+      off[t][2] = t % D - R;
     }
+    compensate_low_slice<T>(D * D, w, off, fixed, tap[Tr::NSW - 1], last_scale);
   }
   for (int q = 0; q < Tr::NSW; ++q)
     for (int i = 0; i < nins; ++i)

@@ -145,32 +145,31 @@ __device__ __forceinline__ void split_f32(float v, typename Tr::T *s) {
 }
 
 // Splits four FP32 values into slice quads (one 8-byte LDS store per slice).  BF16x3:
-// by truncation -- h = v with the low 16 bits cleared, m likewise from v - h, l = the
-// rest -- which is exact (v = h + m + l) for every normal v and costs three 32-bit
-// operations per value, the quads being packed with one byte permute per value pair.
+// each slice is the remainder rounded to nearest (v_cvt_pk_bf16_f32, two values per
+// instruction): h = rn(v), m = rn(v - h), l = rn(v - h - m), exact (v = h + m + l) for
+// every normal v.  Rounding to nearest keeps the lower slices centred on zero; slices
+// cut by truncation carry the sign of v, so the products of level 3 that the kernel
+// drops (w1 u2, w2 u1) would bias every step the same way.
 template <class Tr>
 __device__ __forceinline__ void split4(const float (&v)[4], typename Quad<Tr>::V (&q)[Tr::NSU]) {
   using T = typename Tr::T;
   if constexpr (Tr::NSU == 3) {
-    uint32_t s[3][4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint32_t h = __float_as_uint(v[e]) & 0xffff0000u;
-      const float r1 = v[e] - __uint_as_float(h);
-      const uint32_t m = __float_as_uint(r1) & 0xffff0000u;
-      const float r2 = r1 - __uint_as_float(m);
-      s[0][e] = h;
-      s[1][e] = m;
-      s[2][e] = __float_as_uint(r2);
-    }
+    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+    typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+    float r[4] = {v[0], v[1], v[2], v[3]};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      // high halves of (value 0, value 1) and (value 2, value 3): bytes 2,3 of each word
-      const uint32_t w0 = __builtin_amdgcn_perm(s[k][1], s[k][0], 0x07060302u);
-      const uint32_t w1 = __builtin_amdgcn_perm(s[k][3], s[k][2], 0x07060302u);
-      typedef uint32_t u2v __attribute__((ext_vector_type(2)));
+      const bf2 p0 = {(__bf16)r[0], (__bf16)r[1]};
+      const bf2 p1 = {(__bf16)r[2], (__bf16)r[3]};
+      const uint32_t w0 = __builtin_bit_cast(uint32_t, p0), w1 = __builtin_bit_cast(uint32_t, p1);
       const u2v w = {w0, w1};
       q[k] = __builtin_bit_cast(typename Quad<Tr>::V, w);
+      if (k < 2) {  // remainders for the next slice (exact)
+        r[0] -= __uint_as_float(w0 << 16);
+        r[1] -= __uint_as_float(w0 & 0xffff0000u);
+        r[2] -= __uint_as_float(w1 << 16);
+        r[3] -= __uint_as_float(w1 & 0xffff0000u);
+      }
     }
   } else {
 #pragma unroll

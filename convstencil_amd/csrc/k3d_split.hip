@@ -16,6 +16,9 @@
 //     kernel): level l keeps a ring of 2R+1 planes of slices, input plane p + 1 is loaded
 //     into registers while plane p is computed, and a fused launch is exact for a zero
 //     halo only (the plan fuses only then).
+#include <vector>
+
+#include "cs_wcomp.h"
 #include "k2d_split2_body.h"
 
 namespace cs {
@@ -398,7 +401,28 @@ static size_t build_split3(int R, int shape, const double *w, unsigned char *out
   const size_t bytes = (size_t)Tr::NSW * nins * 64 * 8 * sizeof(T);
   if (bytes > cap) return 0;
   T *o = reinterpret_cast<T *>(out);
-  const int D = 2 * R + 1;
+  const int D = 2 * R + 1, N = D * D * D;
+  if (N > 343) return 0;
+  // slices of every tap; BF16x3 moves its last slice by ulps so that the weights' errors
+  // cancel in the mass and the first moments (cs_wcomp.h)
+  std::vector<T> tap((size_t)3 * N);
+  for (int t = 0; t < N; ++t) {
+    T sl[3];
+    split_weight<Tr>(w[t], sl);
+    for (int q = 0; q < Tr::NSW; ++q) tap[(size_t)q * N + t] = sl[q];
+  }
+  if constexpr (Tr::NSW == Tr::NSU && Tr::NSW == 3) {
+    std::vector<double> fixed((size_t)N);
+    std::vector<int> offv((size_t)3 * N);
+    for (int t = 0; t < N; ++t) {
+      fixed[(size_t)t] = (double)(float)tap[(size_t)t] + (double)(float)tap[(size_t)N + t] / Tr::SCALE;
+      offv[(size_t)3 * t] = t / (D * D) - R;
+      offv[(size_t)3 * t + 1] = (t / D) % D - R;
+      offv[(size_t)3 * t + 2] = t % D - R;
+    }
+    compensate_low_slice<T>(N, w, reinterpret_cast<const int(*)[3]>(offv.data()), fixed.data(), &tap[(size_t)2 * N],
+                            Tr::SCALE * Tr::SCALE);
+  }
   for (int q = 0; q < Tr::NSW; ++q) {
     int i = 0;
     for (int dz = -R; dz <= R; ++dz) {
@@ -408,11 +432,10 @@ static size_t build_split3(int R, int shape, const double *w, unsigned char *out
           for (int j = 0; j < 8; ++j) {
             const int m = lane & 15, c = 4 * li + (lane >> 4);
             const int dy = ct.ty[c] - (m >> 3), dx = ct.tx[c] + j - (m & 7);
-            double v = 0.0;
-            if (ct.live[c] && dy >= -R && dy <= R && dx >= -R && dx <= R) v = w[((dz + R) * D + (dy + R)) * D + (dx + R)];
-            T sl[3];
-            split_weight<Tr>(v, sl);
-            o[((size_t)(q * nins + i) * 64 + lane) * 8 + j] = sl[q];
+            T v = (T)0.0f;
+            if (ct.live[c] && dy >= -R && dy <= R && dx >= -R && dx <= R)
+              v = tap[(size_t)q * N + ((dz + R) * D + (dy + R)) * D + (dx + R)];
+            o[((size_t)(q * nins + i) * 64 + lane) * 8 + j] = v;
           }
     }
   }

@@ -56,6 +56,12 @@ typedef enum cs_precision {
   CS_PREC_FP16W3 = 4  /* 2-D: weights in 3 FP16 slices, state in 2 (4 products: the weights' error no
                          longer biases every step), FP32 accumulate, FP64 recombine; values below 2^15 */
 } cs_precision;
+/* FP16 range guard (CS_PREC_FP16X2 / CS_PREC_FP16W3, plans and batches): the library keeps a
+ * bound on the state's magnitude -- 0 for a new plan, max |cell| (halo included) of an
+ * uploaded array, 1 after a random fill -- and multiplies it by max(1, sum |w|) per step.  A
+ * run whose bound would reach 2^15 is refused with CS_ERR_INVALID before anything is
+ * launched.  Values written into the device buffers directly (cs_plan_device_buffers) are
+ * outside the guard. */
 
 /* Padded array layout.  Dimensions are listed slowest first; a 2-D grid has
  * ext[0] = n[0] = 1, org[0] = 0 and a 1-D grid also has ext[1] = n[1] = 1.
@@ -141,6 +147,10 @@ cs_status cs_batch_destroy(cs_batch *batch);
 cs_status cs_batch_upload(cs_batch *batch, int32_t grid, const double *host_in);
 /* Seeds grid i with seeds[i] (as cs_plan_fill_random). */
 cs_status cs_batch_fill_random(cs_batch *batch, const uint64_t *seeds);
+/* The persistent launch needs every workgroup resident at once; its grid-wide barriers wait
+ * a bounded time.  A barrier that times out sets the batch's sticky error word: that run,
+ * and every later cs_batch_run / cs_batch_time / cs_batch_download of this batch, returns
+ * CS_ERR_DEVICE (the grids' contents are undefined from then on). */
 cs_status cs_batch_run(cs_batch *batch, int64_t steps, cs_report *report);
 cs_status cs_batch_time(cs_batch *batch, int64_t steps, int32_t warmup, int32_t reps, cs_report *report);
 cs_status cs_batch_download(cs_batch *batch, int32_t grid, double *host_out);

@@ -17,7 +17,9 @@ extern "C" {
 
 /* Number of compiled kernel variants of a family: 1, 2, 3 = FP64 kernels for 1-, 2-, 3-D grids;
  * 4 = 2-D and 5 = 3-D split-precision kernels (there info[1] holds the cs_precision; the 3-D
- * split kernels' z-chunk length is in their name, "zc<n>"). */
+ * split kernels' z-chunk length is in their name, "zc<n>"); 6 = the pipelined persistent 2-D
+ * FP64 kernels (two LDS tile buffers, producer waves "p<n>" in the name; info[8] = both
+ * buffers' bytes). */
 int32_t cs_debug_variant_count(int32_t ndim);
 /* info[0..11] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
  * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
@@ -84,6 +86,18 @@ int32_t cs_debug_ctable3(int32_t radius, int32_t shape, int32_t dz, int32_t *c0,
  * with three weight slices).  Returns the byte count, or 0. */
 int64_t cs_debug_bfrag_split3(int32_t precision, int32_t radius, int32_t shape, const double *w, void *out,
                               int64_t cap);
+
+/* The FP16 range guard of plans and batches (CS_PREC_FP16X2 / CS_PREC_FP16W3 need |values| < 2^15):
+ * from a bound amax on the state's magnitude and the weights' sum of magnitudes wabs, *bound
+ * receives the bound after `steps` steps (amax * max(1, wabs)^steps); returns 1 if a run of
+ * that many steps is allowed for `precision`, 0 if it is refused. */
+int32_t cs_debug_fp16_range(int32_t precision, double amax, double wabs, int64_t steps, double *bound);
+/* The guard's bound of an uploaded array: max |a[i]|, +inf if any value is not finite. */
+double cs_debug_abs_max(const double *a, int64_t n);
+/* The batch's error latch: feeds the device error word's successive values flags[0..n) to a
+ * host-only batch object and stores the status each read returns (0 = CS_OK, else
+ * CS_ERR_DEVICE: once a barrier has timed out, every later call fails).  Returns 0. */
+int32_t cs_debug_batch_latch(const uint32_t *flags, int32_t n, int32_t *status);
 
 /* Times `reps` plain device copies of `count` doubles (needs a GPU); the kernel is the
  * reference for HBM byte counters.  Returns 0 on success. */

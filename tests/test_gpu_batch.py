@@ -116,3 +116,32 @@ def test_batch_rejects_split_forms_without_bodies(gpu):
     for prec in ("fp16w3", "fp32x2"):
         with pytest.raises(RuntimeError):
             Batch([L], [1], [dense], precision=prec)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2"])
+def test_batch_split_error_after_100_steps(gpu, prec):
+    """The batched workload's own depth: 24 grids, sides log-uniform in [128, 512], stencils
+    drawn from {box2d1r, star2d3r} with their own seeded weights, T = 100 -- every grid
+    within the split bar against the oracle.  Over 100 steps a weight error that repeats
+    every step (its mass and first moments) and slices that carry the sign of the state
+    (with the level-3 products dropped) would both grow with T."""
+    rng = np.random.default_rng(77)
+    G, T = 24, 100
+    sides = np.exp(rng.uniform(np.log(128), np.log(512), (G, 2))).astype(int)
+    names = rng.choice(["box2d1r", "star2d3r"], G)
+    grids = [_grid(str(names[i]), [int(sides[i, 0]), int(sides[i, 1])], 500 + i) for i in range(G)]
+    print(f"case batch {prec} T={T} grids={G}", flush=True)
+    with Batch([g[0] for g in grids], [g[1] for g in grids], [g[4] for g in grids], precision=prec,
+               use_graph=True) as b:
+        for i, g in enumerate(grids):
+            b.upload(i, g[5])
+        b.run(T)
+        got = [b.download(i) for i in range(G)]
+    threads = oracle.max_threads()
+    worst = (0.0, 0.0)
+    for i, (L, r, offs, w, dense, x) in enumerate(grids):
+        want = oracle.run(L, offs, w, x, T, threads=threads)
+        rm, rl2 = rel_errors(L.interior(got[i]), L.interior(want))
+        worst = (max(worst[0], rm), max(worst[1], rl2))
+        assert rl2 <= 1e-6 and rm <= 5e-6, (i, names[i], sides[i], rm, rl2)
+    print(f"worst rel max {worst[0]:.2e} rel L2 {worst[1]:.2e}", flush=True)

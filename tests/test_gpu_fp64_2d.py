@@ -171,3 +171,64 @@ def test_full_size_box2d1r_windows(gpu):
     for (y0, x0) in [(0, 0), (0, n - 96), (n - 96, 0), (n - 96, n - 96), (5000, 3333), (64 * 37 + 3, 64 * 90 - 5)]:
         worst = max(worst, _window_check(L, out, x, offs, w, steps, r, y0, x0, 96))
     assert worst <= TOL, worst
+
+
+def _stream_ids():
+    import test_kernel_emulation as em
+    return [(v["id"], v["name"], v["R"], v["shape"], v["kf"], v["by"], v["bx"]) for v in em.VARIANTS_STREAM]
+
+
+@pytest.mark.parametrize("vid,vname,R,shape,kf,by,bx", _stream_ids())
+def test_every_stream_variant(gpu, vid, vname, R, shape, kf, by, bx):
+    # the pipelined persistent kernels (one workgroup per CU walking its tiles with two
+    # LDS buffers): 3 x 3 tiles (edge and interior tiles), and grids with several tiles
+    # per workgroup so that buffers are reused across interior and edge tiles
+    print(f"variant {vname}", flush=True)
+    name = {(1, 0): "box2d1r", (1, 1): "star2d1r", (3, 0): "box2d3r", (3, 1): "star2d3r"}[(R, shape)]
+    for sizes in ([3 * by - 3, 3 * bx - 5], [1100, 1301], [2011, 2100]):
+        L, R, offs, w, dense, x = setup(name, sizes, vid)
+        for steps in (kf, 2 * kf + 1):
+            with Plan(L, R, dense, variant=vid) as p:
+                p.upload(x)
+                p.run(steps)
+                assert f"variant={vid}" in p.describe()
+                got = p.download()
+            check(L, got, oracle.run(L, offs, w, x, steps, threads=0))
+
+
+@pytest.mark.parametrize("vid", [0, 811106])
+def test_full_size_box2d1r_windows_variants(gpu, vid):
+    # the measured configuration (box2d1r 10240^2, halo 1, T = 64) with the pipelined
+    # kernel forced (0: the plan's default choice)
+    n, steps = 10240, 64
+    L, r, offs, w, dense, _ = setup("box2d1r", [8, 8], 1234)  # weights only
+    L = Layout.with_halo([n, n], 1)
+    x = fill_random_host(L, 1234)
+    with Plan(L, r, dense, use_graph=True, variant=vid) as p:
+        p.fill_random(1234)
+        p.run(steps)
+        out = p.download()
+    worst = 0.0
+    for (y0, x0) in [(0, 0), (0, n - 96), (n - 96, 0), (n - 96, n - 96), (5000, 3333), (64 * 37 + 3, 64 * 90 - 5)]:
+        worst = max(worst, _window_check(L, out, x, offs, w, steps, r, y0, x0, 96))
+    assert worst <= TOL, worst
+
+
+@pytest.mark.parametrize("name", ["star2d3r", "box2d3r"])
+def test_full_size_radius3_windows(gpu, name):
+    # the radius-3 configuration: 8192^2, halo 3, T = 64, FP64 (autotuned plan, as measured),
+    # checked on windows at the corners, edges and inside whose 192-cell light cone the oracle
+    # recomputes exactly
+    n, steps = 8192, 64
+    L, r, offs, w, dense, _ = setup(name, [8, 8], 4321)
+    L = Layout.with_halo([n, n], 3)
+    x = fill_random_host(L, 4321)
+    with Plan(L, r, dense, use_graph=True, autotune=1) as p:
+        p.fill_random(4321)
+        p.run(steps)
+        out = p.download()
+    worst = 0.0
+    for (y0, x0) in [(0, 0), (n - 64, n - 64), (0, 4000), (4100, 2050)]:
+        worst = max(worst, _window_check(L, out, x, offs, w, steps, r, y0, x0, 64))
+    print(f"{name} 8192^2 T=64 worst window rel max {worst:.2e}", flush=True)
+    assert worst <= TOL, worst

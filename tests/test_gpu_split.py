@@ -49,10 +49,11 @@ def test_split_error(gpu, prec, name, sizes, steps):
 # in every step; rounded plainly they took FP16x2 to 1.17e-6.  The library's FP16x2
 # weights are compensated (their errors cancel in the sum and the first moments of the
 # taps), and FP16 with three weight slices removes the bias outright; both meet 1e-6.
-BOUND_64 = {"bf16x3": 1e-6, "fp16x2": 1e-6, "fp16w3": 1e-6}
+# FP32x2 (the hi/lo form, three FP32 products) carries 48 bits and meets it by far.
+BOUND_64 = {"bf16x3": 1e-6, "fp16x2": 1e-6, "fp16w3": 1e-6, "fp32x2": 1e-6}
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2", "fp16w3"])
+@pytest.mark.parametrize("prec", ["bf16x3", "fp16x2", "fp16w3", "fp32x2"])
 def test_box2d3r_64_steps(gpu, prec):
     # box2d3r, halo 3, T = 64 on a 1024^2 grid, full oracle comparison
     L, r, offs, w, dense, _ = _setup("box2d3r", [8, 8], 64)

@@ -101,6 +101,34 @@ def test_f64_capacity_checked(tmp_path):
     assert rc == _lib.CS_ERR_INVALID
 
 
+@pytest.mark.parametrize("ext,org,n", [
+    # 2^32 * 2^32 wraps a 64-bit product to 0; with ext[0] = 2 it would wrap to a small positive count
+    ([2, 1 << 32, 1 << 32], [0, 0, 0], [2, 4, 4]),
+    ([1, 1 << 31, (1 << 33) + 1], [0, 0, 0], [1, 2, 2]),
+    # cell count fits but its byte count does not
+    ([1, 1 << 31, 1 << 31], [0, 0, 0], [1, 2, 2]),
+    # org + n overflows int64
+    ([1, 4, 8], [0, 0, (1 << 63) - 1], [1, 2, 2]),
+])
+def test_f64_header_with_overflowing_extents_is_refused(tmp_path, ext, org, n):
+    """A crafted header whose extents overflow the cell or byte count is refused before any
+    read (a wrapped product could otherwise pass the capacity check)."""
+    import ctypes
+    import struct
+    from convstencil_amd import _lib
+    hdr = b"CSGRID64" + struct.pack("<II", 1, 3) + struct.pack("<9q", *ext, *org, *n) + bytes(8)
+    assert len(hdr) == 96
+    (tmp_path / "o.f64").write_bytes(hdr + bytes(8 * 16))
+    lib = _lib.load()
+    cl = _lib.cs_layout()
+    buf = np.zeros(16)
+    rc = lib.cs_grid_load_f64(os.fsencode(tmp_path / "o.f64"), ctypes.byref(cl),
+                              buf.ctypes.data_as(_lib.c_double_p), buf.size)
+    assert rc == _lib.CS_ERR_INVALID
+    with pytest.raises(ConvStencilError):
+        cs.load_f64(tmp_path / "o.f64")
+
+
 # ---------------------------------------------------------------- command lines (GPU)
 
 @pytest.mark.gpu
