@@ -60,6 +60,7 @@ int32_t cs_debug_dtable(int32_t radius, int32_t shape, int32_t *hdr, int32_t *ty
   hdr[0] = t.np;
   hdr[1] = t.ns;
   hdr[2] = t.pp;
+  hdr[3] = t.map;
   for (int ps = 0; ps < t.np; ++ps)
     for (int q = 0; q < 4; ++q) {
       ty[4 * ps + q] = t.pty[ps][q];

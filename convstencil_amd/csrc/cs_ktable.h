@@ -199,6 +199,7 @@ namespace cs {
 // each) come first, single k-steps last, four entries (one per k lane group) per
 // step; unused slots repeat a live address with zero weight (live = 0).
 struct DTable {
+  int map;             // lane -> cell map of the blocks (dil_row / dil_col): 0 rows, 1 skew
   int np;              // pair steps
   int ns;              // single k-steps
   int pp;              // parity of every pair's first column t.x
@@ -207,6 +208,19 @@ struct DTable {
   int tmin, tmax;      // rows t.y read, relative to the block row 2b
   int cmin, cmax;      // columns t.x read (a pair's second column included), relative to 2m
 };
+
+// Where the lanes of a v_mfma_f64_4x4x4 instruction sit in an 8 x 8 output block.  Lane
+// l holds A_b[l & 3][l >> 4] and D_b[l >> 4][l & 3] of block b = (l >> 2) & 3; the pair
+// (block b, A/D row i) owns one 2 x 2 sub-block, at row pair dil_row(b, i) and column pair
+// dil_col(b, i) (any bijection of (b, i) onto the 4 x 4 sub-blocks computes the same
+// stencil: the B fragments, the same for every block, only depend on the 2 x 2 offset of
+// D's column index).  Map 0 puts block b on row pair b and row i on column pair i (the
+// 3-D kernels); map 1 puts row i on row pair i and skews the column pair by the block,
+// (b + i) mod 4, which together with the order of each step's taps over the four lane
+// groups (order_dtable) makes both the 16-byte A reads and the 8-byte result writes free
+// of LDS bank conflicts at pitches of 8 (mod 16) doubles (lds_cost below).
+constexpr int dil_row(int map, int b, int i) { return map == 1 ? i : b; }
+constexpr int dil_col(int map, int b, int i) { return map == 1 ? ((b + i) & 3) : i; }
 
 // Row spans of a 2-D tap set of radius R: taps (dy, dx) with dx in [lo[dy + R], hi[dy + R]]
 // (lo > hi: no taps in that row).  The dilated tables below are built from these.
@@ -343,7 +357,75 @@ constexpr DTable make_dtable_spans(const Spans &sp, int pp_force) {
   return best;
 }
 
-constexpr DTable make_dtable(int R, int shape) { return make_dtable_spans(spans_2d(R, shape), -1); }
+constexpr int lds_cost(const int (&addr)[64], int bytes, bool write);
+
+// Model cost of the A reads of one step of table t (taps of lane group q = order[q]) at pitch
+// px under t's map.
+constexpr int dstep_read_cost(const DTable &t, int step, const int (&order)[4], int px) {
+  const bool pair = step < t.np;
+  int a[64] = {};
+  for (int l = 0; l < 64; ++l) {
+    const int q = order[l >> 4], b = (l >> 2) & 3, i = l & 3;
+    const int ty = pair ? t.pty[step][q] : t.sty[step - t.np][q];
+    const int tx = pair ? t.ptx[step][q] : t.stx[step - t.np][q];
+    a[l] = ((2 * dil_row(t.map, b, i) + ty + 16) * px + 2 * dil_col(t.map, b, i) + tx + 16) * 8;
+  }
+  return lds_cost(a, pair ? 16 : 8, false);
+}
+
+// Puts each step's four taps on the lane groups in the order with the cheapest reads at
+// the pitches the skewed map favours (8 mod 16 doubles; two samples); ties keep the order
+// of construction.  Only the order within a step changes, so k-steps, reads and the
+// footprint stay those of the table.
+constexpr DTable order_dtable(DTable t) {
+  for (int st = 0; st < t.np + t.ns; ++st) {
+    const bool pair = st < t.np;
+    int best[4] = {0, 1, 2, 3}, best_c = 1 << 30;
+    for (int p = 0; p < 24; ++p) {
+      // permutation p of {0, 1, 2, 3} (Lehmer code)
+      int avail[4] = {0, 1, 2, 3}, ord[4] = {}, k = p, n = 4;
+      for (int j = 0; j < 4; ++j) {
+        int f = 1;
+        for (int x = 2; x < n; ++x) f *= x;
+        const int idx = k / f;
+        k %= f;
+        ord[j] = avail[idx];
+        for (int x = idx; x < n - 1; ++x) avail[x] = avail[x + 1];
+        --n;
+      }
+      const int c = dstep_read_cost(t, st, ord, 72) + dstep_read_cost(t, st, ord, 88);
+      if (c < best_c) {
+        best_c = c;
+        for (int j = 0; j < 4; ++j) best[j] = ord[j];
+      }
+    }
+    int y[4] = {}, x[4] = {}, lv[4] = {};
+    for (int j = 0; j < 4; ++j) {
+      y[j] = pair ? t.pty[st][best[j]] : t.sty[st - t.np][best[j]];
+      x[j] = pair ? t.ptx[st][best[j]] : t.stx[st - t.np][best[j]];
+      lv[j] = pair ? t.plive[st][best[j]] : t.slive[st - t.np][best[j]];
+    }
+    for (int j = 0; j < 4; ++j) {
+      if (pair) {
+        t.pty[st][j] = y[j];
+        t.ptx[st][j] = x[j];
+        t.plive[st][j] = lv[j];
+      } else {
+        t.sty[st - t.np][j] = y[j];
+        t.stx[st - t.np][j] = x[j];
+        t.slive[st - t.np][j] = lv[j];
+      }
+    }
+  }
+  return t;
+}
+
+// 2-D tables: the skewed map, taps ordered for it.
+constexpr DTable make_dtable(int R, int shape) {
+  DTable t = make_dtable_spans(spans_2d(R, shape), -1);
+  t.map = 1;
+  return order_dtable(t);
+}
 
 // Number of k-steps of a dilated table.
 constexpr int dtable_ksteps(const DTable &t) { return 2 * t.np + t.ns; }
@@ -600,18 +682,11 @@ constexpr int lds_wgs(long bytes) { return bytes > 0 ? (int)(163840 / bytes) : 6
 constexpr int dtable_lds_cost(const DTable &t, int px) {
   int cost = 0;
   int a[64] = {};
-  for (int ps = 0; ps < t.np + t.ns; ++ps) {
-    const bool pair = ps < t.np;
-    for (int l = 0; l < 64; ++l) {
-      const int q = l >> 4, b4 = (l >> 2) & 3, m = l & 3;
-      const int ty = pair ? t.pty[ps][q] : t.sty[ps - t.np][q], tx = pair ? t.ptx[ps][q] : t.stx[ps - t.np][q];
-      a[l] = ((2 * b4 + ty + 16) * px + 2 * m + tx + 16) * 8;
-    }
-    cost += lds_cost(a, pair ? 16 : 8, false);
-  }
-  for (int l = 0; l < 64; ++l) {
-    const int b4 = (l >> 2) & 3, n = l & 3, m = l >> 4;
-    a[l] = ((2 * b4 + (n >> 1)) * px + 2 * m + (n & 1)) * 8;
+  const int ident[4] = {0, 1, 2, 3};
+  for (int ps = 0; ps < t.np + t.ns; ++ps) cost += dstep_read_cost(t, ps, ident, px);
+  for (int l = 0; l < 64; ++l) {  // D_b[i][n], i = l >> 4: cell (2 row + (n >> 1), 2 col + (n & 1))
+    const int b = (l >> 2) & 3, n = l & 3, i = l >> 4;
+    a[l] = ((2 * dil_row(t.map, b, i) + (n >> 1)) * px + 2 * dil_col(t.map, b, i) + (n & 1)) * 8;
   }
   return cost + lds_cost(a, 8, true);
 }

@@ -518,17 +518,23 @@ __device__ __forceinline__ void lane_setup(const double *__restrict__ bfrag, int
     t.ocol = 0;
   } else {
     // off[i]: byte offset of read i's cell for this lane from the block origin, set for
-    // the representative reads of the read plan only (the others are immediates)
-    const int q = lane >> 4, b4 = (lane >> 2) & 3, m = lane & 3;
+    // the representative reads of the read plan only (the others are immediates).  Lane l
+    // feeds A_b[l & 3][l >> 4] of block b = (l >> 2) & 3: tap group q = l >> 4 of the 2 x 2
+    // sub-block at (dil_row, dil_col)(b, l & 3) of the 8 x 8 block (cs_ktable.h)
+    const int q = lane >> 4, b = (lane >> 2) & 3, ai = lane & 3;
+    const int arow = 2 * dil_row(C::DT.map, b, ai), acol = 2 * dil_col(C::DT.map, b, ai);
 #pragma unroll
     for (int i = 0; i < C::RPL.n; ++i) {
       if (C::RPL.rep[i] != i) continue;
       const int ty = i < C::DT.np ? C::DT.pty[i][q] : C::DT.sty[i - C::DT.np][q];
       const int tx = i < C::DT.np ? C::DT.ptx[i][q] : C::DT.stx[i - C::DT.np][q];
-      off[i] = ((2 * b4 + ty) * PX + 2 * m + tx + C::CS) * 8;
+      off[i] = ((arow + ty) * PX + acol + tx + C::CS) * 8;
     }
-    t.orow = 2 * b4 + ((lane & 3) >> 1);
-    t.ocol = 2 * (lane >> 4) + (lane & 1);
+    // the lane's result D_b[l >> 4][n], n = l & 3: cell (2 row + (n >> 1), 2 col + (n & 1))
+    // of the sub-block of (b, l >> 4)
+    const int di = lane >> 4, n = lane & 3;
+    t.orow = 2 * dil_row(C::DT.map, b, di) + (n >> 1);
+    t.ocol = 2 * dil_col(C::DT.map, b, di) + (n & 1);
     t.vw = (t.orow * PX + t.ocol + C::CS) * 8;
   }
 }

@@ -26,26 +26,22 @@ namespace cs {
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(1))) void glb_void_t;
 
-// Tile (EY x EX cells from (ay0, ax0)) into LDS rows of pitch PX, columns from CS on:
-// each row is ceil(2 EX / 64) 256-byte chunks, one global_load_lds_dword per chunk (lane
-// i copies dword i of the chunk), rows dealt round-robin to the NP producer waves.
-// Nothing here waits for the copies; the caller waits for vmcnt(0).  A tile at the domain
-// edge first gets zeros in every cell (ds_write, completed before any copy is issued),
-// then copies only the cells inside the array: lanes and rows outside are masked off.
+// Rows [r0, r1) of a tile (EY x EX cells from (ay0, ax0)) into LDS rows of pitch PX,
+// columns from CS on: each row is ceil(2 EX / 64) 256-byte chunks, one
+// global_load_lds_dword per chunk (lane i copies dword i of the chunk), rows dealt
+// round-robin to the NP producer waves.  Nothing here waits for the copies; the caller
+// waits for vmcnt(0).  In a tile at the domain edge only the cells inside the array are
+// copied (rows and lanes outside are masked off); the rest must have been zeroed first.
 template <class C, int NP>
-__device__ __forceinline__ void dma_tile(const double *__restrict__ in, const Geo &g, const TileCtx &t, double *buf,
-                                         int pw, int lane, int ptid) {
+__device__ __forceinline__ void dma_rows(const double *__restrict__ in, const Geo &g, const TileCtx &t, double *buf,
+                                         int pw, int lane, int r0, int r1) {
   constexpr int DW = 2 * C::EX;            // dwords per row
   constexpr int CH = (DW + 63) / 64;       // chunks per row
   constexpr int TAIL = DW - 64 * (CH - 1);  // active lanes of the last chunk
-  if (!t.full) {
-    for (int i = ptid; i < C::BUF; i += NP * 64) buf[i] = 0.0;
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-  }
   const char *g0 = reinterpret_cast<const char *>(in + t.ay0 * t.pitch + t.ax0) + lane * 4;
   char *l0 = reinterpret_cast<char *>(buf + C::CS);
   const int64_t rstep = t.pitch * 8;
-  for (int r = pw; r < C::EY; r += NP) {
+  for (int r = r0 + pw; r < r1; r += NP) {
     const int64_t ay = t.ay0 + r;
     if (!t.full && (ay < 0 || ay >= g.ext[1])) continue;  // uniform over the wave
     const char *gr = g0 + r * rstep;
@@ -62,10 +58,12 @@ __device__ __forceinline__ void dma_tile(const double *__restrict__ in, const Ge
   }
 }
 
-template <int N>
-__device__ __forceinline__ void pass_barriers() {
-#pragma unroll
-  for (int i = 0; i < N; ++i) __builtin_amdgcn_s_barrier();
+// A tile at the domain edge starts from zeros (ds_write, completed before any copy).
+template <class C, int NP>
+__device__ __forceinline__ void zero_tile(const TileCtx &t, double *buf, int ptid) {
+  if (t.full) return;
+  for (int i = ptid; i < C::BUF; i += NP * 64) buf[i] = 0.0;
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 }
 
 // Position k (step G) of the XCD-ordered tile walk, advanced to the next existing tile.
@@ -81,6 +79,10 @@ __global__ __launch_bounds__((NW + NP) * 64) void k2d_stream(const double *__res
   static_assert(C::NBUF == 1, "one LDS buffer per tile");
   constexpr int NBAR = ST == 2 ? 2 * KF : 2 * (KF - 1);  // barriers inside one tile's steps
   constexpr int SLOT = C::BUF + 8;                        // doubles per tile buffer (+ spare cells)
+  // the next tile is issued in NPH slices of RPH rows, one before each of the first NPH
+  // step barriers (the last two steps' barriers get none, so the copies land early)
+  constexpr int NPH = NBAR > 2 ? NBAR - 2 : 1;
+  constexpr int RPH = (C::EY + NPH - 1) / NPH;
   extern __shared__ __align__(16) double lds[];
   const int64_t ntiles = g.tiles[1] * g.tiles[2];
   const int64_t total = ((ntiles + 7) / 8) * 8, G = gridDim.x;
@@ -101,18 +103,28 @@ __global__ __launch_bounds__((NW + NP) * 64) void k2d_stream(const double *__res
     t.orow = t.ocol = 0;
     {
       tile_setup<C>(g, xcd_tile(k, ntiles), t, out);
-      dma_tile<C, NP>(in, g, t, lds, pw, lane, ptid);
+      zero_tile<C, NP>(t, lds, ptid);
+      dma_rows<C, NP>(in, g, t, lds, pw, lane, 0, C::EY);
       asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
       __builtin_amdgcn_s_barrier();  // tile k is in buffer 0
     }
     for (;;) {
       const int64_t kn = stream_next(k + G, G, total, ntiles);
       double *nb = lds + (cur ^ 1) * SLOT;
-      if (kn < total) {
+      const bool more = kn < total;
+      if (more) {
         tile_setup<C>(g, xcd_tile(kn, ntiles), t, out);
-        dma_tile<C, NP>(in, g, t, nb, pw, lane, ptid);
+        zero_tile<C, NP>(t, nb, ptid);
       }
-      pass_barriers<NBAR>();  // the consumers' fused steps of tile k
+      // the next tile's rows, a slice before each of the consumers' step barriers of tile k
+      // (a wave may have at most 63 memory instructions in flight: issuing a whole tile at
+      // once would stall the producers, and with them the consumers at their first barrier)
+#pragma unroll
+      for (int ph = 0; ph < NBAR; ++ph) {
+        if (more && ph < NPH) dma_rows<C, NP>(in, g, t, nb, pw, lane, ph * RPH, min(C::EY, (ph + 1) * RPH));
+        __builtin_amdgcn_s_barrier();
+      }
+      if (more && NBAR < NPH) dma_rows<C, NP>(in, g, t, nb, pw, lane, NBAR * RPH, C::EY);
       asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)" ::: "memory");
       __builtin_amdgcn_s_barrier();  // end of tile k: tile kn is in the other buffer
       if (kn >= total) break;
@@ -202,12 +214,13 @@ static hipError_t launch_stream(const double *in, double *out, const double *bfr
 
 // Variant ids 800000 + R * 1000 + (SH == star) * 10 + KF % 10 + 100 * (tile code): tile code 1
 // = 64 x 64 with 8 consumer waves, 2 = 96 x 64 / 8, 3 = 64 x 64 / 12 (output from registers),
-// 4 = 80 x 80 / 10.
+// 4 = 80 x 80 / 10; two producer waves, or four for codes 5 (64 x 64 / 8) and 6 (96 x 64 / 8).
 static const Variant kStreamVariants[] = {
     CS_V2DS(811106, 1, SHAPE_BOX, 6, 64, 64, 8, 2, 2),   CS_V2DS(811104, 1, SHAPE_BOX, 4, 64, 64, 8, 2, 2),
     CS_V2DS(811108, 1, SHAPE_BOX, 8, 64, 64, 8, 2, 2),   CS_V2DS(811206, 1, SHAPE_BOX, 6, 96, 64, 8, 2, 2),
     CS_V2DS(811306, 1, SHAPE_BOX, 6, 64, 64, 12, 2, 1),  CS_V2DS(811406, 1, SHAPE_BOX, 6, 80, 80, 10, 2, 2),
     CS_V2DS(811116, 1, SHAPE_STAR, 6, 64, 64, 8, 2, 2),  CS_V2DS(813103, 3, SHAPE_BOX, 3, 64, 64, 8, 2, 2),
+    CS_V2DS(811506, 1, SHAPE_BOX, 6, 64, 64, 8, 4, 2),   CS_V2DS(811606, 1, SHAPE_BOX, 6, 96, 64, 8, 4, 2),
     CS_V2DS(813113, 3, SHAPE_STAR, 3, 64, 64, 8, 2, 2),
 };
 

@@ -31,10 +31,13 @@ int32_t cs_debug_variant_info(int32_t ndim, int32_t i, int64_t *info, char *name
  * layout (mf = 4): dy[], c0[], live[] (cap entries each); returns the table length, or -1. */
 int32_t cs_debug_ktable(int32_t mf, int32_t radius, int32_t shape, int32_t *dy, int32_t *c0, int32_t *live,
                         int32_t cap);
-/* Dilated 4x4x4 table of the 2-D FP64 kernels (8 x 8 output blocks): hdr = {pair steps np, single
- * steps ns, pair column parity}; entry e = 4 * step + q (pair steps first, then single steps) is the
- * footprint cell (ty[e], tx[e]) read by lane group q (a pair also reads tx + 1), live[e] = 0 for a
- * zero-weight slot.  Returns 4 * (np + ns), or -1 if cap is too small. */
+/* Dilated 4x4x4 table of the 2-D FP64 kernels (8 x 8 output blocks): hdr[4] = {pair steps np, single
+ * steps ns, pair column parity, lane map}; entry e = 4 * step + q (pair steps first, then single
+ * steps) is the footprint cell (ty[e], tx[e]) read by lane group q (a pair also reads tx + 1),
+ * live[e] = 0 for a zero-weight slot.  Lane map 1: lane l (block b = (l >> 2) & 3, A row i = l & 3)
+ * reads at row 2 i + ty, column 2 ((b + i) mod 4) + tx of the block, and holds the result of D row
+ * l >> 4 at the same kind of sub-block; map 0: row 2 b, column 2 i.  Returns 4 * (np + ns), or -1
+ * if cap is too small. */
 int32_t cs_debug_dtable(int32_t radius, int32_t shape, int32_t *hdr, int32_t *ty, int32_t *tx, int32_t *live,
                         int32_t cap);
 /* B fragments bfrag[ks * 64 + lane] for dense weights w[(2r+1)^2]; returns the k-step count,

@@ -76,18 +76,27 @@ def bfrag(mf, R, shape, w):
 
 
 def dtable(R, shape):
-    """Dilated 4x4x4 table: (np, ns, pp, ty[e], tx[e], live[e]) with e = 4 * step + q."""
+    """Dilated 4x4x4 table: (np, ns, pp, ty[e], tx[e], live[e], map) with e = 4 * step + q."""
     lib = _dbg()
     ip = ctypes.POINTER(ctypes.c_int32)
     lib.cs_debug_dtable.restype = ctypes.c_int32
     lib.cs_debug_dtable.argtypes = [ctypes.c_int32, ctypes.c_int32, ip, ip, ip, ip, ctypes.c_int32]
     cap = 256
-    hdr = (ctypes.c_int32 * 3)()
+    hdr = (ctypes.c_int32 * 4)()
     ty, tx, live = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
     n = lib.cs_debug_dtable(R, shape, hdr, ty, tx, live, cap)
     assert n > 0 and n == 4 * (hdr[0] + hdr[1])
     return hdr[0], hdr[1], hdr[2], np.array(ty[:n]).reshape(-1, 4), np.array(tx[:n]).reshape(-1, 4), \
-        np.array(live[:n]).reshape(-1, 4)
+        np.array(live[:n]).reshape(-1, 4), hdr[3]
+
+
+def dil_row(dmap, b, i):
+    """Row pair of the 2 x 2 sub-block of (block b, A/D row i) in an 8 x 8 block."""
+    return i if dmap == 1 else b
+
+
+def dil_col(dmap, b, i):
+    return (b + i) & 3 if dmap == 1 else i
 
 
 def lane_offsets(v, tab):
@@ -97,12 +106,14 @@ def lane_offsets(v, tab):
     if v["mf"] == 16:
         dy, c0, live = tab
         return np.array([((lanes & 15) + dy[ks]) * px + c0[ks] + (lanes >> 4) for ks in range(len(dy))])
-    npair, ns, pp, ty, tx, live = tab
-    q, b4, m = lanes >> 4, (lanes >> 2) & 3, lanes & 3
-    return np.array([(2 * b4 + ty[e][q]) * px + 2 * m + tx[e][q] + v["cs"] for e in range(npair + ns)])
+    npair, ns, pp, ty, tx, live = tab[:6]
+    dmap = tab[6] if len(tab) > 6 else 0
+    q, b, i = lanes >> 4, (lanes >> 2) & 3, lanes & 3
+    return np.array([(2 * dil_row(dmap, b, i) + ty[e][q]) * px + 2 * dil_col(dmap, b, i) + tx[e][q] + v["cs"]
+                     for e in range(npair + ns)])
 
 
-def mfma_block(v, a_lanes, bf):
+def mfma_block(v, a_lanes, bf, dmap=0):
     """a_lanes[ks][lane], bf[ks][lane] -> (rows, cols, vals) of the block's outputs per lane."""
     lanes = np.arange(64)
     if v["mf"] == 16:
@@ -126,10 +137,11 @@ def mfma_block(v, a_lanes, bf):
         A[b4, lanes & 3, lanes >> 4] = a_lanes[ks]
         B[b4, lanes >> 4, lanes & 3] = bf[ks]
         acc += A @ B
-    # lane l = 16 m + 4 b + n holds D_b[m][n] = cell (2b + (n >> 1), 2m + (n & 1))
+    # lane l = 16 m + 4 b + n holds D_b[m][n]: cell (2 row + (n >> 1), 2 col + (n & 1)) of the
+    # 2 x 2 sub-block (row, col) = (dil_row, dil_col)(b, m)
     n = lanes & 3
-    rows = (2 * b4 + (n >> 1))[:, None]
-    cols = (2 * (lanes >> 4) + (n & 1))[:, None]
+    rows = (2 * dil_row(dmap, b4, lanes >> 4) + (n >> 1))[:, None]
+    cols = (2 * dil_col(dmap, b4, lanes >> 4) + (n & 1))[:, None]
     vals = acc[b4, lanes >> 4, n][:, None]
     return rows, cols, vals
 
@@ -218,7 +230,7 @@ def _tile_steps(v, tab, bf, offs, layout, lds, tile, out):
             # every LDS row read stays inside the row (no wrap into the next row)
             a = a_operands(v, tab, lds, idx)
             assert np.isfinite(a).all(), (v["name"], "reads an unwritten LDS cell")
-            rows, cols, vals = mfma_block(v, a, bf)
+            rows, cols, vals = mfma_block(v, a, bf, tab[6] if v["mf"] == 4 else 0)
             ty, tx, vv = y + rows.reshape(-1), xx + cols.reshape(-1), vals.reshape(-1)
             inside = np.ones(ty.shape, dtype=bool) if full else \
                 (Y0 + ty >= 0) & (Y0 + ty < n[1]) & (X0 + tx >= 0) & (X0 + tx < n[2])
@@ -331,7 +343,7 @@ def test_dtables_cover_every_tap():
     # k-slots of the dilated table: slot t serves output v_n through tap t - v_n
     for R in (1, 2, 3):
         for shape in (0, 1):
-            npair, ns, pp, ty, tx, live = dtable(R, shape)
+            npair, ns, pp, ty, tx, live, dmap = dtable(R, shape)
             slots = []
             for e in range(4 * npair):
                 if live[e // 4][e % 4]:
