@@ -69,6 +69,9 @@ SIGNATURES = {
     "cs_tune_store": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_double]),
     "cs_tune_key": (ctypes.c_int, [ctypes.POINTER(cs_layout), ctypes.POINTER(cs_stencil), ctypes.POINTER(cs_options),
                                    ctypes.c_char_p, ctypes.c_int32]),
+    "cs_run_2d_noalloc": (ctypes.c_int, [ctypes.POINTER(cs_layout), ctypes.POINTER(cs_stencil), ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(cs_options), ctypes.c_void_p,
+                                         ctypes.POINTER(ctypes.c_int32)]),
     "cs_fill_random_host": (None, [c_double_p, ctypes.POINTER(cs_layout), ctypes.c_uint64]),
     "cs_fill_weights_host": (None, [c_double_p, ctypes.c_int32, ctypes.c_uint64]),
     "cs_grid_save_txt": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(cs_layout), c_double_p]),

@@ -132,6 +132,28 @@ int max_kf(int ndim, int mf, int R, int shape, int prec = CS_PREC_FP64, int st =
   return m;
 }
 
+// The main variant of a plan: a forced id, else the preferred family's default depth.
+const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, int tile_variant) {
+  const int mf = prec != CS_PREC_FP64 ? prec : 4;
+  const Variant *v = nullptr;
+  if (tile_variant > 0) {
+    // a forced variant of the stencil's shape class, or a box variant (it computes any
+    // stencil of its radius: the taps outside the stencil get zero weights)
+    v = find_variant(nd, 0, R, shape, 0, tile_variant, prec);
+    if (!v) v = find_variant(nd, 0, R, cs::SHAPE_BOX, 0, tile_variant, prec);
+  } else if (prec == CS_PREC_FP64) {
+    // 3-D: the 8 x 8 block family (mf 8) where it exists, else the register-blocked FMA
+    // kernels (mf 3, radius 2-3), else the 4 x 16 strips
+    for (int fmf : {nd == 3 ? 8 : mf, nd == 3 ? 3 : mf, mf}) {
+      const int top = max_kf(nd, fmf, R, shape, prec);
+      int kf = fusion_depth > 0 ? std::min(fusion_depth, top) : default_kf(nd, R, top, shape, true);
+      for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
+      if (v) break;
+    }
+  }
+  return v;
+}
+
 cs_status select_variants(cs_plan *p) {
   const int nd = p->ndim;
   const int prec = p->opt.precision;
@@ -144,21 +166,10 @@ cs_status select_variants(cs_plan *p) {
   const int mf = prec != CS_PREC_FP64 ? prec : 4;
   const Variant *v = nullptr;
   if (p->opt.tile_variant > 0) {
-    // a forced variant of the stencil's shape class, or a box variant (it computes any
-    // stencil of its radius: the taps outside the stencil get zero weights)
-    v = find_variant(nd, 0, p->R, p->shape, 0, p->opt.tile_variant, prec);
-    if (!v) v = find_variant(nd, 0, p->R, cs::SHAPE_BOX, 0, p->opt.tile_variant, prec);
+    v = pick_main(nd, p->R, p->shape, prec, p->opt.fusion_depth, p->opt.tile_variant);
     if (!v) return CS_ERR_INVALID;
   } else if (prec == CS_PREC_FP64) {
-    // 3-D: the 8 x 8 block family (mf 8) where it exists, else the register-blocked FMA
-    // kernels (mf 3, radius 2-3), else the 4 x 16 strips
-    for (int fmf : {nd == 3 ? 8 : mf, nd == 3 ? 3 : mf, mf}) {
-      const int top = max_kf(nd, fmf, p->R, p->shape, prec);
-      int kf = p->opt.fusion_depth > 0 ? std::min(p->opt.fusion_depth, top)
-                                       : default_kf(nd, p->R, top, p->shape, true);
-      for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, p->R, p->shape, kf, 0, prec);
-      if (v) break;
-    }
+    v = pick_main(nd, p->R, p->shape, prec, p->opt.fusion_depth, 0);
   } else {
     const int tries[3][2] = {{p->shape, 2}, {cs::SHAPE_BOX, 2}, {cs::SHAPE_BOX, 0}};
     for (const auto &tr : tries) {
@@ -463,6 +474,26 @@ cs_status autotune_plan(cs_plan *p, int classified) {
 }
 
 }  // namespace
+
+namespace cs {
+
+// The FP64 kernel choice of a plan, made without a plan (nothing allocated, no device
+// touched): the main variant, the single-step variant and one variant per remainder depth
+// 1 .. kf - 1 (vrem[r], null where the family has none), as select_variants picks them.
+cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int tile_variant, const Variant **vmain,
+                             const Variant **vone, const Variant **vrem, int vrem_cap) {
+  const Variant *v = pick_main(ndim, R, shape, CS_PREC_FP64, fusion_depth, tile_variant);
+  if (!v) return tile_variant > 0 ? CS_ERR_INVALID : CS_ERR_UNSUPPORTED;
+  if (v->kf > vrem_cap) return CS_ERR_UNSUPPORTED;
+  *vmain = v;
+  *vone = v->kf == 1 ? v : find_variant(ndim, v->mf, R, v->shape, 1, 0, CS_PREC_FP64);
+  if (!*vone) return CS_ERR_UNSUPPORTED;
+  for (int r = 0; r < vrem_cap; ++r)
+    vrem[r] = r >= 1 && r < v->kf ? find_variant(ndim, v->mf, R, v->shape, r, 0, CS_PREC_FP64) : nullptr;
+  return CS_OK;
+}
+
+}  // namespace cs
 
 extern "C" {
 

@@ -173,6 +173,23 @@ cs_status cs_tune_store(const char *key, int32_t variant, double ms_per_step);
 cs_status cs_tune_key(const cs_layout *layout, const cs_stencil *stencil, const cs_options *opt, char *buf,
                       int32_t len);
 
+/* ---- allocation-free device run ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): advances
+ * a 2-D FP64 grid held in two caller-owned device buffers by `steps` steps, ping-ponging
+ * between them on `stream` (a hipStream_t; NULL = the default stream), with the plan's
+ * kernels and default fusion depth (opt: fusion_depth and tile_variant as for a plan;
+ * precision FP64 only, use_graph and autotune 0).  buf0 holds the initial state; both
+ * buffers must carry the same halo (fused launches read only their input's halo), else
+ * pass fusion_depth 1 for the reference's alternating-halo single steps.  *final_buffer
+ * receives 0 or 1, the buffer that holds the result.  Nothing is allocated: the kernels'
+ * B fragments (radius <= 3: at most 16 x 64 doubles) live in a fixed device array of the
+ * library, one per GPU, that every call on that GPU shares; calls that overlap on one GPU
+ * race on it.  Larger radii return CS_ERR_UNSUPPORTED.  The call only enqueues kernels:
+ * synchronise the stream before reading the result. */
+cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil *stencil, double *buf0, double *buf1,
+                            int64_t steps, const cs_options *opt, void *stream, int32_t *final_buffer);
+
 /* ---- one-shot host-to-host run (what the reference launchers do, minus printing) ---- */
 cs_status cs_run(const cs_layout *layout, const cs_stencil *stencil, const double *host_in, double *host_out,
                  int64_t steps, const cs_options *opt, cs_report *report);

@@ -30,7 +30,7 @@ DEBUG = _declared("convstencil_debug.h")
 
 
 def test_headers_parse():
-    assert "cs_plan_create" in PUBLIC and "gpu_box_2d1r" in PUBLIC
+    assert "cs_plan_create" in PUBLIC and "gpu_box_2d1r" in PUBLIC and "cs_run_2d_noalloc" in PUBLIC
     assert "cs_debug_variant_info" in DEBUG
     assert len(PUBLIC) >= 40
 
