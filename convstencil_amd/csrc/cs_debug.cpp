@@ -150,6 +150,8 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
   return t.n;
 }
 
+int32_t cs_debug_ctable_map(int32_t radius, int32_t shape) { return cs::make_ctable(radius, shape).map; }
+
 int32_t cs_debug_ctable3(int32_t radius, int32_t shape, int32_t dz, int32_t *c0, int32_t *ty, int32_t *tx,
                          int32_t *live, int32_t cap) {
   if (radius < 1 || radius > 3 || dz < -radius || dz > radius || !c0) return -1;

@@ -494,6 +494,7 @@ namespace cs {
 // block columns satisfy x + CS + c0 = 0 (mod 8).  box2d1r: 8 chunks (2 MFMAs per slice
 // product per 256 outputs), box2d3r: 16 (4 MFMAs), star2d3r: 12 (3 MFMAs).
 struct CTable {
+  int map;           // anchor map of the 16 x 16 blocks (split_arow / split_aoct): 0 plain, 1 skew
   int n;             // chunks, a multiple of 4 (4 per MFMA)
   int c0;            // chunk start residue mod 8
   int ty[64], tx[64], live[64];
@@ -710,6 +711,16 @@ constexpr int best_pitch_dtables(const DTable *ts, int nt, int px0, long bpx, lo
   return best;
 }
 
+// Anchors of the split form's 16 x 16 blocks.  D column n of an MFMA (lane l: n = l & 15)
+// is the 2 x 8 cell group at row pair split_arow(n) and column octet split_aoct(n) of the
+// block; D row m = 4 (l >> 4) + v is the cell (m >> 3, m & 7) inside it.  Any bijection of
+// n onto the 8 x 2 groups computes the same stencil (the A fragments only depend on m).
+// Map 0 is n & 7, n >> 3 (the 3-D kernels); map 1 flips the octet of the row pairs 4..7,
+// which with the chunk order of order_ctable makes the 16-byte B reads conflict-free and
+// halves the conflicts of the 8-byte result writes at pitches of 8 (mod 16) elements.
+constexpr int split_arow(int map, int n) { return (void)map, n & 7; }
+constexpr int split_aoct(int map, int n) { return map == 1 ? ((n >> 3) ^ ((n >> 2) & 1)) : (n >> 3); }
+
 // LDS cost of one 16 x 16 block of the split form at pitch px (elements of `esz` bytes):
 // one 16-byte read per slice and instruction, one 8-byte write per slice.
 constexpr int ctable_lds_cost(const CTable &t, int px, int esz) {
@@ -718,13 +729,13 @@ constexpr int ctable_lds_cost(const CTable &t, int px, int esz) {
   for (int i = 0; i < t.n / 4; ++i) {
     for (int l = 0; l < 64; ++l) {
       const int g = l >> 4, n = l & 15, c = 4 * i + g;
-      a[l] = ((2 * (n & 7) + t.ty[c] + 16) * px + 8 * (n >> 3) + t.tx[c] + 16) * esz;
+      a[l] = ((2 * split_arow(t.map, n) + t.ty[c] + 16) * px + 8 * split_aoct(t.map, n) + t.tx[c] + 16) * esz;
     }
     cost += lds_cost(a, 16, false);
   }
   for (int l = 0; l < 64; ++l) {
     const int g = l >> 4, n = l & 15;
-    a[l] = ((2 * (n & 7) + (g >> 1)) * px + 8 * (n >> 3) + 4 * (g & 1)) * esz;
+    a[l] = ((2 * split_arow(t.map, n) + (g >> 1)) * px + 8 * split_aoct(t.map, n) + 4 * (g & 1)) * esz;
   }
   return cost + lds_cost(a, 8, true);
 }
@@ -759,11 +770,11 @@ constexpr int best_pitch_ctable(const CTable &t, int px0, int esz, long bpx, lon
 // 1..3 is tried at pitch px, the cheapest kept (stopping at a conflict-free choice).
 constexpr int kOrderPitch = 112;  // 2-byte slices: 16 (mod 32) elements
 
-constexpr int cinstr_cost(const int (&ty)[4], const int (&tx)[4], int px, int esz) {
+constexpr int cinstr_cost(const int (&ty)[4], const int (&tx)[4], int px, int esz, int map) {
   int a[64] = {};
   for (int l = 0; l < 64; ++l) {
     const int g = l >> 4, n = l & 15;
-    a[l] = ((2 * (n & 7) + ty[g] + 16) * px + 8 * (n >> 3) + tx[g] + 16) * esz;
+    a[l] = ((2 * split_arow(map, n) + ty[g] + 16) * px + 8 * split_aoct(map, n) + tx[g] + 16) * esz;
   }
   return lds_cost(a, 16, false);
 }
@@ -784,7 +795,7 @@ constexpr CTable order_ctable(const CTable &t, int px, int esz) {
           if (used[x3] || x3 == x1 || x3 == x2) continue;
           const int ty[4] = {t.ty[c0], t.ty[x1], t.ty[x2], t.ty[x3]};
           const int tx[4] = {t.tx[c0], t.tx[x1], t.tx[x2], t.tx[x3]};
-          const int c = cinstr_cost(ty, tx, px, esz);
+          const int c = cinstr_cost(ty, tx, px, esz, t.map);
           if (c < best) {
             best = c;
             b1 = x1;
@@ -805,8 +816,14 @@ constexpr CTable order_ctable(const CTable &t, int px, int esz) {
   return o;
 }
 
-// Chunk table of the split-precision 2-D kernels (and their host-side A fragments).
-constexpr CTable make_ctable(int R, int shape) { return order_ctable(make_ctable_rows(R, shape), kOrderPitch, 2); }
+// Chunk table of the split-precision 2-D kernels (and their host-side A fragments): the
+// skewed anchor map, chunks ordered for it at a pitch of 8 (mod 16) elements.
+constexpr int kOrderPitchSkew = 88;
+constexpr CTable make_ctable(int R, int shape) {
+  CTable t = make_ctable_rows(R, shape);
+  t.map = 1;
+  return order_ctable(t, kOrderPitchSkew, 2);
+}
 
 // One evaluation per stencil class for the kernels' compile-time tables.
 template <int R, int SH>

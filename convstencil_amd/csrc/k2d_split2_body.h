@@ -342,20 +342,22 @@ __device__ __forceinline__ void split_lane_setup(const typename C::T *__restrict
   t.lane = tid & 63;
   t.wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = t.lane, grp = lane >> 4, n = lane & 15;
-  // D[m][n], m = 4 grp + v: cell (2 (n & 7) + (grp >> 1), 8 (n >> 3) + 4 (grp & 1) + v)
-  t.orow = 2 * (n & 7) + (grp >> 1);
-  t.ocol = 8 * (n >> 3) + 4 * (grp & 1);
+  // D[m][n], m = 4 grp + v: cell (2 arow + (grp >> 1), 8 aoct + 4 (grp & 1) + v) of the
+  // block, where (arow, aoct) is the anchor of column n (cs_ktable.h, split_arow / split_aoct)
+  const int arow = split_arow(C::CT.map, n), aoct = split_aoct(C::CT.map, n);
+  t.orow = 2 * arow + (grp >> 1);
+  t.ocol = 8 * aoct + 4 * (grp & 1);
   t.vw = (t.orow * C::PX + t.ocol + C::CS) * (int)sizeof(T);
 #pragma unroll
   for (int q = 0; q < C::NSW; ++q)
 #pragma unroll
     for (int i = 0; i < C::NINS; ++i)
       w[q][i] = *reinterpret_cast<const V *>(bfrag + ((size_t)(q * C::NINS + i) * 64 + lane) * 8);
-  // B reads: chunk 4 i + grp of instruction i, anchor u_n = (2 (n & 7), 8 (n >> 3))
+  // B reads: chunk 4 i + grp of instruction i, anchor u_n = (2 arow, 8 aoct)
 #pragma unroll
   for (int i = 0; i < C::NINS; ++i) {
     if (C::RPL.rep[i] != i) continue;
-    off[i] = ((2 * (n & 7) + C::CT.ty[4 * i + grp]) * C::PX + 8 * (n >> 3) + C::CT.tx[4 * i + grp] + C::CS) *
+    off[i] = ((2 * arow + C::CT.ty[4 * i + grp]) * C::PX + 8 * aoct + C::CT.tx[4 * i + grp] + C::CS) *
              (int)sizeof(T);
   }
 }

@@ -72,6 +72,11 @@ int64_t cs_debug_bfrag_split2(int32_t precision, int32_t radius, int32_t shape, 
 int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty, int32_t *tx, int32_t *live,
                         int32_t cap);
 
+/* Anchor map of that chunk table's 16 x 16 blocks: D column n (lane l: n = l & 15) is the 2 x 8
+ * cell group at row pair n & 7 and column octet (n >> 3) ^ ((n >> 2) & 1) for map 1, n >> 3 for
+ * map 0 (the 3-D split kernels); D row m = 4 (l >> 4) + v is its cell (m >> 3, m & 7). */
+int32_t cs_debug_ctable_map(int32_t radius, int32_t shape);
+
 /* Dilated table of plane dz of the 3-D 8 x 8 block kernels (variants with mfma shape 8), in the
  * layout of cs_debug_dtable; all planes share the pair parity hdr[2].  Returns 4 * (np + ns) or -1. */
 int32_t cs_debug_dtable3(int32_t radius, int32_t shape, int32_t dz, int32_t *hdr, int32_t *ty, int32_t *tx,

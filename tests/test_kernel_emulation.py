@@ -1045,11 +1045,16 @@ def emulate_split2(v, layout, dense, x):
     for q in range(NSW):
         for i in range(NINS):
             A[q, i][nn[:, None], 8 * grp[:, None] + np.arange(8)[None, :]] = wf[q, i]
-    off = np.array([(2 * (nn & 7) + cty[4 * i + grp]) * PX + 8 * (nn >> 3) + ctx[4 * i + grp] + CS
+    lib.cs_debug_ctable_map.restype = ctypes.c_int32
+    lib.cs_debug_ctable_map.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    amap = lib.cs_debug_ctable_map(R, SH)
+    arow = nn & 7
+    aoct = ((nn >> 3) ^ ((nn >> 2) & 1)) if amap == 1 else (nn >> 3)
+    off = np.array([(2 * arow + cty[4 * i + grp]) * PX + 8 * aoct + ctx[4 * i + grp] + CS
                     for i in range(NINS)])
-    # D lane l holds m = 4 (l >> 4) + v, n = l & 15: cell (2 (n & 7) + (m >> 3), 8 (n >> 3) + (m & 7))
-    orow = 2 * (nn & 7) + (grp >> 1)
-    ocol = 8 * (nn >> 3) + 4 * (grp & 1)
+    # D lane l holds m = 4 (l >> 4) + v, n = l & 15: cell (2 arow + (m >> 3), 8 aoct + (m & 7))
+    orow = 2 * arow + (grp >> 1)
+    ocol = 8 * aoct + 4 * (grp & 1)
     INV = np.float32(1.0 / SCALE)
     ext, org, n = layout.ext, layout.org, layout.n
     pitch = ext[2]
