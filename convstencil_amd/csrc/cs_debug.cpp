@@ -17,6 +17,8 @@ static const cs::Variant *family(int32_t ndim, int *n) {
   if (ndim == 2) return cs::variants_2d_f64(n);
   if (ndim == 3) return cs::variants_3d_f64(n);
   if (ndim == 6) return cs::variants_2d_stream(n);
+  if (ndim == 7) return cs::variants_1d_reg(n);
+  if (ndim == 8) return cs::variants_2d_reg(n);
   return nullptr;
 }
 
@@ -148,6 +150,25 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
     live[i] = t.live[i];
   }
   return t.n;
+}
+
+int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t kf, int64_t slots, int64_t *split) {
+  if (ny < 1 || nx < 1 || ox < 1 || kf < 1 || slots < 1 || !split) return -1;
+  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, kf, slots, cs::kRegEdgeCost);
+  const int64_t v[5] = {sp.strips, sp.he, sp.hm, sp.we, sp.wi};
+  memcpy(split, v, sizeof(v));
+  return 0;
+}
+
+int32_t cs_debug_reg2d_chunk(const int64_t *split, int64_t ny, int64_t wid, int64_t *rows) {
+  if (!split || !rows) return -1;
+  const cs::RegSplit sp{split[0], split[1], split[2], split[3], split[4]};
+  int64_t strip = 0, y0 = 0, y1 = 0;
+  if (!cs::reg_chunk(sp, ny, wid, &strip, &y0, &y1)) return 0;
+  rows[0] = strip;
+  rows[1] = y0;
+  rows[2] = y1;
+  return 1;
 }
 
 int32_t cs_debug_ctable_map(int32_t radius, int32_t shape) { return cs::make_ctable(radius, shape).map; }

@@ -94,6 +94,15 @@ std::vector<Variant> joined3(const Variant *(*a)(int *), const Variant *(*b)(int
   return v;
 }
 
+std::vector<Variant> joined4(const Variant *(*a)(int *), const Variant *(*b)(int *), const Variant *(*c)(int *),
+                             const Variant *(*d)(int *)) {
+  std::vector<Variant> v = joined3(a, b, c);
+  int nd = 0;
+  const Variant *z = d(&nd);
+  v.insert(v.end(), z, z + nd);
+  return v;
+}
+
 const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   *cnt = 0;
   if (prec != CS_PREC_FP64)
@@ -101,8 +110,8 @@ const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   if (ndim < 1 || ndim > 3) return nullptr;
   // the MFMA kernels, then the --breakdown ladder's FMA kernels
   static const std::vector<Variant> all[3] = {
-      joined(cs::variants_1d_f64, cs::variants_1d_ablation),
-      joined3(cs::variants_2d_f64, cs::variants_2d_stream, cs::variants_2d_ablation),
+      joined3(cs::variants_1d_reg, cs::variants_1d_f64, cs::variants_1d_ablation),
+      joined4(cs::variants_2d_f64, cs::variants_2d_reg, cs::variants_2d_stream, cs::variants_2d_ablation),
       joined3(cs::variants_3d_f64, cs::variants_3d_fma, cs::variants_3d_ablation)};
   *cnt = (int)all[ndim - 1].size();
   return all[ndim - 1].data();
@@ -142,11 +151,12 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
     v = find_variant(nd, 0, R, shape, 0, tile_variant, prec);
     if (!v) v = find_variant(nd, 0, R, cs::SHAPE_BOX, 0, tile_variant, prec);
   } else if (prec == CS_PREC_FP64) {
-    // 3-D: the 8 x 8 block family (mf 8) where it exists, else the register-blocked FMA
-    // kernels (mf 3, radius 2-3), else the 4 x 16 strips
-    for (int fmf : {nd == 3 ? 8 : mf, nd == 3 ? 3 : mf, mf}) {
+    // 1-D: the register-resident kernels (mf 6) at their deepest compiled fusion, else the
+    // 4 x 4 x 4 blocks; 3-D: the 8 x 8 block family (mf 8) where it exists, else the
+    // register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips
+    for (int fmf : {nd == 3 ? 8 : nd == 1 ? 6 : mf, nd == 3 ? 3 : mf, mf}) {
       const int top = max_kf(nd, fmf, R, shape, prec);
-      int kf = fusion_depth > 0 ? std::min(fusion_depth, top) : default_kf(nd, R, top, shape, true);
+      int kf = fusion_depth > 0 ? std::min(fusion_depth, top) : fmf == 6 ? top : default_kf(nd, R, top, shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
       if (v) break;
     }
@@ -206,9 +216,9 @@ cs_status select_variants(cs_plan *p) {
     CS_HIP(hipMemcpy(p->bfrag, hb.data(), bytes, hipMemcpyHostToDevice));
     return CS_OK;
   }
-  // the FMA kernels (the ladder's, and the 3-D mf 3 family) read the dense weights
-  // (zero-padded to whole 64-entry rows)
-  if (v->mf == 1 || v->mf == 2 || v->mf == 3) {
+  // the FMA kernels (the ladder's, the 3-D mf 3 family and the 2-D register-streaming
+  // mf 7 family) read the dense weights (zero-padded to whole 64-entry rows)
+  if (v->mf == 1 || v->mf == 2 || v->mf == 3 || v->mf == 7) {
     std::vector<double> h((p->w.size() + 63) / 64 * 64, 0.0);
     std::copy(p->w.begin(), p->w.end(), h.begin());
     if (p->bfrag) (void)hipFree(p->bfrag);

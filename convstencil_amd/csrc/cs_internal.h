@@ -14,6 +14,21 @@ namespace cs {
 // accesses across it.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Lane shifts of a double across the wave (v_mov_b32_dpp wave_shr:1 / wave_shl:1, bound_ctrl:
+// the lane without a source receives zero).  lane_from_lower: lane l gets lane l - 1's value;
+// lane_from_upper: lane l gets lane l + 1's value.
+__device__ __forceinline__ double lane_from_lower(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_from_upper(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -58,6 +73,95 @@ __device__ __forceinline__ void prefetch_consume(const double (&pf)[Prefetch<C>:
   for (int i = 0; i < Prefetch<C>::N; ++i) asm volatile("" ::"v"(pf[i]));
 }
 
+// Decomposition of one launch of the register-streaming 2-D kernels (k2d_reg.hip) into
+// waves: each wave sweeps rows [y0, y1) of one strip of output columns.  Strips 0 and
+// strips - 1 (the edge strips, which run the checked path) are cut into `we` chunks of
+// `he` rows; every other strip into `wi` chunks: `he` rows at the top, `he` at the bottom
+// (the checked path) and chunks of `hm` rows between them; hm = 0 marks strips of at most
+// 2 he rows, cut like the edge strips.  Waves: the edge strips' first.
+struct RegSplit {
+  int64_t strips, he, hm, we, wi;
+};
+
+__host__ __device__ inline int64_t reg_waves(const RegSplit &sp) {
+  const int64_t ne = sp.strips >= 2 ? 2 : 1;
+  return ne * sp.we + (sp.strips - ne) * sp.wi;
+}
+
+// Strip and rows [y0, y1) of wave wid; false for a wave without work.
+__host__ __device__ inline bool reg_chunk(const RegSplit &sp, int64_t ny, int64_t wid, int64_t *strip, int64_t *y0,
+                                          int64_t *y1) {
+  const int64_t ne = sp.strips >= 2 ? 2 : 1;
+  int64_t a, b;
+  if (wid < ne * sp.we) {
+    *strip = wid < sp.we ? 0 : sp.strips - 1;
+    a = (wid % sp.we) * sp.he;
+    b = a + sp.he;
+  } else {
+    const int64_t u = wid - ne * sp.we;
+    *strip = 1 + u / sp.wi;
+    if (*strip >= sp.strips - 1) return false;
+    const int64_t c = u % sp.wi;
+    if (sp.hm == 0) {  // a short strip: chunks of he rows from the top
+      a = c * sp.he;
+      b = a + sp.he;
+    } else if (c == 0) {
+      a = 0;
+      b = sp.he;
+    } else if (c == sp.wi - 1) {
+      a = ny - sp.he;
+      b = ny;
+    } else {
+      a = sp.he + (c - 1) * sp.hm;
+      b = a + sp.hm < ny - sp.he ? a + sp.hm : ny - sp.he;
+    }
+  }
+  if (a >= ny || b <= a) return false;
+  *y0 = a;
+  *y1 = b < ny ? b : ny;
+  return true;
+}
+
+// Time of a wave on the checked path (edge strips, first and last chunk of a strip)
+// relative to the plain path, per row.
+constexpr double kRegEdgeCost = 1.5;
+
+// The decomposition for a grid of ny x nx interior cells, `ox` output columns per strip,
+// fusion depth kf and `slots` resident waves: the smallest per-wave cost (rows plus the 2 kf
+// warm-up rows, checked rows counted `edge_cost` times) at which every wave is resident at
+// once, so that no wave waits for a second round.
+inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int kf, int64_t slots, double edge_cost) {
+  RegSplit sp{};
+  sp.strips = (nx + ox - 1) / ox;
+  const int64_t warm = 2 * (int64_t)kf;
+  auto make = [&](int64_t cost) {
+    RegSplit t = sp;
+    t.hm = cost - warm;
+    t.he = (int64_t)((double)cost / edge_cost) - warm;
+    const int64_t least = kf + 6;  // the plain path needs kf + 1 rows above and kf + 5 below
+    if (t.he < least) t.he = least;
+    if (t.hm < 1) t.hm = 1;
+    t.we = (ny + t.he - 1) / t.he;
+    if (ny <= 2 * t.he) {
+      t.wi = t.we;
+      t.hm = 0;
+    } else {
+      t.wi = 2 + (ny - 2 * t.he + t.hm - 1) / t.hm;
+    }
+    return t;
+  };
+  int64_t lo = warm + 1, hi = ny + warm;
+  if (reg_waves(make(hi)) > slots) return make(hi);  // more strips than slots: one chunk per strip
+  while (lo < hi) {
+    const int64_t mid = lo + (hi - lo) / 2;
+    if (reg_waves(make(mid)) <= slots)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return make(lo);
+}
+
 typedef hipError_t (*LaunchFn)(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s);
 
 // One kernel variant of the FP64 stencil2row families.
@@ -77,7 +181,9 @@ struct Variant {
 };
 
 const Variant *variants_1d_f64(int *count);
+const Variant *variants_1d_reg(int *count);   // register-resident FP64 1-D (k1d_reg.hip; mf 6)
 const Variant *variants_2d_f64(int *count);
+const Variant *variants_2d_reg(int *count);     // register-streaming FP64 2-D, radius 1 (k2d_reg.hip; mf 7)
 const Variant *variants_2d_stream(int *count);  // pipelined persistent 2-D FP64 (k2d_stream.hip)
 const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
 const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only

@@ -19,7 +19,9 @@ extern "C" {
  * 4 = 2-D and 5 = 3-D split-precision kernels (there info[1] holds the cs_precision; the 3-D
  * split kernels' z-chunk length is in their name, "zc<n>"); 6 = the pipelined persistent 2-D
  * FP64 kernels (two LDS tile buffers, producer waves "p<n>" in the name; info[8] = both
- * buffers' bytes). */
+ * buffers' bytes); 7 = the register-resident 1-D FP64 kernels (info[1] = 6, info[6] = outputs
+ * per workgroup, info[9] = cells per lane); 8 = the register-streaming 2-D FP64 kernels
+ * (info[1] = 7, info[6] = output columns per wave, info[9] = columns per lane). */
 int32_t cs_debug_variant_count(int32_t ndim);
 /* info[0..11] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
  * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
@@ -76,6 +78,17 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
  * cell group at row pair n & 7 and column octet (n >> 3) ^ ((n >> 2) & 1) for map 1, n >> 3 for
  * map 0 (the 3-D split kernels); D row m = 4 (l >> 4) + v is its cell (m >> 3, m & 7). */
 int32_t cs_debug_ctable_map(int32_t radius, int32_t shape);
+
+/* Wave decomposition of one launch of the register-streaming 2-D kernels (family 8): for
+ * ny x nx interior cells, `ox` output columns per strip, fusion depth kf and `slots` resident
+ * waves, split[0..4] = strips, he, hm, we, wi (rows per checked chunk, rows per plain chunk,
+ * chunks per edge strip, chunks per other strip); waves = the edge strips' chunks first.
+ * Returns 0, or -1 for invalid arguments. */
+int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t kf, int64_t slots, int64_t *split);
+
+/* Wave `wid` of that decomposition: rows[0..2] = strip, first row, end row; returns 1, or 0 for
+ * a wave without work. */
+int32_t cs_debug_reg2d_chunk(const int64_t *split, int64_t ny, int64_t wid, int64_t *rows);
 
 /* Dilated table of plane dz of the 3-D 8 x 8 block kernels (variants with mfma shape 8), in the
  * layout of cs_debug_dtable; all planes share the pair parity hdr[2].  Returns 4 * (np + ns) or -1. */

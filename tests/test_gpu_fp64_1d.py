@@ -96,11 +96,13 @@ def test_reference_gpu_1d2r(gpu):
     assert np.array_equal(out[:4], want[:4])
 
 
-@pytest.mark.parametrize("vid", [511016, 521016, 531016, 511008, 511032, 531008, 933001])
+# 6xxxxx: the register-resident kernels, 600000 + R * 10000 + cells per lane * 100 + depth
+@pytest.mark.parametrize("vid", [511016, 521016, 531016, 511008, 511032, 531008, 933001, 611864, 611032, 612664,
+                                 611808, 621832, 621808, 631816, 631804, 641816, 641802])
 def test_tile_variants(gpu, vid):
     # every alternative tile, forced: partial tiles, fused launches and remainders
     print(f"case variant {vid}", flush=True)
-    r = 3 if vid == 933001 else 1
+    r = 3 if vid == 933001 else (vid // 10000) % 10 if vid >= 600000 else 1
     offs = [(0, 0, d) for d in range(-r, r + 1)]
     for n, steps in ((5000, 37), (513, 16), (100003, 9)):
         L = Layout.with_halo([n], r)

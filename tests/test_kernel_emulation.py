@@ -20,7 +20,7 @@ import numpy as np
 import pytest
 
 from convstencil_amd import _lib
-from convstencil_amd.shapes import Layout
+from convstencil_amd.shapes import Layout, dense_weights, shape_offsets
 from oracle import oracle
 
 
@@ -466,6 +466,74 @@ def test_variant_emulation_1d(v):
                                                                   [v["bx"] // 2 + 3])]:
         x = oracle.fill_data(layout, v["id"])
         got = emulate_1d(v, layout, w, x)
+        offs = [(0, 0, d) for d in range(-R, R + 1)]
+        want = oracle.run(layout, offs, w, x, kf)
+        gi, wi = layout.interior(got), layout.interior(want)
+        err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+        assert err < 1e-13, (v["name"], err)
+
+
+def emulate_1d_reg(v, layout, w, x):
+    """k1d_reg: wave segments of 64 P cells (lane l holds cells l P .. l P + P - 1), KF steps
+    in registers with the R cells beyond each lane's run from lanes l -+ 1 (zero past lanes 0
+    and 63), then the central 64 P - 2 R KF cells stored.  Counts the stores per cell."""
+    R, kf, P, NW = v["R"], v["kf"], v["px"], v["nw"]
+    H = R * kf
+    O = 64 * P - 2 * H
+    assert v["mf"] == 6 and v["bx"] == NW * O and v["lds"] == 0 and O > 0 and P >= R
+    lib = _dbg()
+    cap = 64 * 16
+    bfr = np.zeros(cap)
+    wd = np.ascontiguousarray(w, dtype=np.float64)
+    nks = lib.cs_debug_bfrag_1d(R, wd.ctypes.data_as(_lib.c_double_p), bfr.ctypes.data_as(_lib.c_double_p), cap)
+    assert nks > 0
+    # the kernel's weight reads: entry (k-row f, column 0) of the 4 x 4 x 4 B fragment
+    wk = np.array([bfr[(f >> 2) * 64 + ((f & 3) << 4)] for f in range(2 * R + 1)])
+    assert np.array_equal(wk, wd)
+    ext, org, n = layout.ext[2], layout.org[2], layout.n[2]
+    out = np.zeros(layout.cells)
+    hits = np.zeros(layout.cells, dtype=np.int64)
+    blocks = -(-n // v["bx"])
+    d = np.arange(64 * P)
+    for seg in range(blocks * NW):
+        if seg * O >= n:
+            continue
+        X0 = seg * O - H
+        c = X0 + d
+        a = org + c
+        ok = (a >= 0) & (a < ext)
+        vals = np.where(ok, x[np.where(ok, a, 0)], 0.0)
+        full = X0 >= 0 and X0 + 64 * P <= n
+        for _ in range(kf):
+            pad = np.concatenate([np.zeros(R), vals, np.zeros(R)])
+            new = wk[0] * pad[0:64 * P]
+            for t in range(1, 2 * R + 1):
+                new = new + wk[t] * pad[t:t + 64 * P]
+            if not full:
+                new = np.where((c >= 0) & (c < n), new, vals)
+            vals = new
+        st = (d >= H) & (d < H + O) & (c < n)
+        assert (c[st] >= 0).all() and (a[st] < org + n).all()
+        out[a[st]] = vals[st]
+        hits[a[st]] += 1
+    assert (layout.interior(hits) == 1).all()
+    return out
+
+
+VARIANTS_1D_REG = variants(7)
+
+
+@pytest.mark.parametrize("v", VARIANTS_1D_REG, ids=[v["name"] for v in VARIANTS_1D_REG])
+def test_variant_emulation_1d_reg(v):
+    R, kf = v["R"], v["kf"]
+    rng = np.random.default_rng(v["id"])
+    w = rng.uniform(0.05, 1.0, 2 * R + 1)
+    w /= w.sum()
+    O = v["bx"] // v["nw"]
+    for layout in [Layout.with_halo([v["bx"] + 77], R), Layout(1, [O // 2 + 2 * R + 9], [R + 4], [O // 2 + 3]),
+                   Layout.with_halo([3 * O + 5], R + 2)]:
+        x = oracle.fill_data(layout, v["id"])
+        got = emulate_1d_reg(v, layout, w, x)
         offs = [(0, 0, d) for d in range(-R, R + 1)]
         want = oracle.run(layout, offs, w, x, kf)
         gi, wi = layout.interior(got), layout.interior(want)
@@ -1568,3 +1636,145 @@ def test_variant_emulation_stream(v):
         gi, wi = layout.interior(got), layout.interior(want)
         err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
         assert err < 1e-13, (v["name"], err)
+
+
+# ---------------------------------------------------------------- 2-D register-streaming family
+
+def reg2d_split(ny, nx, ox, kf, slots):
+    lib = _dbg()
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    lib.cs_debug_reg2d_split.restype = ctypes.c_int32
+    lib.cs_debug_reg2d_split.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int64, i64p]
+    lib.cs_debug_reg2d_chunk.restype = ctypes.c_int32
+    lib.cs_debug_reg2d_chunk.argtypes = [i64p, ctypes.c_int64, ctypes.c_int64, i64p]
+    split = (ctypes.c_int64 * 5)()
+    assert lib.cs_debug_reg2d_split(ny, nx, ox, kf, slots, split) == 0
+    strips, he, hm, we, wi = list(split)
+    ne = 2 if strips >= 2 else 1
+    chunks = []
+    for wid in range(ne * we + (strips - ne) * wi):
+        rows = (ctypes.c_int64 * 3)()
+        if lib.cs_debug_reg2d_chunk(split, ny, wid, rows) == 1:
+            chunks.append(tuple(rows))
+    return list(split), chunks
+
+
+def emulate_2d_reg(v, layout, dense, x, slots):
+    """k2d_reg: each wave sweeps rows [y0, y1) of a strip of 64 PX columns (lane l: columns
+    l PX .. l PX + PX - 1, so that a cell's column neighbours are the adjacent cells of the
+    flat lane-major row, zero past lanes 0 and 63), one input row per iteration in blocks of
+    three, level s + 1 holding partial sums for rows yq - 1, yq, yq + 1 in slots (K - s - 1,
+    K - s, K - s + 1) mod 3, cells outside the interior frozen on the checked path, level KF's
+    completed row stored on the output columns.  Checks loads and stores against the array
+    and counts the stores per cell."""
+    PX, KF = v["px"], v["kf"]
+    box = v["shape"] == 0
+    H = KF
+    OX = 64 * PX - 2 * H
+    assert v["mf"] == 7 and v["R"] == 1 and v["bx"] == OX and v["lds"] == 0
+    _, ey, pitch = layout.ext
+    _, oy, ox = layout.org
+    _, ny, nx = layout.n
+    X = x.reshape(ey, pitch)
+    w = np.asarray(dense, dtype=np.float64).reshape(3, 3)
+    out = np.zeros((ey, pitch))
+    hits = np.zeros((ey, pitch), dtype=np.int64)
+    _, chunks = reg2d_split(ny, nx, OX, KF, slots)
+    d = np.arange(64 * PX)
+    for strip, y0, y1 in chunks:
+        xs = strip * OX - H
+        cols = xs + d
+        fast = xs >= 0 and xs + 64 * PX <= nx and y0 - H - 1 >= 0 and y1 + H + 6 <= ny
+        colin = (cols >= 0) & (cols < nx)
+        colarr = (ox + cols >= 0) & (ox + cols < pitch)
+        colout = (d >= H) & (d < H + OX) & (cols < nx)
+
+        def load(r):
+            ay = oy + r
+            if fast:
+                assert 0 <= ay < ey and colarr.all()
+                return X[ay, ox + cols].copy()
+            ok = colarr & (0 <= ay < ey)
+            return np.where(ok, X[ay if 0 <= ay < ey else 0, np.where(ok, ox + cols, 0)], 0.0)
+
+        ys, iters = y0 - H, (y1 - y0) + 2 * H
+        acc = np.zeros((KF, 3, 64 * PX))
+        ring = [load(ys), load(ys + 1), None]
+        for it in range(0, iters, 3):
+            for K in range(3):
+                r = ys + it + K
+                ring[(K + 2) % 3] = load(r + 2)
+                q = ring[K].copy()
+                for s in range(KF):
+                    yq = r - s
+                    sA, sB, sC = (K - s - 1) % 3, (K - s) % 3, (K - s + 1) % 3
+                    qe = np.concatenate([[0.0], q, [0.0]])
+                    lf, mid, rt = qe[:-2], q, qe[2:]
+                    A, B = acc[s, sA], acc[s, sB]
+                    a = A + w[2, 0] * lf + w[2, 1] * mid + w[2, 2] * rt if box else A + w[2, 1] * mid
+                    b = B + w[1, 0] * lf + w[1, 1] * mid + w[1, 2] * rt
+                    if fast:
+                        A, B = a, b
+                    else:
+                        A = np.where(~colin | (yq - 1 < 0) | (yq - 1 >= ny), A, a)
+                        B = np.where(~colin | (yq < 0) | (yq >= ny), q, b)
+                    Cn = w[0, 0] * lf + w[0, 1] * mid + w[0, 2] * rt if box else w[0, 1] * mid
+                    acc[s, sA], acc[s, sB], acc[s, sC] = A, B, Cn
+                    q = A.copy()
+                yo = r - KF
+                if y0 <= yo < y1:
+                    c = ox + cols[colout]
+                    assert 0 <= oy + yo < ey and (c >= 0).all() and (c < pitch).all()
+                    out[oy + yo, c] = q[colout]
+                    hits[oy + yo, c] += 1
+    inner = hits[oy:oy + ny, ox:ox + nx]
+    assert (inner == 1).all(), (int(inner.min()), int(inner.max()))
+    assert hits.sum() == inner.sum()
+    return out.reshape(-1)
+
+
+VARIANTS_REG2D = variants(8)
+
+
+@pytest.mark.parametrize("v", VARIANTS_REG2D, ids=[v["name"] for v in VARIANTS_REG2D])
+def test_variant_emulation_2d_reg(v):
+    # grids of one, two and three strips, with halos of 1 and of odd widths (masks at the
+    # array's edges), few slots (tall chunks: the plain path inside) and many
+    R, kf, OX = 1, v["kf"], v["bx"]
+    name = "box2d1r" if v["shape"] == 0 else "star2d1r"
+    offs = shape_offsets(name)
+    rng = np.random.default_rng(v["id"])
+    w = rng.uniform(0.05, 1.0, len(offs))
+    w /= w.sum()
+    dense = dense_weights(2, R, offs, w)
+    cases = [(Layout.with_halo([6 * kf + 40, 2 * OX + 37], 1), 6),
+             (Layout(2, [4 * kf + 31, OX // 2 + 12], [2, 3], [4 * kf + 26, OX // 2 + 5]), 3),
+             (Layout.with_halo([9 * kf + 25, OX + 11], 1), 64)]
+    for layout, slots in cases:
+        x = oracle.fill_data(layout, v["id"] + slots)
+        got = emulate_2d_reg(v, layout, dense, x, slots)
+        want = oracle.run(layout, offs, w, x, kf)
+        gi, wi = layout.interior(got), layout.interior(want)
+        err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+        assert err < 1e-13, (v["name"], slots, err)
+
+
+@pytest.mark.parametrize("ny,nx,ox,kf,slots", [(10240, 10240, 240, 8, 2048), (8192, 8192, 240, 8, 2048),
+                                               (100, 3000, 240, 8, 2048), (5000, 200, 96, 16, 2048),
+                                               (31, 31, 254, 1, 4096), (1, 7, 240, 8, 16)])
+def test_reg2d_split_covers_once(ny, nx, ox, kf, slots):
+    # every row of every strip in exactly one chunk, never more waves than slots when the
+    # strips fit
+    split, chunks = reg2d_split(ny, nx, ox, kf, slots)
+    strips, he, hm, we, wi = split
+    ne = 2 if strips >= 2 else 1
+    waves = ne * we + (strips - ne) * wi
+    assert waves <= max(slots, strips)
+    cover = np.zeros((strips, ny), dtype=np.int64)
+    for s, y0, y1 in chunks:
+        assert 0 <= y0 < y1 <= ny
+        cover[s, y0:y1] += 1
+    assert (cover == 1).all()
+    # a plain-path chunk needs kf + 1 rows above and kf + 5 below it inside the grid
+    assert he >= kf + 6
