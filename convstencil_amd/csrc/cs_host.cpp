@@ -67,10 +67,11 @@ Geo make_geo(const cs_layout &L, const Variant *v) {
 
 // Default fusion depth when the caller does not choose one (the autotuner's
 // cache, when present, overrides this).
-// FP64 (measured, GStencil/s per physical step): 1d1r 2^20 kf16 878 (kf32 842); box2d1r
-// 10240^2 kf6 925-990 (kf4 861); star2d3r 8192^2 kf3 494 (kf2 454); box3d1r 512^3 kf1
-// 260-265 (kf2 219); star3d1r kf2 303 (kf1 272).  Split precision: kf4 (2-D radius 1),
-// kf2 (2-D radius 2-3 and 3-D).
+// FP64 MFMA families (measured, GStencil/s per physical step): 1d1r 2^20 kf16 878 (kf32
+// 842); box2d1r 10240^2 kf6 925-990 (kf4 861); star2d3r 8192^2 kf3 494 (kf2 454); box3d1r
+// 512^3 kf1 260-265 (kf2 219); star3d1r kf2 303 (kf1 272).  Split precision: kf4 (2-D
+// radius 1), kf2 (2-D radius 2-3 and 3-D).  The register kernels that pick_main prefers
+// for 1-D and 2-D radius 1 choose their own depths there.
 int default_kf(int ndim, int R, int top, int shape, bool fp64) {
   if (ndim == 1) return std::min(top, 16);
   if (ndim == 3) return std::min(top, fp64 && shape == cs::SHAPE3_BOX ? 1 : 2);
@@ -152,11 +153,22 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
     if (!v) v = find_variant(nd, 0, R, cs::SHAPE_BOX, 0, tile_variant, prec);
   } else if (prec == CS_PREC_FP64) {
     // 1-D: the register-resident kernels (mf 6) at their deepest compiled fusion, else the
-    // 4 x 4 x 4 blocks; 3-D: the 8 x 8 block family (mf 8) where it exists, else the
-    // register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips
-    for (int fmf : {nd == 3 ? 8 : nd == 1 ? 6 : mf, nd == 3 ? 3 : mf, mf}) {
+    // 4 x 4 x 4 blocks; 2-D: the register-streaming kernels (mf 7, radius 1) at depth 12,
+    // else the dilated 8 x 8 blocks; 3-D: the 8 x 8 block family (mf 8) where it exists,
+    // else the register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips
+    const int fams[3] = {nd == 3 ? 8 : nd == 1 ? 6 : 7, nd == 3 ? 3 : mf, mf};
+    // a requested depth that some family compiles exactly is honoured exactly
+    if (fusion_depth > 0)
+      for (int fmf : fams)
+        if (!v) v = find_variant(nd, fmf, R, shape, fusion_depth, 0, prec);
+    for (int fmf : fams) {
+      if (v) break;
       const int top = max_kf(nd, fmf, R, shape, prec);
-      int kf = fusion_depth > 0 ? std::min(fusion_depth, top) : fmf == 6 ? top : default_kf(nd, R, top, shape, true);
+      if (top == 0) continue;
+      int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
+               : fmf == 6       ? top
+               : fmf == 7       ? std::min(top, 12)
+                                : default_kf(nd, R, top, shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
       if (v) break;
     }
@@ -652,6 +664,9 @@ cs_status cs_plan_run(cs_plan *p, int64_t steps, cs_report *rep) {
 cs_status cs_plan_time(cs_plan *p, int64_t steps, int32_t warmup, int32_t reps, cs_report *rep) {
   if (!p || reps <= 0 || warmup < 0 || steps <= 0) return CS_ERR_INVALID;
   CS_HIP(hipSetDevice(p->device));
+  // graphs are captured per starting buffer: a run of an odd number of launches alternates
+  // the two, so a warm-up captures both before the timed runs
+  if (warmup > 0 && p->opt.use_graph && (count_launches(p, steps) & 1)) warmup = std::max(warmup, 2);
   for (int i = 0; i < warmup; ++i) CS_TRY(run_steps(p, steps));
   CS_HIP(hipStreamSynchronize(p->stream));
   hipEvent_t e0, e1;

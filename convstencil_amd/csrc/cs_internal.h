@@ -123,14 +123,19 @@ __host__ __device__ inline bool reg_chunk(const RegSplit &sp, int64_t ny, int64_
 }
 
 // Time of a wave on the checked path (edge strips, first and last chunk of a strip)
-// relative to the plain path, per row.
-constexpr double kRegEdgeCost = 1.5;
+// relative to the plain path, per row (measured: 2.3-2.6 balances the two at 10240^2).
+constexpr double kRegEdgeCost = 2.5;
+
+// Iterations per unrolled block of a sweep with a load ring of `ring` rows (the partial-sum
+// slots repeat every 3), and the rows a plain-path chunk needs above it and below it.
+__host__ __device__ constexpr int reg_period(int ring) { return ring % 3 == 0 ? ring : 3 * ring; }
+__host__ __device__ constexpr int reg_least(int kf, int ring) { return kf + reg_period(ring) + ring - 2; }
 
 // The decomposition for a grid of ny x nx interior cells, `ox` output columns per strip,
 // fusion depth kf and `slots` resident waves: the smallest per-wave cost (rows plus the 2 kf
 // warm-up rows, checked rows counted `edge_cost` times) at which every wave is resident at
-// once, so that no wave waits for a second round.
-inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int kf, int64_t slots, double edge_cost) {
+// once, so that no wave waits for a second round; checked chunks are at least `least` rows.
+inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int kf, int64_t slots, double edge_cost, int least) {
   RegSplit sp{};
   sp.strips = (nx + ox - 1) / ox;
   const int64_t warm = 2 * (int64_t)kf;
@@ -138,8 +143,7 @@ inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int kf, int64_t slots,
     RegSplit t = sp;
     t.hm = cost - warm;
     t.he = (int64_t)((double)cost / edge_cost) - warm;
-    const int64_t least = kf + 6;  // the plain path needs kf + 1 rows above and kf + 5 below
-    if (t.he < least) t.he = least;
+    if (t.he < least) t.he = least;  // plain-path chunks start and end `least` rows inside
     if (t.hm < 1) t.hm = 1;
     t.we = (ny + t.he - 1) / t.he;
     if (ny <= 2 * t.he) {

@@ -1,14 +1,15 @@
 // cs_noalloc.hip -- the allocation-free entry point (cs_run_2d_noalloc, include/convstencil.h).
 //
-// A plan (cs_plan_*) owns its device memory: the two grid buffers and the kernels' B
-// fragments (the banded weight matrix of the dilated v_mfma_f64_4x4x4 tables), which it
-// allocates with hipMalloc.  Callers that cannot allocate -- code under HIP graph capture,
-// tight serving loops, harnesses that forbid it -- pass both grid buffers themselves; the
-// B fragments go to `scratch`, a fixed device array of this library (each GPU holds one
-// copy until the process ends).  The fragments are built on the host (the same tables as
-// a plan's) in compact form, 16 values per k-step, and expanded into scratch by a small
-// kernel whose argument carries them, so no host-to-device copy and no allocation is made:
-// the whole call is kernel launches on the caller's stream.
+// A plan (cs_plan_*) owns its device memory: the two grid buffers and the kernels' weight
+// operands (the B fragments of the dilated v_mfma_f64_4x4x4 tables, or the dense weights
+// of the register-streaming kernels), which it allocates with hipMalloc.  Callers that
+// cannot allocate -- code under HIP graph capture, tight serving loops, harnesses that
+// forbid it -- pass both grid buffers themselves; the weight operands go to `scratch`, a
+// fixed device array of this library (each GPU holds one copy until the process ends).
+// They are built on the host (the same tables as a plan's) in compact form, 16 values per
+// k-step or the dense weights, and written into scratch by a small kernel whose argument
+// carries them, so no host-to-device copy and no allocation is made: the whole call is
+// kernel launches on the caller's stream.
 #include <atomic>
 
 #include "cs_host_util.h"
@@ -36,9 +37,13 @@ struct CompactFrags {
 
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
-__global__ void expand_frags(CompactFrags c, int nks, double *dst) {
+// dense == 0: expand nks k-steps of compact fragments; dense == 1: copy c.v[0 .. 63] (the
+// dense weights, zero-padded) to the first 64 entries
+__global__ void expand_frags(CompactFrags c, int nks, int dense, double *dst) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nks * 64) {
+  if (dense) {
+    if (i < 64) dst[i] = c.v[i];
+  } else if (i < nks * 64) {
     const int ks = i >> 6, lane = i & 63;
     dst[i] = c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)];
   }
@@ -87,31 +92,38 @@ extern "C" cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil
   const int shape = classify_2d(R, st->weights);
   const cs::Variant *vmain = nullptr, *vone = nullptr, *vrem[32] = {};
   CS_TRY(cs::pick_fp64_variants(2, R, shape, o.fusion_depth, o.tile_variant, &vmain, &vone, vrem, 32));
-  const int nks = cs::bfrag_ksteps_2d_f64(vmain->mf, R, vmain->shape);
-  if (vmain->mf != 4 || nks > 16 || nks * 64 > cs::kScratchDoubles) {
+  // the dilated tables (mf 4: B fragments) or the register-streaming kernels (mf 7: dense
+  // weights); every launch of a run comes from the main variant's family
+  const bool dense = vmain->mf == 7;
+  const int D = 2 * R + 1;
+  const int nks = dense ? 1 : cs::bfrag_ksteps_2d_f64(vmain->mf, R, vmain->shape);
+  if ((vmain->mf != 4 && !dense) || nks > 16 || nks * 64 > cs::kScratchDoubles) {
     fprintf(stderr, "convstencil: cs_run_2d_noalloc: %d k-steps exceed the scratch array (%d doubles)\n", nks,
             cs::kScratchDoubles);
     return CS_ERR_UNSUPPORTED;
   }
-  // B fragments on the host (a fixed array: nothing is allocated), compacted
-  double full[16 * 64];
+  // weight operands on the host (fixed arrays: nothing is allocated), compacted
   double wd[49];
-  const int D = 2 * R + 1;
   for (int i = 0; i < D * D; ++i) wd[i] = st->weights[i];
-  cs::build_bfrag_2d_f64(4, R, vmain->shape, wd, full);
   cs::CompactFrags c{};
-  for (int ks = 0; ks < nks; ++ks)
-    for (int lane = 0; lane < 64; ++lane) {
-      const double v = full[ks * 64 + lane], e = c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)];
-      if ((lane & 12) == 0)
-        c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)] = v;
-      else if (v != e)
-        return CS_ERR_DEVICE;  // the tables' lane structure changed: never expand wrongly
-    }
+  if (dense) {
+    for (int i = 0; i < D * D; ++i) c.v[i] = wd[i];
+  } else {
+    double full[16 * 64];
+    cs::build_bfrag_2d_f64(4, R, vmain->shape, wd, full);
+    for (int ks = 0; ks < nks; ++ks)
+      for (int lane = 0; lane < 64; ++lane) {
+        const double v = full[ks * 64 + lane], e = c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)];
+        if ((lane & 12) == 0)
+          c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)] = v;
+        else if (v != e)
+          return CS_ERR_DEVICE;  // the tables' lane structure changed: never expand wrongly
+      }
+  }
   double *frags = nullptr;
   CS_TRY(cs::scratch_ptr(dev, &frags));
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(cs::expand_frags, dim3((nks * 64 + 255) / 256), dim3(256), 0, s, c, nks, frags);
+  hipLaunchKernelGGL(cs::expand_frags, dim3((nks * 64 + 255) / 256), dim3(256), 0, s, c, nks, dense ? 1 : 0, frags);
   CS_HIP(hipGetLastError());
   for (const cs::Variant *u : {vmain, vone}) CS_HIP(u->prepare());
   // the reference's ping-pong: fused launches are exact when both buffers carry the same

@@ -37,7 +37,7 @@ struct Reg2D {
 constexpr int mod3(int v) { return ((v % 3) + 3) % 3; }
 
 
-template <int SH, int PX, int KF, bool EDGE>
+template <int SH, int PX, int KF, int RING, bool EDGE>
 struct Sweep2D {
   using C = Reg2D<PX, KF>;
   static constexpr bool BOX = SH == SHAPE_BOX;
@@ -47,7 +47,7 @@ struct Sweep2D {
   double w[9];
   int64_t ny, nx, pitch, ey, oy, ox, cx;
   bool colin[PX], colarr[PX], colout[PX];
-  double ring[3][PX];
+  double ring[RING][PX];
   double acc[KF][3][PX];
 
   __device__ __forceinline__ void load_row(int64_t r, double (&dst)[PX]) {
@@ -69,22 +69,22 @@ struct Sweep2D {
     }
   }
 
-  // Iteration `it + K` (K = it mod 3): input row r enters level 1, every level passes its
-  // completed row on, level KF's completed row r - KF is stored.  ring[K] holds row r,
-  // ring[K + 1] row r + 1 (in flight); row r + 2 is loaded into ring[K + 2], whose row r - 1
-  // was consumed by the previous iteration.
+  // Iteration `it + K` (it a multiple of the period): input row r enters level 1, every
+  // level passes its completed row on, level KF's completed row r - KF is stored.  The
+  // ring holds rows r .. r + RING - 2 (in flight); row r + RING - 1 is loaded into the
+  // entry whose row r - 1 the previous iteration consumed.
   template <int K>
   __device__ __forceinline__ void phase(int64_t it, int64_t ys, int64_t y0, int64_t y1) {
     const int64_t r = ys + it + K;
-    load_row(r + 2, ring[mod3(K + 2)]);
+    load_row(r + RING - 1, ring[(K + RING - 1) % RING]);
     double q[PX];
 #pragma unroll
-    for (int j = 0; j < PX; ++j) q[j] = ring[K][j];
+    for (int j = 0; j < PX; ++j) q[j] = ring[K % RING][j];
 #pragma unroll
     for (int s = 0; s < KF; ++s) {
       // incoming: level s, row yq; accumulators of level s + 1 for rows yq - 1, yq, yq + 1
       const int64_t yq = r - s;
-      const int sA = mod3(K - s - 1), sB = mod3(K - s), sC = mod3(K - s + 1);
+      const int sA = mod3(K % 3 - s - 1), sB = mod3(K % 3 - s), sC = mod3(K % 3 - s + 1);
       double(&A)[PX] = acc[s][sA];
       double(&B)[PX] = acc[s][sB];
       double(&Cn)[PX] = acc[s][sC];
@@ -156,21 +156,26 @@ struct Sweep2D {
       for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int j = 0; j < PX; ++j) acc[s][k][j] = 0.0;
-    load_row(ys, ring[0]);
-    load_row(ys + 1, ring[1]);
-    for (int64_t it = 0; it < iters; it += 3) {
-      phase<0>(it, ys, y0, y1);
-      phase<1>(it, ys, y0, y1);
-      phase<2>(it, ys, y0, y1);
+#pragma unroll
+    for (int k = 0; k < RING - 1; ++k) load_row(ys + k, ring[k]);
+    // the accumulator slots repeat every 3 iterations, the ring every RING
+    for (int64_t it = 0; it < iters; it += reg_period(RING)) phases<0>(it, ys, y0, y1);
+  }
+
+  template <int K>
+  __device__ __forceinline__ void phases(int64_t it, int64_t ys, int64_t y0, int64_t y1) {
+    if constexpr (K < reg_period(RING)) {
+      phase<K>(it, ys, y0, y1);
+      phases<K + 1>(it, ys, y0, y1);
     }
   }
 };
 
-template <int SH, int PX, int KF, bool EDGE>
+template <int SH, int PX, int KF, int RING, bool EDGE>
 __device__ __forceinline__ void sweep(const double *__restrict__ in, double *__restrict__ out, const double *wts,
                                       const Geo &g, int64_t xs, int64_t y0, int64_t y1, int lane) {
   using C = Reg2D<PX, KF>;
-  Sweep2D<SH, PX, KF, EDGE> S;
+  Sweep2D<SH, PX, KF, RING, EDGE> S;
   S.in = in;
   S.out = out;
 #pragma unroll
@@ -192,8 +197,8 @@ __device__ __forceinline__ void sweep(const double *__restrict__ in, double *__r
   S.run(y0, y1);
 }
 
-template <int SH, int PX, int KF, int NW>
-__global__ __launch_bounds__(NW * 64) void k2d_reg(const double *__restrict__ in, double *__restrict__ out,
+template <int SH, int PX, int KF, int RING, int NW, int WPE>
+__global__ __launch_bounds__(NW * 64, WPE) void k2d_reg(const double *__restrict__ in, double *__restrict__ out,
                                                    const double *__restrict__ wts, Geo g, RegSplit sp) {
   using C = Reg2D<PX, KF>;
   const int lane = threadIdx.x & 63;
@@ -201,62 +206,73 @@ __global__ __launch_bounds__(NW * 64) void k2d_reg(const double *__restrict__ in
   int64_t strip, y0, y1;
   if (!reg_chunk(sp, g.n[1], wid, &strip, &y0, &y1)) return;
   const int64_t xs = strip * C::OX - C::H;  // interior column of the strip's first cell
-  // rows the sweep loads or freezes: y0 - KF - 1 .. y1 + KF + 4 (iterations in blocks of 3,
-  // loads 2 rows ahead)
-  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 - C::H - 1 >= 0 && y1 + C::H + 6 <= g.n[1])
-    sweep<SH, PX, KF, false>(in, out, wts, g, xs, y0, y1, lane);
+  // the plain path: every row the sweep freezes or loads (rows y0 - KF .. y1 + KF + period +
+  // RING - 3) and every column of the strip interior
+  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 >= reg_least(KF, RING) && y1 + reg_least(KF, RING) <= g.n[1])
+    sweep<SH, PX, KF, RING, false>(in, out, wts, g, xs, y0, y1, lane);
   else
-    sweep<SH, PX, KF, true>(in, out, wts, g, xs, y0, y1, lane);
+    sweep<SH, PX, KF, RING, true>(in, out, wts, g, xs, y0, y1, lane);
 }
 
 // Resident waves of one variant on this device (workgroups per CU x waves x CUs), set
 // by the variant's prepare() before its first launch.
-template <int SH, int PX, int KF, int NW>
+template <int SH, int PX, int KF, int RING, int NW, int WPE>
 struct RegSlots {
   static int64_t value;
 };
-template <int SH, int PX, int KF, int NW>
-int64_t RegSlots<SH, PX, KF, NW>::value = 0;
+template <int SH, int PX, int KF, int RING, int NW, int WPE>
+int64_t RegSlots<SH, PX, KF, RING, NW, WPE>::value = 0;
 
-template <int SH, int PX, int KF, int NW>
+template <int SH, int PX, int KF, int RING, int NW, int WPE>
 static hipError_t prepare_reg2d() {
   int nb = 0, dev = 0, cus = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k2d_reg<SH, PX, KF, NW>, NW * 64, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, (const void *)k2d_reg<SH, PX, KF, RING, NW, WPE>, NW * 64, 0);
   if (e != hipSuccess) return e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-  RegSlots<SH, PX, KF, NW>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
+  RegSlots<SH, PX, KF, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
   return hipSuccess;
 }
 
-template <int SH, int PX, int KF, int NW>
+template <int SH, int PX, int KF, int RING, int NW, int WPE>
 static hipError_t launch_reg2d(const double *in, double *out, const double *wts, const Geo &g, hipStream_t s) {
   using C = Reg2D<PX, KF>;
-  const int64_t slots = RegSlots<SH, PX, KF, NW>::value > 0 ? RegSlots<SH, PX, KF, NW>::value : 2048;
-  const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, KF, slots, kRegEdgeCost);
+  using S = RegSlots<SH, PX, KF, RING, NW, WPE>;
+  const int64_t slots = S::value > 0 ? S::value : 2048;
+  const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, KF, slots, kRegEdgeCost, reg_least(KF, RING));
   const int64_t waves = reg_waves(sp);
-  hipLaunchKernelGGL((k2d_reg<SH, PX, KF, NW>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0, s, in, out,
-                     wts, g, sp);
+  hipLaunchKernelGGL((k2d_reg<SH, PX, KF, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
+                     s, in, out, wts, g, sp);
   return hipGetLastError();
 }
 
 // Variant::mf 7 marks this family: by = 1 (rows are chunked per launch), bx = output
-// columns per wave, px = columns per lane; the weights are the dense 3 x 3 block.
-#define CS_VR2D(ID, SH, PX, KF, NW)                                                                         \
-  Variant {                                                                                                 \
-    ID, 7, 1, SH, KF, 1, Reg2D<PX, KF>::OX, NW, 0, PX, 0, &launch_reg2d<SH, PX, KF, NW>,                    \
-        &prepare_reg2d<SH, PX, KF, NW>, "k2d_reg<r1," #SH ",p" #PX ",kf" #KF ",w" #NW ">"                  \
+// columns per wave, px = columns per lane, st = rows in the load ring; WPE = the waves per
+// SIMD the compiler is held to (registers); the weights are the dense 3 x 3 block.
+#define CS_VR2D(ID, SH, PX, KF, RING, WPE)                                                                     \
+  Variant {                                                                                                    \
+    ID, 7, 1, SH, KF, 1, Reg2D<PX, KF>::OX, 4, 0, PX, RING, &launch_reg2d<SH, PX, KF, RING, 4, WPE>,          \
+        &prepare_reg2d<SH, PX, KF, RING, 4, WPE>,                                                             \
+        "k2d_reg<r1," #SH ",p" #PX ",kf" #KF ",ring" #RING ",wpe" #WPE ">"                                   \
   }
 
-// ids 700000 + (star) * 10000 + PX * 100 + KF
+// ids 700000 + (ring 6) * 50000 + (star) * 10000 + PX * 100 + KF.  Measured at 10240^2
+// (GStencil/s over 64-step runs, one box): box p2 kf12 ring 6 (2 waves per SIMD) 1902,
+// p2 kf12 ring 3 (3 waves) 1827, p2 kf8 ring 6 (3 waves) 1722, p2 kf16 ring 6 1659, p4 kf8
+// ring 3 (2 waves) 1428; star p2 kf12 ring 3 2428.  Fewer columns per lane cost more
+// recomputed columns and lane shifts per cell but leave registers for more waves and a
+// deeper load ring, which the FP64 FMA issue needs: one wave issues an FP64 FMA about every
+// 12 cycles, two waves about every 6.
 static const Variant kVariantsReg2D[] = {
-    CS_VR2D(700408, SHAPE_BOX, 4, 8, 4),   CS_VR2D(700406, SHAPE_BOX, 4, 6, 4),
-    CS_VR2D(700410, SHAPE_BOX, 4, 10, 4),  CS_VR2D(700212, SHAPE_BOX, 2, 12, 4),
-    CS_VR2D(700216, SHAPE_BOX, 2, 16, 4),  CS_VR2D(700404, SHAPE_BOX, 4, 4, 4),
-    CS_VR2D(700402, SHAPE_BOX, 4, 2, 4),   CS_VR2D(700401, SHAPE_BOX, 4, 1, 4),
-    CS_VR2D(710408, SHAPE_STAR, 4, 8, 4),  CS_VR2D(710406, SHAPE_STAR, 4, 6, 4),
-    CS_VR2D(710404, SHAPE_STAR, 4, 4, 4),  CS_VR2D(710402, SHAPE_STAR, 4, 2, 4),
-    CS_VR2D(710401, SHAPE_STAR, 4, 1, 4),
+    CS_VR2D(750212, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, SHAPE_BOX, 2, 12, 3, 3),
+    CS_VR2D(750208, SHAPE_BOX, 2, 8, 6, 3),   CS_VR2D(750216, SHAPE_BOX, 2, 16, 6, 2),
+    CS_VR2D(700408, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700404, SHAPE_BOX, 4, 4, 3, 2),
+    CS_VR2D(700402, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, SHAPE_BOX, 4, 1, 3, 2),
+    CS_VR2D(710212, SHAPE_STAR, 2, 12, 3, 3), CS_VR2D(760212, SHAPE_STAR, 2, 12, 6, 2),
+    CS_VR2D(760216, SHAPE_STAR, 2, 16, 6, 2), CS_VR2D(760208, SHAPE_STAR, 2, 8, 6, 3),
+    CS_VR2D(710404, SHAPE_STAR, 4, 4, 3, 2),  CS_VR2D(710402, SHAPE_STAR, 4, 2, 3, 2),
+    CS_VR2D(710401, SHAPE_STAR, 4, 1, 3, 2),
 };
 
 const Variant *variants_2d_reg(int *count) {

@@ -21,7 +21,8 @@ extern "C" {
  * FP64 kernels (two LDS tile buffers, producer waves "p<n>" in the name; info[8] = both
  * buffers' bytes); 7 = the register-resident 1-D FP64 kernels (info[1] = 6, info[6] = outputs
  * per workgroup, info[9] = cells per lane); 8 = the register-streaming 2-D FP64 kernels
- * (info[1] = 7, info[6] = output columns per wave, info[9] = columns per lane). */
+ * (info[1] = 7, info[6] = output columns per wave, info[9] = columns per lane, info[10] =
+ * rows in the load ring). */
 int32_t cs_debug_variant_count(int32_t ndim);
 /* info[0..11] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
  * tile rows, tile cols (1-D: tile length), waves, LDS bytes, LDS row pitch / buffer length
@@ -80,11 +81,12 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
 int32_t cs_debug_ctable_map(int32_t radius, int32_t shape);
 
 /* Wave decomposition of one launch of the register-streaming 2-D kernels (family 8): for
- * ny x nx interior cells, `ox` output columns per strip, fusion depth kf and `slots` resident
- * waves, split[0..4] = strips, he, hm, we, wi (rows per checked chunk, rows per plain chunk,
- * chunks per edge strip, chunks per other strip); waves = the edge strips' chunks first.
- * Returns 0, or -1 for invalid arguments. */
-int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t kf, int64_t slots, int64_t *split);
+ * ny x nx interior cells, `ox` output columns per strip, fusion depth kf, a load ring of
+ * `ring` rows and `slots` resident waves, split[0..4] = strips, he, hm, we, wi (rows per
+ * checked chunk, rows per plain chunk, chunks per edge strip, chunks per other strip); waves
+ * = the edge strips' chunks first.  Returns 0, or -1 for invalid arguments. */
+int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t kf, int32_t ring, int64_t slots,
+                             int64_t *split);
 
 /* Wave `wid` of that decomposition: rows[0..2] = strip, first row, end row; returns 1, or 0 for
  * a wave without work. */

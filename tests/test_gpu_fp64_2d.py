@@ -228,7 +228,7 @@ def test_every_reg_variant(gpu, vid, vname, shape, kf, ox):
     check(L, got, oracle.run(L, offs, w, x, kf + 2, threads=0))
 
 
-@pytest.mark.parametrize("vid", [0, 811106, 700408, 700216])
+@pytest.mark.parametrize("vid", [0, 811106, 750212, 700408, 711106])
 def test_full_size_box2d1r_windows_variants(gpu, vid):
     # the measured configuration (box2d1r 10240^2, halo 1, T = 64) with the pipelined
     # kernel forced (0: the plan's default choice)

@@ -1640,16 +1640,16 @@ def test_variant_emulation_stream(v):
 
 # ---------------------------------------------------------------- 2-D register-streaming family
 
-def reg2d_split(ny, nx, ox, kf, slots):
+def reg2d_split(ny, nx, ox, kf, ring, slots):
     lib = _dbg()
     i64p = ctypes.POINTER(ctypes.c_int64)
     lib.cs_debug_reg2d_split.restype = ctypes.c_int32
     lib.cs_debug_reg2d_split.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
-                                         ctypes.c_int64, i64p]
+                                         ctypes.c_int32, ctypes.c_int64, i64p]
     lib.cs_debug_reg2d_chunk.restype = ctypes.c_int32
     lib.cs_debug_reg2d_chunk.argtypes = [i64p, ctypes.c_int64, ctypes.c_int64, i64p]
     split = (ctypes.c_int64 * 5)()
-    assert lib.cs_debug_reg2d_split(ny, nx, ox, kf, slots, split) == 0
+    assert lib.cs_debug_reg2d_split(ny, nx, ox, kf, ring, slots, split) == 0
     strips, he, hm, we, wi = list(split)
     ne = 2 if strips >= 2 else 1
     chunks = []
@@ -1660,15 +1660,22 @@ def reg2d_split(ny, nx, ox, kf, slots):
     return list(split), chunks
 
 
+def reg_period(ring):
+    return ring if ring % 3 == 0 else 3 * ring
+
+
 def emulate_2d_reg(v, layout, dense, x, slots):
     """k2d_reg: each wave sweeps rows [y0, y1) of a strip of 64 PX columns (lane l: columns
     l PX .. l PX + PX - 1, so that a cell's column neighbours are the adjacent cells of the
-    flat lane-major row, zero past lanes 0 and 63), one input row per iteration in blocks of
-    three, level s + 1 holding partial sums for rows yq - 1, yq, yq + 1 in slots (K - s - 1,
+    flat lane-major row, zero past lanes 0 and 63), one input row per iteration in unrolled
+    blocks of the ring's period, rows loaded RING - 1 iterations ahead into a ring of RING
+    rows, level s + 1 holding partial sums for rows yq - 1, yq, yq + 1 in slots (K - s - 1,
     K - s, K - s + 1) mod 3, cells outside the interior frozen on the checked path, level KF's
     completed row stored on the output columns.  Checks loads and stores against the array
     and counts the stores per cell."""
-    PX, KF = v["px"], v["kf"]
+    PX, KF, RING = v["px"], v["kf"], v["st"]
+    PER = reg_period(RING)
+    least = KF + PER + RING - 2
     box = v["shape"] == 0
     H = KF
     OX = 64 * PX - 2 * H
@@ -1680,12 +1687,14 @@ def emulate_2d_reg(v, layout, dense, x, slots):
     w = np.asarray(dense, dtype=np.float64).reshape(3, 3)
     out = np.zeros((ey, pitch))
     hits = np.zeros((ey, pitch), dtype=np.int64)
-    _, chunks = reg2d_split(ny, nx, OX, KF, slots)
+    _, chunks = reg2d_split(ny, nx, OX, KF, RING, slots)
     d = np.arange(64 * PX)
+    plain = 0
     for strip, y0, y1 in chunks:
         xs = strip * OX - H
         cols = xs + d
-        fast = xs >= 0 and xs + 64 * PX <= nx and y0 - H - 1 >= 0 and y1 + H + 6 <= ny
+        fast = xs >= 0 and xs + 64 * PX <= nx and y0 >= least and y1 + least <= ny
+        plain += fast
         colin = (cols >= 0) & (cols < nx)
         colarr = (ox + cols >= 0) & (ox + cols < pitch)
         colout = (d >= H) & (d < H + OX) & (cols < nx)
@@ -1700,15 +1709,15 @@ def emulate_2d_reg(v, layout, dense, x, slots):
 
         ys, iters = y0 - H, (y1 - y0) + 2 * H
         acc = np.zeros((KF, 3, 64 * PX))
-        ring = [load(ys), load(ys + 1), None]
-        for it in range(0, iters, 3):
-            for K in range(3):
+        ring = [load(ys + k) for k in range(RING - 1)] + [None]
+        for it in range(0, iters, PER):
+            for K in range(PER):
                 r = ys + it + K
-                ring[(K + 2) % 3] = load(r + 2)
-                q = ring[K].copy()
+                ring[(K + RING - 1) % RING] = load(r + RING - 1)
+                q = ring[K % RING].copy()
                 for s in range(KF):
                     yq = r - s
-                    sA, sB, sC = (K - s - 1) % 3, (K - s) % 3, (K - s + 1) % 3
+                    sA, sB, sC = (K % 3 - s - 1) % 3, (K % 3 - s) % 3, (K % 3 - s + 1) % 3
                     qe = np.concatenate([[0.0], q, [0.0]])
                     lf, mid, rt = qe[:-2], q, qe[2:]
                     A, B = acc[s, sA], acc[s, sB]
@@ -1731,6 +1740,7 @@ def emulate_2d_reg(v, layout, dense, x, slots):
     inner = hits[oy:oy + ny, ox:ox + nx]
     assert (inner == 1).all(), (int(inner.min()), int(inner.max()))
     assert hits.sum() == inner.sum()
+    emulate_2d_reg.plain_chunks = plain
     return out.reshape(-1)
 
 
@@ -1750,23 +1760,26 @@ def test_variant_emulation_2d_reg(v):
     dense = dense_weights(2, R, offs, w)
     cases = [(Layout.with_halo([6 * kf + 40, 2 * OX + 37], 1), 6),
              (Layout(2, [4 * kf + 31, OX // 2 + 12], [2, 3], [4 * kf + 26, OX // 2 + 5]), 3),
-             (Layout.with_halo([9 * kf + 25, OX + 11], 1), 64)]
-    for layout, slots in cases:
+             (Layout.with_halo([9 * kf + 25, OX + 11], 1), 64),
+             (Layout.with_halo([16 * kf + 120, 3 * OX], 1), 15)]
+    for i, (layout, slots) in enumerate(cases):
         x = oracle.fill_data(layout, v["id"] + slots)
         got = emulate_2d_reg(v, layout, dense, x, slots)
+        if i == 3:
+            assert emulate_2d_reg.plain_chunks > 0  # the plain path ran
         want = oracle.run(layout, offs, w, x, kf)
         gi, wi = layout.interior(got), layout.interior(want)
         err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
         assert err < 1e-13, (v["name"], slots, err)
 
 
-@pytest.mark.parametrize("ny,nx,ox,kf,slots", [(10240, 10240, 240, 8, 2048), (8192, 8192, 240, 8, 2048),
-                                               (100, 3000, 240, 8, 2048), (5000, 200, 96, 16, 2048),
-                                               (31, 31, 254, 1, 4096), (1, 7, 240, 8, 16)])
-def test_reg2d_split_covers_once(ny, nx, ox, kf, slots):
+@pytest.mark.parametrize("ny,nx,ox,kf,ring,slots", [(10240, 10240, 104, 12, 6, 2048), (8192, 8192, 240, 8, 3, 2048),
+                                                    (100, 3000, 240, 8, 3, 2048), (5000, 200, 96, 16, 6, 2048),
+                                                    (31, 31, 254, 1, 3, 4096), (1, 7, 240, 8, 6, 16)])
+def test_reg2d_split_covers_once(ny, nx, ox, kf, ring, slots):
     # every row of every strip in exactly one chunk, never more waves than slots when the
     # strips fit
-    split, chunks = reg2d_split(ny, nx, ox, kf, slots)
+    split, chunks = reg2d_split(ny, nx, ox, kf, ring, slots)
     strips, he, hm, we, wi = split
     ne = 2 if strips >= 2 else 1
     waves = ne * we + (strips - ne) * wi
@@ -1776,5 +1789,5 @@ def test_reg2d_split_covers_once(ny, nx, ox, kf, slots):
         assert 0 <= y0 < y1 <= ny
         cover[s, y0:y1] += 1
     assert (cover == 1).all()
-    # a plain-path chunk needs kf + 1 rows above and kf + 5 below it inside the grid
-    assert he >= kf + 6
+    # a plain-path chunk needs the rows it freezes and loads inside the grid
+    assert he >= kf + reg_period(ring) + ring - 2
