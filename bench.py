@@ -11,9 +11,10 @@ HBM; GStencil/s counts interior points x time steps / s, as the reference does
 Prints one JSON line.  `roofline` describes the dominant kernel: algorithmic
 bytes (8 B read + 8 B written per interior point per launch) over its mean
 launch time, measured with HIP events on the plan stream around a run made of
-that kernel's launches only; beside it the executed MFMA work per launch from
-the committed rocprofv3 counter summary (profiles/counters.json) over the same
-launch time, against the FP64 MFMA peak, and `traffic`, the measured HBM bytes
+that kernel's launches only; beside it the executed MFMA work (or, for the
+vector-FMA kernels, the executed FP64 VALU work) per launch from the committed
+rocprofv3 counter summary (profiles/counters.json) over the same launch time,
+against the FP64 peak, and `traffic`, the measured HBM bytes
 per launch (profiles/traffic.json), or null.  `cpu_baseline` is the CPU oracle
 (the reference's loop, restated) timed on this host's cores on a bounded sample
 of the same workload.  `extras` carries every other configuration of the
@@ -387,6 +388,15 @@ def main():
         roof["mfma"] = {"executed_tflops": round(ach, 2), "peak_tflops": peak, "frac": round(ach / peak, 4),
                         "executed_flops_per_launch": cnt["mfma_flops_per_launch"], "source": cnt.get("source")}
         for k in ("lds_bank_conflict_frac", "valu_insts_per_launch", "l2_hit_frac"):
+            if k in cnt:
+                roof[k] = cnt[k]
+    if cnt and cnt.get("fp64_valu_flops_per_launch"):
+        # vector-FMA kernels: executed FP64 VALU work against the FP64 vector peak
+        ach = float(cnt["fp64_valu_flops_per_launch"]) / (ms_launch * 1e-3) / 1e12
+        roof["fp64_valu"] = {"executed_tflops": round(ach, 2), "peak_tflops": FP64_PEAK_TFLOPS,
+                             "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                             "executed_flops_per_launch": cnt["fp64_valu_flops_per_launch"], "source": cnt.get("source")}
+        for k in ("valu_insts_per_launch", "l2_hit_frac"):
             if k in cnt:
                 roof[k] = cnt[k]
 
