@@ -152,10 +152,10 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
   return t.n;
 }
 
-int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t kf, int32_t ring, int64_t slots,
-                             int64_t *split) {
-  if (ny < 1 || nx < 1 || ox < 1 || kf < 1 || ring < 2 || slots < 1 || !split) return -1;
-  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, kf, slots, cs::kRegEdgeCost, cs::reg_least(kf, ring));
+int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t radius, int32_t kf, int32_t ring,
+                             int64_t slots, int64_t *split) {
+  if (ny < 1 || nx < 1 || ox < 1 || radius < 1 || kf < 1 || ring < 2 || slots < 1 || !split) return -1;
+  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, radius * kf, slots, cs::kRegEdgeCost, cs::reg_least(radius, kf, ring));
   const int64_t v[5] = {sp.strips, sp.he, sp.hm, sp.we, sp.wi};
   memcpy(split, v, sizeof(v));
   return 0;

@@ -153,8 +153,8 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
     if (!v) v = find_variant(nd, 0, R, cs::SHAPE_BOX, 0, tile_variant, prec);
   } else if (prec == CS_PREC_FP64) {
     // 1-D: the register-resident kernels (mf 6) at their deepest compiled fusion, else the
-    // 4 x 4 x 4 blocks; 2-D: the register-streaming kernels (mf 7, radius 1) at depth 12,
-    // else the dilated 8 x 8 blocks; 3-D: the 8 x 8 block family (mf 8) where it exists,
+    // 4 x 4 x 4 blocks; 2-D: the register-streaming kernels (mf 7), else the dilated 8 x 8
+    // blocks; 3-D: the 8 x 8 block family (mf 8) where it exists,
     // else the register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips
     const int fams[3] = {nd == 3 ? 8 : nd == 1 ? 6 : 7, nd == 3 ? 3 : mf, mf};
     // a requested depth that some family compiles exactly is honoured exactly
@@ -165,9 +165,12 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       if (v) break;
       const int top = max_kf(nd, fmf, R, shape, prec);
       if (top == 0) continue;
+      // register kernels (measured at 10240^2 / 8192^2, 64 steps): 2-D radius 1 depth 8
+      // (1662-1745 GStencil/s; depth 12 1617-1902), star2d3r depth 4 (773; 3: 627, 6: 499),
+      // box2d3r depth 2 (330; 3: 315, the dilated tables 300); 1-D the deepest compiled
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
                : fmf == 6       ? top
-               : fmf == 7       ? std::min(top, 12)
+               : fmf == 7       ? std::min(top, R == 1 ? 8 : shape == cs::SHAPE_STAR ? 4 : 2)
                                 : default_kf(nd, R, top, shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
       if (v) break;

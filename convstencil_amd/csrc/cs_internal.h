@@ -126,19 +126,21 @@ __host__ __device__ inline bool reg_chunk(const RegSplit &sp, int64_t ny, int64_
 // relative to the plain path, per row (measured: 2.3-2.6 balances the two at 10240^2).
 constexpr double kRegEdgeCost = 2.5;
 
-// Iterations per unrolled block of a sweep with a load ring of `ring` rows (the partial-sum
-// slots repeat every 3), and the rows a plain-path chunk needs above it and below it.
-__host__ __device__ constexpr int reg_period(int ring) { return ring % 3 == 0 ? ring : 3 * ring; }
-__host__ __device__ constexpr int reg_least(int kf, int ring) { return kf + reg_period(ring) + ring - 2; }
+// Iterations per unrolled block of a radius-R sweep with a load ring of `ring` rows (the
+// 2R + 1 partial-sum slots and the ring both repeat), and the rows a plain-path chunk needs
+// above it and below it inside the grid.
+__host__ __device__ constexpr int reg_gcd(int a, int b) { return b == 0 ? a : reg_gcd(b, a % b); }
+__host__ __device__ constexpr int reg_period(int R, int ring) { return (2 * R + 1) / reg_gcd(2 * R + 1, ring) * ring; }
+__host__ __device__ constexpr int reg_least(int R, int kf, int ring) { return R * kf + reg_period(R, ring) + ring - 2; }
 
 // The decomposition for a grid of ny x nx interior cells, `ox` output columns per strip,
-// fusion depth kf and `slots` resident waves: the smallest per-wave cost (rows plus the 2 kf
-// warm-up rows, checked rows counted `edge_cost` times) at which every wave is resident at
+// h = R kf warm-up rows on either side and `slots` resident waves: the smallest per-wave cost
+// (rows plus the 2 h warm-up rows, checked rows counted `edge_cost` times) at which every wave is resident at
 // once, so that no wave waits for a second round; checked chunks are at least `least` rows.
-inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int kf, int64_t slots, double edge_cost, int least) {
+inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int h, int64_t slots, double edge_cost, int least) {
   RegSplit sp{};
   sp.strips = (nx + ox - 1) / ox;
-  const int64_t warm = 2 * (int64_t)kf;
+  const int64_t warm = 2 * (int64_t)h;
   auto make = [&](int64_t cost) {
     RegSplit t = sp;
     t.hm = cost - warm;

@@ -1,54 +1,58 @@
-// k2d_reg.hip -- register-streaming FP64 2-D kernels for gfx950 (radius 1, box and star).
+// k2d_reg.hip -- register-streaming FP64 2-D kernels for gfx950 (box and star, radius 1-3).
 //
-// Replaces kernel2d (reference src/2d/gpu.cu:30-70) for the radius-1 time loop
-// (src/2d/gpu.cu:316-329).  On gfx950 FP64 vector FMAs and FP64 MFMAs share one peak
-// (128 flops per clock and CU); the dilated 8 x 8 MFMA tables execute about twice the
-// useful multiply-adds of a 9-point stencil, while vector FMAs execute exactly those.
+// Replaces kernel2d (reference src/2d/gpu.cu:30-70) and its time loop (src/2d/gpu.cu:316-329,
+// 336-425) where vector FMAs beat the MFMA tables.  On gfx950 FP64 vector FMAs and FP64 MFMAs
+// share one peak (128 flops per clock and CU); the dilated 8 x 8 MFMA tables execute about
+// twice the useful multiply-adds of a 9-point box and more for the stars, while vector FMAs
+// execute exactly the stencil's taps.
 //
 // A wave owns a strip of 64 PX columns (lane l: columns l PX .. l PX + PX - 1) and sweeps a
 // chunk of rows downwards, one input row per iteration, advancing every row KF steps on
-// the way.  Level s + 1 of the time loop keeps partial sums for the rows that the
-// incoming level-s row contributes to: an incoming row y adds its dy = +1 taps to row
-// y - 1 (which is then complete and becomes the incoming row of level s + 2), its dy = 0
-// taps to row y and starts row y + 1 with its dy = -1 taps.  Per level two partial rows
-// persist between iterations; the incoming rows themselves are never stored.  The column
-// neighbours of a lane's run come from lanes l -+ 1 (v_mov_b32_dpp wave shifts; lanes 0
-// and 63 receive zeros), so the valid columns shrink by one per level from each end of
-// the strip: the central 64 PX - 2 KF columns are the outputs and strips overlap by 2 KF
-// recomputed columns.  A chunk starts KF rows above its first output row and ends KF
-// rows below its last.  Cells outside the interior keep their value at every level (the
-// reference never writes the halo): when such a row or column arrives at a level its
-// value is taken as the level's result and no later tap changes it.  No LDS, no barrier.
+// the way.  Level s + 1 of the time loop keeps partial sums for the 2R + 1 rows that the
+// incoming level-s row y contributes to: its dy = +R taps complete row y - R (which then
+// becomes the incoming row of level s + 2), its dy = -R taps start row y + R, and the taps
+// in between add to the rows in between.  Per level 2R partial rows persist between
+// iterations; the incoming rows themselves are never stored.  The column neighbours of a
+// lane's run come from the lanes on either side (v_mov_b32_dpp wave shifts, repeated for
+// cells more than PX columns away; the lanes at the wave's ends receive zeros), so the
+// valid columns shrink by R per level from each end of the strip: the central 64 PX - 2 R KF
+// columns are the outputs and strips overlap by 2 R KF recomputed columns.  A chunk starts
+// R KF rows above its first output row and ends R KF rows below its last.  Cells outside
+// the interior keep their value at every level (the reference never writes the halo): when
+// such a row or column arrives at a level its value is taken as the level's result and no
+// later tap changes it.  No LDS, no barrier.
 //
-// Sum order per output: rows dy = -1, 0, +1, columns dx = -1, 0, +1 within each (the
+// Sum order per output: rows dy = -R .. R, columns dx = -R .. R within each (the
 // reference's order), as a chain of FMAs.
 #include "cs_internal.h"
 
 namespace cs {
 
-template <int PX, int KF>
+template <int R, int PX, int KF>
 struct Reg2D {
-  static constexpr int H = KF;                // recomputed columns per side; warm-up rows
+  static constexpr int H = R * KF;            // recomputed columns per side; warm-up rows
   static constexpr int OX = 64 * PX - 2 * H;  // output columns per wave
+  static constexpr int NS = 2 * R + 1;        // partial-sum rows per level
   static_assert(OX > 0, "strip too narrow for the fusion depth");
 };
 
+constexpr int modp(int v, int m) { return ((v % m) + m) % m; }
 
-constexpr int mod3(int v) { return ((v % 3) + 3) % 3; }
-
-
-template <int SH, int PX, int KF, int RING, bool EDGE>
+template <int R, int SH, int PX, int KF, int RING, bool EDGE>
 struct Sweep2D {
-  using C = Reg2D<PX, KF>;
+  using C = Reg2D<R, PX, KF>;
+  static constexpr int D = 2 * R + 1, NS = C::NS;
   static constexpr bool BOX = SH == SHAPE_BOX;
+  static constexpr int PER = reg_period(R, RING);
+  static constexpr int NL = (R + PX - 1) / PX;  // lanes a halo reaches across
 
   const double *__restrict__ in;
   double *__restrict__ out;
-  double w[9];
+  double w[D * D];
   int64_t ny, nx, pitch, ey, oy, ox, cx;
   bool colin[PX], colarr[PX], colout[PX];
   double ring[RING][PX];
-  double acc[KF][3][PX];
+  double acc[KF][NS][PX];
 
   __device__ __forceinline__ void load_row(int64_t r, double (&dst)[PX]) {
     const int64_t ay = oy + r;
@@ -69,8 +73,35 @@ struct Sweep2D {
     }
   }
 
+  // qe[R + c] = cell c of the lane's run, c = -R .. PX + R - 1: the run itself and the R cells
+  // on either side, which lie in the lanes up to NL away; lo / hi hold, after pass a, the
+  // cells of lanes l - a / l + a that are still needed
+  __device__ __forceinline__ void extend(const double (&q)[PX], double (&qe)[PX + 2 * R]) {
+    double lo[PX], hi[PX];
+#pragma unroll
+    for (int j = 0; j < PX; ++j) {
+      qe[R + j] = q[j];
+      lo[j] = q[j];
+      hi[j] = q[j];
+    }
+#pragma unroll
+    for (int a = 1; a <= NL; ++a) {
+#pragma unroll
+      for (int j = 0; j < PX; ++j) {
+        if (j >= a * PX - R) {  // cell j - a PX in [-R, -1]
+          lo[j] = lane_from_lower(lo[j]);
+          qe[R + j - a * PX] = lo[j];
+        }
+        if (j < PX + R - a * PX) {  // cell j + a PX in [PX, PX + R - 1]
+          hi[j] = lane_from_upper(hi[j]);
+          qe[R + j + a * PX] = hi[j];
+        }
+      }
+    }
+  }
+
   // Iteration `it + K` (it a multiple of the period): input row r enters level 1, every
-  // level passes its completed row on, level KF's completed row r - KF is stored.  The
+  // level passes its completed row on, level KF's completed row r - R KF is stored.  The
   // ring holds rows r .. r + RING - 2 (in flight); row r + RING - 1 is loaded into the
   // entry whose row r - 1 the previous iteration consumed.
   template <int K>
@@ -82,68 +113,65 @@ struct Sweep2D {
     for (int j = 0; j < PX; ++j) q[j] = ring[K % RING][j];
 #pragma unroll
     for (int s = 0; s < KF; ++s) {
-      // incoming: level s, row yq; accumulators of level s + 1 for rows yq - 1, yq, yq + 1
-      const int64_t yq = r - s;
-      const int sA = mod3(K % 3 - s - 1), sB = mod3(K % 3 - s), sC = mod3(K % 3 - s + 1);
-      double(&A)[PX] = acc[s][sA];
-      double(&B)[PX] = acc[s][sB];
-      double(&Cn)[PX] = acc[s][sC];
-      const double ql = lane_from_lower(q[PX - 1]), qr = lane_from_upper(q[0]);
-      double qe[PX + 2];
-      qe[0] = ql;
-      qe[PX + 1] = qr;
+      // incoming: level s, row yq; level s + 1's partial sum for row y sits in slot
+      // (y - ys) mod NS, and yq - ys = it + K - R s with `it` a multiple of NS
+      const int64_t yq = r - R * s;
+      double qe[PX + 2 * R];
+      extend(q, qe);
 #pragma unroll
-      for (int j = 0; j < PX; ++j) qe[j + 1] = q[j];
+      for (int dy = R; dy >= -R; --dy) {
+        // contribution of the taps of row offset dy to row yq - dy (dy = R completes it,
+        // dy = -R starts it)
+        double(&T)[PX] = acc[s][modp(K - R * s - dy, NS)];
+        const int64_t yt = yq - dy;
+        const bool frozen = EDGE && (yt < 0 || yt >= ny);
 #pragma unroll
-      for (int j = 0; j < PX; ++j) {
-        double a = A[j], b = B[j];
-        if constexpr (BOX) {
-          a = __builtin_fma(w[6], qe[j], a);
-          a = __builtin_fma(w[7], qe[j + 1], a);
-          a = __builtin_fma(w[8], qe[j + 2], a);
-        } else {
-          a = __builtin_fma(w[7], qe[j + 1], a);
+        for (int j = 0; j < PX; ++j) {
+          double t = dy == -R ? 0.0 : T[j];
+          bool first = dy == -R;
+#pragma unroll
+          for (int dx = -R; dx <= R; ++dx) {
+            if (!BOX && dy != 0 && dx != 0) continue;
+            const double wv = w[(dy + R) * D + dx + R], v = qe[R + j + dx];
+            t = first ? wv * v : __builtin_fma(wv, v, t);
+            first = false;
+          }
+          if constexpr (EDGE) {
+            // rows above yq arrived (and froze) earlier; row yq freezes now; rows below are
+            // overwritten when they arrive
+            if (dy > 0)
+              t = frozen || !colin[j] ? T[j] : t;
+            else if (dy == 0)
+              t = frozen || !colin[j] ? q[j] : t;
+          }
+          T[j] = t;
         }
-        b = __builtin_fma(w[3], qe[j], b);
-        b = __builtin_fma(w[4], qe[j + 1], b);
-        b = __builtin_fma(w[5], qe[j + 2], b);
-        if constexpr (EDGE) {
-          // a cell outside the interior keeps the value it arrived with: row yq - 1 (or this
-          // column) arrived frozen one iteration ago, row yq arrives now
-          const bool fa = yq - 1 < 0 || yq - 1 >= ny || !colin[j];
-          const bool fb = yq < 0 || yq >= ny || !colin[j];
-          A[j] = fa ? A[j] : a;
-          B[j] = fb ? q[j] : b;
-        } else {
-          A[j] = a;
-          B[j] = b;
-        }
-      }
+        // keep the partial sums computed here: left to itself the compiler sinks these FMAs
+        // to their next use and keeps every level's incoming row alive instead
+        if (dy < R) {
 #pragma unroll
-      for (int j = 0; j < PX; ++j) {
-        if constexpr (BOX) {
-          double c = w[0] * qe[j];
-          c = __builtin_fma(w[1], qe[j + 1], c);
-          Cn[j] = __builtin_fma(w[2], qe[j + 2], c);
-        } else {
-          Cn[j] = w[1] * qe[j + 1];
+          for (int j = 0; j < PX; ++j) asm volatile("" : "+v"(T[j]));
         }
       }
-      // keep the partial sums computed here: left to itself the compiler sinks these FMAs
-      // to their next use and keeps every level's incoming row alive instead
+      const double(&A)[PX] = acc[s][modp(K - R * s - R, NS)];
 #pragma unroll
-      for (int j = 0; j < PX; ++j) {
-        asm volatile("" : "+v"(B[j]), "+v"(Cn[j]));
-        q[j] = A[j];
-      }
+      for (int j = 0; j < PX; ++j) q[j] = A[j];
     }
-    // q: level KF, row r - KF
-    const int64_t yo = r - KF;
+    // q: level KF, row r - R KF
+    const int64_t yo = r - C::H;
     if (yo >= y0 && yo < y1) {
       double *p = out + (oy + yo) * pitch + ox + cx;
 #pragma unroll
       for (int j = 0; j < PX; ++j)
         if (colout[j]) p[j] = q[j];
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void phases(int64_t it, int64_t ys, int64_t y0, int64_t y1) {
+    if constexpr (K < PER) {
+      phase<K>(it, ys, y0, y1);
+      phases<K + 1>(it, ys, y0, y1);
     }
   }
 
@@ -153,33 +181,25 @@ struct Sweep2D {
 #pragma unroll
     for (int s = 0; s < KF; ++s)
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
+      for (int k = 0; k < NS; ++k)
 #pragma unroll
         for (int j = 0; j < PX; ++j) acc[s][k][j] = 0.0;
 #pragma unroll
     for (int k = 0; k < RING - 1; ++k) load_row(ys + k, ring[k]);
-    // the accumulator slots repeat every 3 iterations, the ring every RING
-    for (int64_t it = 0; it < iters; it += reg_period(RING)) phases<0>(it, ys, y0, y1);
-  }
-
-  template <int K>
-  __device__ __forceinline__ void phases(int64_t it, int64_t ys, int64_t y0, int64_t y1) {
-    if constexpr (K < reg_period(RING)) {
-      phase<K>(it, ys, y0, y1);
-      phases<K + 1>(it, ys, y0, y1);
-    }
+    // the partial-sum slots repeat every NS iterations, the ring every RING
+    for (int64_t it = 0; it < iters; it += PER) phases<0>(it, ys, y0, y1);
   }
 };
 
-template <int SH, int PX, int KF, int RING, bool EDGE>
+template <int R, int SH, int PX, int KF, int RING, bool EDGE>
 __device__ __forceinline__ void sweep(const double *__restrict__ in, double *__restrict__ out, const double *wts,
                                       const Geo &g, int64_t xs, int64_t y0, int64_t y1, int lane) {
-  using C = Reg2D<PX, KF>;
-  Sweep2D<SH, PX, KF, RING, EDGE> S;
+  using C = Reg2D<R, PX, KF>;
+  Sweep2D<R, SH, PX, KF, RING, EDGE> S;
   S.in = in;
   S.out = out;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) S.w[i] = wts[i];
+  for (int i = 0; i < (2 * R + 1) * (2 * R + 1); ++i) S.w[i] = wts[i];
   S.ny = g.n[1];
   S.nx = g.n[2];
   S.pitch = g.ext[2];
@@ -197,82 +217,89 @@ __device__ __forceinline__ void sweep(const double *__restrict__ in, double *__r
   S.run(y0, y1);
 }
 
-template <int SH, int PX, int KF, int RING, int NW, int WPE>
+template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
 __global__ __launch_bounds__(NW * 64, WPE) void k2d_reg(const double *__restrict__ in, double *__restrict__ out,
-                                                   const double *__restrict__ wts, Geo g, RegSplit sp) {
-  using C = Reg2D<PX, KF>;
+                                                        const double *__restrict__ wts, Geo g, RegSplit sp) {
+  using C = Reg2D<R, PX, KF>;
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * NW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int64_t strip, y0, y1;
   if (!reg_chunk(sp, g.n[1], wid, &strip, &y0, &y1)) return;
   const int64_t xs = strip * C::OX - C::H;  // interior column of the strip's first cell
-  // the plain path: every row the sweep freezes or loads (rows y0 - KF .. y1 + KF + period +
-  // RING - 3) and every column of the strip interior
-  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 >= reg_least(KF, RING) && y1 + reg_least(KF, RING) <= g.n[1])
-    sweep<SH, PX, KF, RING, false>(in, out, wts, g, xs, y0, y1, lane);
+  // the plain path: every row the sweep freezes or loads (rows y0 - R KF .. y1 + R KF +
+  // period + RING - 3) and every column of the strip interior
+  constexpr int least = reg_least(R, KF, RING);
+  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 >= least && y1 + least <= g.n[1])
+    sweep<R, SH, PX, KF, RING, false>(in, out, wts, g, xs, y0, y1, lane);
   else
-    sweep<SH, PX, KF, RING, true>(in, out, wts, g, xs, y0, y1, lane);
+    sweep<R, SH, PX, KF, RING, true>(in, out, wts, g, xs, y0, y1, lane);
 }
 
 // Resident waves of one variant on this device (workgroups per CU x waves x CUs), set
 // by the variant's prepare() before its first launch.
-template <int SH, int PX, int KF, int RING, int NW, int WPE>
+template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
 struct RegSlots {
   static int64_t value;
 };
-template <int SH, int PX, int KF, int RING, int NW, int WPE>
-int64_t RegSlots<SH, PX, KF, RING, NW, WPE>::value = 0;
+template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
+int64_t RegSlots<R, SH, PX, KF, RING, NW, WPE>::value = 0;
 
-template <int SH, int PX, int KF, int RING, int NW, int WPE>
+template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
 static hipError_t prepare_reg2d() {
   int nb = 0, dev = 0, cus = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, (const void *)k2d_reg<SH, PX, KF, RING, NW, WPE>, NW * 64, 0);
+      &nb, (const void *)k2d_reg<R, SH, PX, KF, RING, NW, WPE>, NW * 64, 0);
   if (e != hipSuccess) return e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-  RegSlots<SH, PX, KF, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
+  RegSlots<R, SH, PX, KF, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
   return hipSuccess;
 }
 
-template <int SH, int PX, int KF, int RING, int NW, int WPE>
+template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
 static hipError_t launch_reg2d(const double *in, double *out, const double *wts, const Geo &g, hipStream_t s) {
-  using C = Reg2D<PX, KF>;
-  using S = RegSlots<SH, PX, KF, RING, NW, WPE>;
+  using C = Reg2D<R, PX, KF>;
+  using S = RegSlots<R, SH, PX, KF, RING, NW, WPE>;
   const int64_t slots = S::value > 0 ? S::value : 2048;
-  const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, KF, slots, kRegEdgeCost, reg_least(KF, RING));
+  const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, C::H, slots, kRegEdgeCost, reg_least(R, KF, RING));
   const int64_t waves = reg_waves(sp);
-  hipLaunchKernelGGL((k2d_reg<SH, PX, KF, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
+  hipLaunchKernelGGL((k2d_reg<R, SH, PX, KF, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
                      s, in, out, wts, g, sp);
   return hipGetLastError();
 }
 
 // Variant::mf 7 marks this family: by = 1 (rows are chunked per launch), bx = output
 // columns per wave, px = columns per lane, st = rows in the load ring; WPE = the waves per
-// SIMD the compiler is held to (registers); the weights are the dense 3 x 3 block.
-#define CS_VR2D(ID, SH, PX, KF, RING, WPE)                                                                     \
-  Variant {                                                                                                    \
-    ID, 7, 1, SH, KF, 1, Reg2D<PX, KF>::OX, 4, 0, PX, RING, &launch_reg2d<SH, PX, KF, RING, 4, WPE>,          \
-        &prepare_reg2d<SH, PX, KF, RING, 4, WPE>,                                                             \
-        "k2d_reg<r1," #SH ",p" #PX ",kf" #KF ",ring" #RING ",wpe" #WPE ">"                                   \
+// SIMD the compiler is held to (registers); the weights are the dense (2R+1)^2 block.
+#define CS_VR2D(ID, R, SH, PX, KF, RING, WPE)                                                                 \
+  Variant {                                                                                                   \
+    ID, 7, R, SH, KF, 1, Reg2D<R, PX, KF>::OX, 4, 0, PX, RING, &launch_reg2d<R, SH, PX, KF, RING, 4, WPE>,    \
+        &prepare_reg2d<R, SH, PX, KF, RING, 4, WPE>,                                                         \
+        "k2d_reg<r" #R "," #SH ",p" #PX ",kf" #KF ",ring" #RING ",wpe" #WPE ">"                             \
   }
 
-// ids 700000 + (ring 6) * 50000 + (star) * 10000 + PX * 100 + KF.  Measured at 10240^2
-// (GStencil/s over 64-step runs, one box): box p2 kf12 ring 6 (2 waves per SIMD) 1902,
-// p2 kf12 ring 3 (3 waves) 1827, p2 kf8 ring 6 (3 waves) 1722, p2 kf16 ring 6 1659, p4 kf8
-// ring 3 (2 waves) 1428; star p2 kf12 ring 3 2428.  Fewer columns per lane cost more
-// recomputed columns and lane shifts per cell but leave registers for more waves and a
-// deeper load ring, which the FP64 FMA issue needs: one wave issues an FP64 FMA about every
-// 12 cycles, two waves about every 6.
+// ids 700000 + (ring of 2 (2R + 1) rows) * 50000 + (star) * 10000 + (R > 1) * R * 1000 + PX * 100
+// + KF.  Measured (GStencil/s over 64-step runs; boxes differ by up to 10%): box2d1r 10240^2
+// p2 kf8 ring 6 (3 waves per SIMD) 1662-1745, p2 kf12 ring 6 (2 waves) 1617-1902, p2 kf12
+// ring 3 (3 waves) 1827, p4 kf8 ring 3 (2 waves) 1352-1428; star2d1r p2 kf12 ring 3 2428;
+// star2d3r 8192^2 p2 kf4 ring 7 773 (kf3 627, kf6 499, kf2 428); box2d3r p2 kf2 330 (kf3
+// 315).  Fewer columns per lane cost more recomputed columns and lane shifts per cell but
+// leave registers for more waves and a deeper load ring, which the FP64 FMA issue needs:
+// one wave issues an FP64 FMA about every 12 cycles, two waves about every 6.
 static const Variant kVariantsReg2D[] = {
-    CS_VR2D(750212, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, SHAPE_BOX, 2, 12, 3, 3),
-    CS_VR2D(750208, SHAPE_BOX, 2, 8, 6, 3),   CS_VR2D(750216, SHAPE_BOX, 2, 16, 6, 2),
-    CS_VR2D(700408, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700404, SHAPE_BOX, 4, 4, 3, 2),
-    CS_VR2D(700402, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, SHAPE_BOX, 4, 1, 3, 2),
-    CS_VR2D(710212, SHAPE_STAR, 2, 12, 3, 3), CS_VR2D(760212, SHAPE_STAR, 2, 12, 6, 2),
-    CS_VR2D(760216, SHAPE_STAR, 2, 16, 6, 2), CS_VR2D(760208, SHAPE_STAR, 2, 8, 6, 3),
-    CS_VR2D(710404, SHAPE_STAR, 4, 4, 3, 2),  CS_VR2D(710402, SHAPE_STAR, 4, 2, 3, 2),
-    CS_VR2D(710401, SHAPE_STAR, 4, 1, 3, 2),
+    CS_VR2D(750212, 1, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, 1, SHAPE_BOX, 2, 12, 3, 3),
+    CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),   CS_VR2D(750216, 1, SHAPE_BOX, 2, 16, 6, 2),
+    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),
+    CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
+    CS_VR2D(710212, 1, SHAPE_STAR, 2, 12, 3, 3), CS_VR2D(760212, 1, SHAPE_STAR, 2, 12, 6, 2),
+    CS_VR2D(760216, 1, SHAPE_STAR, 2, 16, 6, 2), CS_VR2D(760208, 1, SHAPE_STAR, 2, 8, 6, 3),
+    CS_VR2D(710404, 1, SHAPE_STAR, 4, 4, 3, 2),  CS_VR2D(710402, 1, SHAPE_STAR, 4, 2, 3, 2),
+    CS_VR2D(710401, 1, SHAPE_STAR, 4, 1, 3, 2),
+    // radius 3 (star2d3r, box2d3r): rings of 7 rows
+    CS_VR2D(713204, 3, SHAPE_STAR, 2, 4, 7, 2),  CS_VR2D(713203, 3, SHAPE_STAR, 2, 3, 7, 3),
+    CS_VR2D(713206, 3, SHAPE_STAR, 2, 6, 7, 2),  CS_VR2D(713202, 3, SHAPE_STAR, 2, 2, 7, 4),
+    CS_VR2D(713201, 3, SHAPE_STAR, 2, 1, 7, 4),  CS_VR2D(703203, 3, SHAPE_BOX, 2, 3, 7, 3),
+    CS_VR2D(703202, 3, SHAPE_BOX, 2, 2, 7, 4),   CS_VR2D(703201, 3, SHAPE_BOX, 2, 1, 7, 4),
 };
 
 const Variant *variants_2d_reg(int *count) {

@@ -81,12 +81,12 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
 int32_t cs_debug_ctable_map(int32_t radius, int32_t shape);
 
 /* Wave decomposition of one launch of the register-streaming 2-D kernels (family 8): for
- * ny x nx interior cells, `ox` output columns per strip, fusion depth kf, a load ring of
- * `ring` rows and `slots` resident waves, split[0..4] = strips, he, hm, we, wi (rows per
- * checked chunk, rows per plain chunk, chunks per edge strip, chunks per other strip); waves
- * = the edge strips' chunks first.  Returns 0, or -1 for invalid arguments. */
-int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t kf, int32_t ring, int64_t slots,
-                             int64_t *split);
+ * ny x nx interior cells, `ox` output columns per strip, the stencil's radius, fusion depth
+ * kf, a load ring of `ring` rows and `slots` resident waves, split[0..4] = strips, he, hm, we,
+ * wi (rows per checked chunk, rows per plain chunk, chunks per edge strip, chunks per other
+ * strip); waves = the edge strips' chunks first.  Returns 0, or -1 for invalid arguments. */
+int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t radius, int32_t kf, int32_t ring,
+                             int64_t slots, int64_t *split);
 
 /* Wave `wid` of that decomposition: rows[0..2] = strip, first row, end row; returns 1, or 0 for
  * a wave without work. */

@@ -198,16 +198,16 @@ def test_every_stream_variant(gpu, vid, vname, R, shape, kf, by, bx):
 
 def _reg_ids():
     import test_kernel_emulation as em
-    return [(v["id"], v["name"], v["shape"], v["kf"], v["bx"]) for v in em.VARIANTS_REG2D]
+    return [(v["id"], v["name"], v["R"], v["shape"], v["kf"], v["bx"]) for v in em.VARIANTS_REG2D]
 
 
-@pytest.mark.parametrize("vid,vname,shape,kf,ox", _reg_ids())
-def test_every_reg_variant(gpu, vid, vname, shape, kf, ox):
+@pytest.mark.parametrize("vid,vname,R,shape,kf,ox", _reg_ids())
+def test_every_reg_variant(gpu, vid, vname, R, shape, kf, ox):
     # the register-streaming kernels: grids of one to many strips (edge strips, the plain
     # path in tall chunks), a padded layout with odd halo widths, fused launches and remainders
     print(f"variant {vname}", flush=True)
-    name = "box2d1r" if shape == 0 else "star2d1r"
-    for sizes in ([3 * kf + 5, ox // 2 + 3], [1100, 1301], [2011, 2 * ox + 7], [7, 5000]):
+    name = {(1, 0): "box2d1r", (1, 1): "star2d1r", (3, 0): "box2d3r", (3, 1): "star2d3r"}[(R, shape)]
+    for sizes in ([3 * R * kf + 5, ox // 2 + 3], [1100, 1301], [2011, 2 * ox + 7], [7, 5000]):
         L, R, offs, w, dense, x = setup(name, sizes, vid)
         for steps in (kf, 2 * kf + 1):
             with Plan(L, R, dense, variant=vid) as p:
@@ -216,7 +216,7 @@ def test_every_reg_variant(gpu, vid, vname, shape, kf, ox):
                 assert f"variant={vid}" in p.describe()
                 got = p.download()
             check(L, got, oracle.run(L, offs, w, x, steps, threads=0))
-    L = Layout(2, [903, 1507], [2, 3], [899, 1500])
+    L = Layout(2, [909, 1511], [4, 5], [899, 1500])
     ndim, R, _ = SHAPES[name]
     offs = shape_offsets(name)
     w = oracle.fill_weights(len(offs), vid + 1)

@@ -15,6 +15,7 @@ block coverage, halo masking and MFMA lane algebra in numpy, with:
   * the result compared with the CPU oracle (1e-13 relative).
 """
 import ctypes
+import math
 
 import numpy as np
 import pytest
@@ -1640,16 +1641,16 @@ def test_variant_emulation_stream(v):
 
 # ---------------------------------------------------------------- 2-D register-streaming family
 
-def reg2d_split(ny, nx, ox, kf, ring, slots):
+def reg2d_split(ny, nx, ox, radius, kf, ring, slots):
     lib = _dbg()
     i64p = ctypes.POINTER(ctypes.c_int64)
     lib.cs_debug_reg2d_split.restype = ctypes.c_int32
     lib.cs_debug_reg2d_split.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
-                                         ctypes.c_int32, ctypes.c_int64, i64p]
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, i64p]
     lib.cs_debug_reg2d_chunk.restype = ctypes.c_int32
     lib.cs_debug_reg2d_chunk.argtypes = [i64p, ctypes.c_int64, ctypes.c_int64, i64p]
     split = (ctypes.c_int64 * 5)()
-    assert lib.cs_debug_reg2d_split(ny, nx, ox, kf, ring, slots, split) == 0
+    assert lib.cs_debug_reg2d_split(ny, nx, ox, radius, kf, ring, slots, split) == 0
     strips, he, hm, we, wi = list(split)
     ne = 2 if strips >= 2 else 1
     chunks = []
@@ -1660,34 +1661,37 @@ def reg2d_split(ny, nx, ox, kf, ring, slots):
     return list(split), chunks
 
 
-def reg_period(ring):
-    return ring if ring % 3 == 0 else 3 * ring
+def reg_period(R, ring):
+    ns = 2 * R + 1
+    return ns // math.gcd(ns, ring) * ring
 
 
 def emulate_2d_reg(v, layout, dense, x, slots):
     """k2d_reg: each wave sweeps rows [y0, y1) of a strip of 64 PX columns (lane l: columns
-    l PX .. l PX + PX - 1, so that a cell's column neighbours are the adjacent cells of the
-    flat lane-major row, zero past lanes 0 and 63), one input row per iteration in unrolled
-    blocks of the ring's period, rows loaded RING - 1 iterations ahead into a ring of RING
-    rows, level s + 1 holding partial sums for rows yq - 1, yq, yq + 1 in slots (K - s - 1,
-    K - s, K - s + 1) mod 3, cells outside the interior frozen on the checked path, level KF's
+    l PX .. l PX + PX - 1, so that a cell's column neighbours are the nearby cells of the
+    flat lane-major row, zero past the wave's ends), one input row per iteration in unrolled
+    blocks of the period, rows loaded RING - 1 iterations ahead into a ring of RING rows,
+    level s + 1 holding partial sums for rows yq - R .. yq + R in slots (y - ys) mod (2R + 1),
+    the incoming row's taps of row offset dy = R .. -R added to row yq - dy (dy = R completes
+    it, dy = -R starts it), cells outside the interior frozen on the checked path, level KF's
     completed row stored on the output columns.  Checks loads and stores against the array
     and counts the stores per cell."""
-    PX, KF, RING = v["px"], v["kf"], v["st"]
-    PER = reg_period(RING)
-    least = KF + PER + RING - 2
+    R, PX, KF, RING = v["R"], v["px"], v["kf"], v["st"]
+    NS, H = 2 * R + 1, R * KF
+    PER = reg_period(R, RING)
+    least = H + PER + RING - 2
     box = v["shape"] == 0
-    H = KF
     OX = 64 * PX - 2 * H
-    assert v["mf"] == 7 and v["R"] == 1 and v["bx"] == OX and v["lds"] == 0
+    assert v["mf"] == 7 and v["bx"] == OX and v["lds"] == 0 and PER % NS == 0 and PER % RING == 0
     _, ey, pitch = layout.ext
     _, oy, ox = layout.org
     _, ny, nx = layout.n
     X = x.reshape(ey, pitch)
-    w = np.asarray(dense, dtype=np.float64).reshape(3, 3)
+    D = 2 * R + 1
+    w = np.asarray(dense, dtype=np.float64).reshape(D, D)
     out = np.zeros((ey, pitch))
     hits = np.zeros((ey, pitch), dtype=np.int64)
-    _, chunks = reg2d_split(ny, nx, OX, KF, RING, slots)
+    _, chunks = reg2d_split(ny, nx, OX, R, KF, RING, slots)
     d = np.arange(64 * PX)
     plain = 0
     for strip, y0, y1 in chunks:
@@ -1708,7 +1712,7 @@ def emulate_2d_reg(v, layout, dense, x, slots):
             return np.where(ok, X[ay if 0 <= ay < ey else 0, np.where(ok, ox + cols, 0)], 0.0)
 
         ys, iters = y0 - H, (y1 - y0) + 2 * H
-        acc = np.zeros((KF, 3, 64 * PX))
+        acc = np.zeros((KF, NS, 64 * PX))
         ring = [load(ys + k) for k in range(RING - 1)] + [None]
         for it in range(0, iters, PER):
             for K in range(PER):
@@ -1716,22 +1720,27 @@ def emulate_2d_reg(v, layout, dense, x, slots):
                 ring[(K + RING - 1) % RING] = load(r + RING - 1)
                 q = ring[K % RING].copy()
                 for s in range(KF):
-                    yq = r - s
-                    sA, sB, sC = (K % 3 - s - 1) % 3, (K % 3 - s) % 3, (K % 3 - s + 1) % 3
-                    qe = np.concatenate([[0.0], q, [0.0]])
-                    lf, mid, rt = qe[:-2], q, qe[2:]
-                    A, B = acc[s, sA], acc[s, sB]
-                    a = A + w[2, 0] * lf + w[2, 1] * mid + w[2, 2] * rt if box else A + w[2, 1] * mid
-                    b = B + w[1, 0] * lf + w[1, 1] * mid + w[1, 2] * rt
-                    if fast:
-                        A, B = a, b
-                    else:
-                        A = np.where(~colin | (yq - 1 < 0) | (yq - 1 >= ny), A, a)
-                        B = np.where(~colin | (yq < 0) | (yq >= ny), q, b)
-                    Cn = w[0, 0] * lf + w[0, 1] * mid + w[0, 2] * rt if box else w[0, 1] * mid
-                    acc[s, sA], acc[s, sB], acc[s, sC] = A, B, Cn
-                    q = A.copy()
-                yo = r - KF
+                    yq = r - R * s
+                    qe = np.concatenate([np.zeros(R), q, np.zeros(R)])
+                    for dy in range(R, -R - 1, -1):
+                        sl = (K - R * s - dy) % NS
+                        T = acc[s, sl]
+                        t = None
+                        for dx in range(-R, R + 1):
+                            if not box and dy != 0 and dx != 0:
+                                continue
+                            term = w[dy + R, dx + R] * qe[R + dx:R + dx + 64 * PX]
+                            t = term if (t is None and dy == -R) else (T if t is None else t) + term
+                        if not fast:
+                            yt = yq - dy
+                            fr = (yt < 0 or yt >= ny) | ~colin
+                            if dy > 0:
+                                t = np.where(fr, T, t)
+                            elif dy == 0:
+                                t = np.where(fr, q, t)
+                        acc[s, sl] = t
+                    q = acc[s, (K - R * s - R) % NS].copy()
+                yo = r - H
                 if y0 <= yo < y1:
                     c = ox + cols[colout]
                     assert 0 <= oy + yo < ey and (c >= 0).all() and (c < pitch).all()
@@ -1751,17 +1760,18 @@ VARIANTS_REG2D = variants(8)
 def test_variant_emulation_2d_reg(v):
     # grids of one, two and three strips, with halos of 1 and of odd widths (masks at the
     # array's edges), few slots (tall chunks: the plain path inside) and many
-    R, kf, OX = 1, v["kf"], v["bx"]
-    name = "box2d1r" if v["shape"] == 0 else "star2d1r"
+    R, kf, OX = v["R"], v["kf"], v["bx"]
+    name = {(1, 0): "box2d1r", (1, 1): "star2d1r", (3, 0): "box2d3r", (3, 1): "star2d3r"}[(R, v["shape"])]
     offs = shape_offsets(name)
     rng = np.random.default_rng(v["id"])
     w = rng.uniform(0.05, 1.0, len(offs))
     w /= w.sum()
     dense = dense_weights(2, R, offs, w)
-    cases = [(Layout.with_halo([6 * kf + 40, 2 * OX + 37], 1), 6),
-             (Layout(2, [4 * kf + 31, OX // 2 + 12], [2, 3], [4 * kf + 26, OX // 2 + 5]), 3),
-             (Layout.with_halo([9 * kf + 25, OX + 11], 1), 64),
-             (Layout.with_halo([16 * kf + 120, 3 * OX], 1), 15)]
+    H = R * kf
+    cases = [(Layout.with_halo([6 * H + 40, 2 * OX + 37], R), 6),
+             (Layout(2, [4 * H + 31 + 2 * R, OX // 2 + 12 + 2 * R], [R + 1, R + 2], [4 * H + 26, OX // 2 + 5]), 3),
+             (Layout.with_halo([9 * H + 25, OX + 11], R), 64),
+             (Layout.with_halo([16 * H + 120, 3 * OX], R), 15)]
     for i, (layout, slots) in enumerate(cases):
         x = oracle.fill_data(layout, v["id"] + slots)
         got = emulate_2d_reg(v, layout, dense, x, slots)
@@ -1773,13 +1783,15 @@ def test_variant_emulation_2d_reg(v):
         assert err < 1e-13, (v["name"], slots, err)
 
 
-@pytest.mark.parametrize("ny,nx,ox,kf,ring,slots", [(10240, 10240, 104, 12, 6, 2048), (8192, 8192, 240, 8, 3, 2048),
-                                                    (100, 3000, 240, 8, 3, 2048), (5000, 200, 96, 16, 6, 2048),
-                                                    (31, 31, 254, 1, 3, 4096), (1, 7, 240, 8, 6, 16)])
-def test_reg2d_split_covers_once(ny, nx, ox, kf, ring, slots):
+@pytest.mark.parametrize("ny,nx,ox,R,kf,ring,slots", [(10240, 10240, 104, 1, 12, 6, 2048),
+                                                      (8192, 8192, 104, 3, 4, 7, 2048),
+                                                      (8192, 8192, 240, 1, 8, 3, 2048), (100, 3000, 240, 1, 8, 3, 2048),
+                                                      (5000, 200, 96, 1, 16, 6, 2048), (31, 31, 254, 1, 1, 3, 4096),
+                                                      (1, 7, 240, 1, 8, 6, 16), (50, 70, 122, 3, 1, 7, 512)])
+def test_reg2d_split_covers_once(ny, nx, ox, R, kf, ring, slots):
     # every row of every strip in exactly one chunk, never more waves than slots when the
     # strips fit
-    split, chunks = reg2d_split(ny, nx, ox, kf, ring, slots)
+    split, chunks = reg2d_split(ny, nx, ox, R, kf, ring, slots)
     strips, he, hm, we, wi = split
     ne = 2 if strips >= 2 else 1
     waves = ne * we + (strips - ne) * wi
@@ -1790,4 +1802,4 @@ def test_reg2d_split_covers_once(ny, nx, ox, kf, ring, slots):
         cover[s, y0:y1] += 1
     assert (cover == 1).all()
     # a plain-path chunk needs the rows it freezes and loads inside the grid
-    assert he >= kf + reg_period(ring) + ring - 2
+    assert he >= R * kf + reg_period(R, ring) + ring - 2
