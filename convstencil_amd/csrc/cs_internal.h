@@ -78,7 +78,8 @@ __device__ __forceinline__ void prefetch_consume(const double (&pf)[Prefetch<C>:
 // strips - 1 (the edge strips, which run the checked path) are cut into `we` chunks of
 // `he` rows; every other strip into `wi` chunks: `he` rows at the top, `he` at the bottom
 // (the checked path) and chunks of `hm` rows between them; hm = 0 marks strips of at most
-// 2 he rows, cut like the edge strips.  Waves: the edge strips' first.
+// 2 he rows, cut like the edge strips.  Waves: the edge strips' chunks first, then the other
+// strips' chunk by chunk.
 struct RegSplit {
   int64_t strips, he, hm, we, wi;
 };
@@ -98,10 +99,12 @@ __host__ __device__ inline bool reg_chunk(const RegSplit &sp, int64_t ny, int64_
     a = (wid % sp.we) * sp.he;
     b = a + sp.he;
   } else {
-    const int64_t u = wid - ne * sp.we;
-    *strip = 1 + u / sp.wi;
-    if (*strip >= sp.strips - 1) return false;
-    const int64_t c = u % sp.wi;
+    // consecutive waves take the same chunk of neighbouring strips, so that the columns
+    // that neighbouring strips share are loaded at about the same time on one CU (L2 hits)
+    const int64_t ni = sp.strips - ne, u = wid - ne * sp.we;
+    if (ni <= 0 || u >= ni * sp.wi) return false;
+    const int64_t c = u / ni;
+    *strip = 1 + u % ni;
     if (sp.hm == 0) {  // a short strip: chunks of he rows from the top
       a = c * sp.he;
       b = a + sp.he;
