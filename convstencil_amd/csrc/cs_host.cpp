@@ -35,13 +35,15 @@ struct cs_plan {
   int device = 0;
   hipStream_t stream = nullptr;
   double *buf[2] = {nullptr, nullptr};
-  double *bfrag = nullptr;  // B fragments for the selected MFMA shape
+  double *bfrag = nullptr;  // B fragments for the selected MFMA shape (or dense weights)
+  int64_t frag_off = 0;     // when the register kernels' dense weights come first: where the
+                            // MFMA single-step kernel's B fragments start (doubles)
   int cur = 0;              // buffer holding the current state
   int64_t steps_done = 0;
   bool halo_uniform = true;  // both buffers carry the same halo -> fusion is exact
   const Variant *vmain = nullptr;
   std::vector<const Variant *> vrem;  // vrem[r] runs r fused steps (r < vmain->kf), may be null
-  const Variant *vone = nullptr;      // kf == 1 variant of the same family
+  const Variant *vone = nullptr;      // single-step variant (the register families use the MFMA one)
   std::map<std::pair<int64_t, int>, hipGraphExec_t> graphs;
   int64_t launches = 0;
   int tuned = 0;  // 0 defaults or a forced variant, 1 cached autotuner choice, 2 timed now
@@ -142,6 +144,19 @@ int max_kf(int ndim, int mf, int R, int shape, int prec = CS_PREC_FP64, int st =
   return m;
 }
 
+// The single-step variant of an FP64 plan whose main variant is v: v itself when it is a
+// single-step one (a variant forced by id must be the one that runs), else its family's
+// first single-step variant -- except that the register kernels' single steps (alternating
+// halos, remainders of one step) run on the LDS-staged MFMA single-step kernel, which keeps
+// more bytes in flight: measured (GStencil/s) box2d1r 10240^2 300 against 203, star2d3r
+// 8192^2 286 against 253; box2d3r keeps its own (246 against 204).
+const Variant *single_step(int nd, int R, int shape, const Variant *v) {
+  if (v->kf == 1) return v;
+  if ((v->mf == 6 || v->mf == 7) && !(nd == 2 && R > 1 && shape == cs::SHAPE_BOX))
+    if (const Variant *m = find_variant(nd, 4, R, shape, 1, 0, CS_PREC_FP64)) return m;
+  return find_variant(nd, v->mf, R, shape, 1, 0, CS_PREC_FP64);
+}
+
 // The main variant of a plan: a forced id, else the preferred family's default depth.
 const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, int tile_variant) {
   const int mf = prec != CS_PREC_FP64 ? prec : 4;
@@ -212,7 +227,8 @@ cs_status select_variants(cs_plan *p) {
   p->vmain = v;
   // single-step launches: the chosen variant itself when it is a single-step one (a variant
   // forced by id must be the one that runs), else the family's first single-step variant
-  p->vone = v->kf == 1 ? v : find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec, fam);
+  p->vone = prec == CS_PREC_FP64 ? single_step(nd, p->R, p->shape, v)
+                                  : v->kf == 1 ? v : find_variant(nd, v->mf, p->R, p->shape, 1, 0, prec, fam);
   if (!p->vone) return CS_ERR_UNSUPPORTED;
   p->vrem.assign(v->kf, nullptr);
   for (int r = 1; r < v->kf; ++r) p->vrem[r] = find_variant(nd, v->mf, p->R, p->shape, r, 0, prec, fam);
@@ -232,10 +248,17 @@ cs_status select_variants(cs_plan *p) {
     return CS_OK;
   }
   // the FMA kernels (the ladder's, the 3-D mf 3 family and the 2-D register-streaming
-  // mf 7 family) read the dense weights (zero-padded to whole 64-entry rows)
+  // mf 7 family) read the dense weights (zero-padded to whole 64-entry rows); an mf 7 plan
+  // whose single steps run on the MFMA kernel appends that kernel's B fragments
+  p->frag_off = 0;
   if (v->mf == 1 || v->mf == 2 || v->mf == 3 || v->mf == 7) {
     std::vector<double> h((p->w.size() + 63) / 64 * 64, 0.0);
     std::copy(p->w.begin(), p->w.end(), h.begin());
+    if (v->mf == 7 && p->vone->mf == 4) {
+      p->frag_off = (int64_t)h.size();
+      h.resize(h.size() + (size_t)cs::bfrag_ksteps_2d_f64(4, p->R, p->shape) * 64);
+      cs::build_bfrag_2d_f64(4, p->R, p->shape, p->w.data(), h.data() + p->frag_off);
+    }
     if (p->bfrag) (void)hipFree(p->bfrag);
     p->bfrag = nullptr;
     CS_HIP(hipMalloc(&p->bfrag, h.size() * sizeof(double)));
@@ -265,7 +288,8 @@ cs_status select_variants(cs_plan *p) {
 
 cs_status launch_one(cs_plan *p, const Variant *v) {
   const Geo g = make_geo(p->L, v);
-  CS_HIP(v->launch(p->buf[p->cur], p->buf[p->cur ^ 1], p->bfrag, g, p->stream));
+  const double *wts = v->mf == 4 && p->frag_off > 0 ? p->bfrag + p->frag_off : p->bfrag;
+  CS_HIP(v->launch(p->buf[p->cur], p->buf[p->cur ^ 1], wts, g, p->stream));
   p->cur ^= 1;
   p->launches += 1;
   return CS_OK;
@@ -511,7 +535,7 @@ cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int t
   if (!v) return tile_variant > 0 ? CS_ERR_INVALID : CS_ERR_UNSUPPORTED;
   if (v->kf > vrem_cap) return CS_ERR_UNSUPPORTED;
   *vmain = v;
-  *vone = v->kf == 1 ? v : find_variant(ndim, v->mf, R, v->shape, 1, 0, CS_PREC_FP64);
+  *vone = single_step(ndim, R, v->shape, v);
   if (!*vone) return CS_ERR_UNSUPPORTED;
   for (int r = 0; r < vrem_cap; ++r)
     vrem[r] = r >= 1 && r < v->kf ? find_variant(ndim, v->mf, R, v->shape, r, 0, CS_PREC_FP64) : nullptr;

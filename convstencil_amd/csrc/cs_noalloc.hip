@@ -92,12 +92,15 @@ extern "C" cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil
   const int shape = classify_2d(R, st->weights);
   const cs::Variant *vmain = nullptr, *vone = nullptr, *vrem[32] = {};
   CS_TRY(cs::pick_fp64_variants(2, R, shape, o.fusion_depth, o.tile_variant, &vmain, &vone, vrem, 32));
-  // the dilated tables (mf 4: B fragments) or the register-streaming kernels (mf 7: dense
-  // weights); every launch of a run comes from the main variant's family
-  const bool dense = vmain->mf == 7;
+  // the dilated tables (mf 4: B fragments) and the register-streaming kernels (mf 7: dense
+  // weights, their single steps on the mf 4 kernel as in a plan): scratch holds the dense
+  // weights first when the main variant is mf 7, then the B fragments when an mf 4 kernel runs
+  const bool dense = vmain->mf == 7, frags4 = vmain->mf == 4 || vone->mf == 4;
+  if (vmain->mf != 4 && !dense) return CS_ERR_UNSUPPORTED;
   const int D = 2 * R + 1;
-  const int nks = dense ? 1 : cs::bfrag_ksteps_2d_f64(vmain->mf, R, vmain->shape);
-  if ((vmain->mf != 4 && !dense) || nks > 16 || nks * 64 > cs::kScratchDoubles) {
+  const int nks = frags4 ? cs::bfrag_ksteps_2d_f64(4, R, vmain->shape) : 0;
+  const int64_t off4 = dense ? 64 : 0;  // doubles before the B fragments
+  if (nks > 16 || off4 + nks * 64 > cs::kScratchDoubles) {
     fprintf(stderr, "convstencil: cs_run_2d_noalloc: %d k-steps exceed the scratch array (%d doubles)\n", nks,
             cs::kScratchDoubles);
     return CS_ERR_UNSUPPORTED;
@@ -105,17 +108,17 @@ extern "C" cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil
   // weight operands on the host (fixed arrays: nothing is allocated), compacted
   double wd[49];
   for (int i = 0; i < D * D; ++i) wd[i] = st->weights[i];
-  cs::CompactFrags c{};
-  if (dense) {
-    for (int i = 0; i < D * D; ++i) c.v[i] = wd[i];
-  } else {
+  cs::CompactFrags cd{}, cf{};
+  if (dense)
+    for (int i = 0; i < D * D; ++i) cd.v[i] = wd[i];
+  if (frags4) {
     double full[16 * 64];
     cs::build_bfrag_2d_f64(4, R, vmain->shape, wd, full);
     for (int ks = 0; ks < nks; ++ks)
       for (int lane = 0; lane < 64; ++lane) {
-        const double v = full[ks * 64 + lane], e = c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)];
+        const double v = full[ks * 64 + lane], e = cf.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)];
         if ((lane & 12) == 0)
-          c.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)] = v;
+          cf.v[ks * 16 + (lane >> 4) * 4 + (lane & 3)] = v;
         else if (v != e)
           return CS_ERR_DEVICE;  // the tables' lane structure changed: never expand wrongly
       }
@@ -123,7 +126,9 @@ extern "C" cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil
   double *frags = nullptr;
   CS_TRY(cs::scratch_ptr(dev, &frags));
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(cs::expand_frags, dim3((nks * 64 + 255) / 256), dim3(256), 0, s, c, nks, dense ? 1 : 0, frags);
+  if (dense) hipLaunchKernelGGL(cs::expand_frags, dim3(1), dim3(256), 0, s, cd, 1, 1, frags);
+  if (frags4)
+    hipLaunchKernelGGL(cs::expand_frags, dim3((nks * 64 + 255) / 256), dim3(256), 0, s, cf, nks, 0, frags + off4);
   CS_HIP(hipGetLastError());
   for (const cs::Variant *u : {vmain, vone}) CS_HIP(u->prepare());
   // the reference's ping-pong: fused launches are exact when both buffers carry the same
@@ -153,7 +158,7 @@ extern "C" cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil
     if (v != vmain && v != vone) CS_HIP(v->prepare());
     g.tiles[1] = (layout->n[1] + v->by - 1) / v->by;
     g.tiles[2] = (layout->n[2] + v->bx - 1) / v->bx;
-    CS_HIP(v->launch(b[cur], b[cur ^ 1], frags, g, s));
+    CS_HIP(v->launch(b[cur], b[cur ^ 1], v->mf == 4 ? frags + off4 : frags, g, s));
     cur ^= 1;
     left -= v->kf;
   }

@@ -130,6 +130,9 @@ static const Variant kVariantsReg1D[] = {
     CS_VR1D(611864, 1, 18, 64, 4), CS_VR1D(611832, 1, 18, 32, 4), CS_VR1D(611816, 1, 18, 16, 4),
     CS_VR1D(611808, 1, 18, 8, 4),  CS_VR1D(611804, 1, 18, 4, 4),  CS_VR1D(611802, 1, 18, 2, 4),
     CS_VR1D(611801, 1, 18, 1, 4),  CS_VR1D(611032, 1, 10, 32, 4), CS_VR1D(612664, 1, 26, 64, 4),
+    // shorter segments: more waves per SIMD (FP64 issue) for more recomputed cells
+    CS_VR1D(611064, 1, 10, 64, 4), CS_VR1D(611264, 1, 12, 64, 4), CS_VR1D(610864, 1, 8, 64, 4),
+    CS_VR1D(610832, 1, 8, 32, 4),
     CS_VR1D(621832, 2, 18, 32, 4), CS_VR1D(621816, 2, 18, 16, 4), CS_VR1D(621808, 2, 18, 8, 4),
     CS_VR1D(621804, 2, 18, 4, 4),  CS_VR1D(621802, 2, 18, 2, 4),  CS_VR1D(621801, 2, 18, 1, 4),
     CS_VR1D(631816, 3, 18, 16, 4), CS_VR1D(631808, 3, 18, 8, 4),  CS_VR1D(631804, 3, 18, 4, 4),

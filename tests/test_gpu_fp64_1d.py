@@ -97,7 +97,7 @@ def test_reference_gpu_1d2r(gpu):
 
 
 # 6xxxxx: the register-resident kernels, 600000 + R * 10000 + cells per lane * 100 + depth
-@pytest.mark.parametrize("vid", [511016, 521016, 531016, 511008, 511032, 531008, 933001, 611864, 611032, 612664,
+@pytest.mark.parametrize("vid", [511016, 521016, 531016, 511008, 511032, 531008, 933001, 611864, 611032, 612664, 611064, 610864,
                                  611808, 621832, 621808, 631816, 631804, 641816, 641802])
 def test_tile_variants(gpu, vid):
     # every alternative tile, forced: partial tiles, fused launches and remainders
