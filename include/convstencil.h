@@ -183,9 +183,9 @@ cs_status cs_tune_key(const cs_layout *layout, const cs_stencil *stencil, const 
  * buffers must carry the same halo (fused launches read only their input's halo), else
  * pass fusion_depth 1 for the reference's alternating-halo single steps.  *final_buffer
  * receives 0 or 1, the buffer that holds the result.  Nothing is allocated: the kernels'
- * B fragments (radius <= 3: at most 16 x 64 doubles) live in a fixed device array of the
- * library, one per GPU, that every call on that GPU shares; calls that overlap on one GPU
- * race on it.  Larger radii return CS_ERR_UNSUPPORTED.  The call only enqueues kernels:
+ * weight operands (dense weights and/or B fragments, radius <= 3: at most 1024 doubles)
+ * live in a fixed device array of the library, one per GPU, that every call on that GPU
+ * shares; calls that overlap on one GPU race on it.  Larger radii return CS_ERR_UNSUPPORTED.  The call only enqueues kernels:
  * synchronise the stream before reading the result. */
 cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil *stencil, double *buf0, double *buf1,
                             int64_t steps, const cs_options *opt, void *stream, int32_t *final_buffer);
