@@ -1,8 +1,14 @@
-// cs_batch.cpp -- host side of the persistent batched launch (cs_batch_* in convstencil.h).
+// cs_batch.cpp -- host side of the batched workload (cs_batch_* in convstencil.h).
 //
-// Turns `steps` time steps of every grid into step blocks of work items (see
-// k2d_batch.hip), uploads the item table once per step count, and launches the
-// reset kernel plus the persistent kernel, optionally captured in a hipGraph.
+// FP64 batches whose grids all carry a uniform halo run on the register-streaming 2-D
+// kernels (k2d_reg.hip): launch j of the run advances every grid whose launch sequence
+// has a j-th launch, all grids of one kernel variant in one multi-grid launch (the kernel
+// boundary orders the steps; no device-wide barrier).  Every other batch runs the
+// persistent kernel: `steps` time steps of every grid become step blocks of work items
+// (see k2d_batch.hip), the item table is uploaded once per step count, and the reset
+// kernel plus the persistent kernel are launched.  Both are optionally captured in a
+// hipGraph.
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -15,6 +21,21 @@
 using namespace cs::host;
 
 namespace {
+
+// One multi-grid launch of a register-streaming variant.
+struct RegLaunch {
+  const cs::Variant *v = nullptr;
+  cs::RegItem *items = nullptr;
+  int nitems = 0, flip = 0;
+  int chain = 0;  // the launches of one stencil class form a chain (one stream)
+};
+
+struct RegSeq {
+  std::vector<RegLaunch> launches;
+  std::vector<int> parity_flip;  // per grid: its launches (mod 2)
+  int chains = 1;
+  hipGraphExec_t graph = nullptr;
+};
 
 struct Seq {
   cs::BatchItem *items = nullptr;
@@ -48,7 +69,21 @@ struct cs_batch {
   int64_t launches = 0;
   unsigned *d_error = nullptr;  // sticky device error word: set by a barrier that timed out
   bool failed = false;          // latched once the device word has been seen set
+  // register-streaming path (FP64, every grid's stencil has the family)
+  bool reg = false;
+  std::vector<double *> wdense;         // dense weights per grid
+  cs::RegGrid *d_rgrids = nullptr;
+  std::vector<cs::RegGrid> h_rgrids;    // host copy; must outlive the asynchronous upload
+  std::map<int64_t, RegSeq> rseqs;      // keyed by steps
+  // grids of different stencil classes are independent: their chains of launches run on
+  // side streams (default priority) forked from and joined to the batch's stream
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> events;
 };
+
+namespace cs {
+void reg_sequence(int R, int shape, int64_t steps, std::vector<const Variant *> *seq);
+}
 
 namespace {
 
@@ -126,6 +161,183 @@ cs_status build_seq(cs_batch *b, int64_t steps, Seq *s) {
   return CS_OK;
 }
 
+void free_rseq(RegSeq &s) {
+  if (s.graph) (void)hipGraphExecDestroy(s.graph);
+  for (RegLaunch &l : s.launches)
+    if (l.items) (void)hipFree(l.items);
+  s = RegSeq{};
+}
+
+// Waves of grid g for variant v: strips of v's output columns cut into chunks of about 64
+// rows (the checked chunks shorter), as the single-grid launch cuts them.  Measured on the
+// batch of 256 grids (FP64, GStencil/s): 224-row chunks 513, 128 622, 64 662, 40 639 --
+// many short waves balance the launches' tails better than the warm-up rows they add.
+void reg_items(const cs_batch *b, int g, const cs::Variant *v, std::vector<cs::RegItem> *out) {
+  const cs_layout &L = b->layouts[(size_t)g];
+  const int64_t ny = L.n[1], nx = L.n[2];
+  if (ny <= 0 || nx <= 0) return;
+  const int ox = v->bx, h = v->R * v->kf;
+  const int64_t strips = (nx + ox - 1) / ox;
+  const int64_t slots = strips * std::max<int64_t>(1, (ny + 63) / 64);
+  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, h, slots, cs::kRegEdgeCost, cs::reg_least(v->R, v->kf, v->st));
+  const int64_t waves = cs::reg_waves(sp);
+  for (int64_t w = 0; w < waves; ++w) {
+    int64_t strip = 0, y0 = 0, y1 = 0;
+    if (cs::reg_chunk(sp, ny, w, &strip, &y0, &y1)) out->push_back(cs::RegItem{g, (int)strip, (int)y0, (int)y1});
+  }
+}
+
+// The register path's launches for `steps` steps: launch j groups, per variant, every grid
+// whose sequence has a j-th launch.
+cs_status build_rseq(cs_batch *b, int64_t steps, RegSeq *s) {
+  std::vector<std::vector<const cs::Variant *>> seq((size_t)b->count);
+  std::vector<int> chain((size_t)b->count);
+  std::vector<std::pair<int, int>> classes;  // (R, shape) in order of first appearance
+  size_t nj = 0;
+  for (int g = 0; g < b->count; ++g) {
+    const std::pair<int, int> c(b->R[(size_t)g], b->shape[(size_t)g]);
+    size_t k = 0;
+    while (k < classes.size() && classes[k] != c) ++k;
+    if (k == classes.size()) classes.push_back(c);
+    chain[(size_t)g] = (int)k;
+    cs::reg_sequence(b->R[(size_t)g], b->shape[(size_t)g], steps, &seq[(size_t)g]);
+    if (seq[(size_t)g].empty()) return CS_ERR_UNSUPPORTED;
+    nj = std::max(nj, seq[(size_t)g].size());
+    s->parity_flip.push_back((int)(seq[(size_t)g].size() & 1));
+  }
+  std::map<std::pair<int, const cs::Variant *>, std::vector<cs::RegItem>> cache;
+  for (size_t j = 0; j < nj; ++j) {
+    std::map<const cs::Variant *, std::vector<cs::RegItem>> groups;
+    std::map<const cs::Variant *, int> group_chain;
+    for (int g = 0; g < b->count; ++g) {
+      if (j >= seq[(size_t)g].size()) continue;
+      const cs::Variant *v = seq[(size_t)g][j];
+      group_chain[v] = chain[(size_t)g];  // a variant serves one stencil class
+      auto key = std::make_pair(g, v);
+      auto c = cache.find(key);
+      if (c == cache.end()) {
+        std::vector<cs::RegItem> it;
+        reg_items(b, g, v, &it);
+        c = cache.emplace(key, it).first;
+      }
+      std::vector<cs::RegItem> &dst = groups[v];
+      dst.insert(dst.end(), c->second.begin(), c->second.end());
+    }
+    for (auto &kv : groups) {
+      if (kv.second.empty()) continue;
+      if (kv.second.size() > (size_t)0x7fffffff) return CS_ERR_UNSUPPORTED;
+      RegLaunch l;
+      l.v = kv.first;
+      l.nitems = (int)kv.second.size();
+      l.flip = (int)(j & 1);
+      l.chain = group_chain[kv.first];
+      CS_HIP(hipMalloc(&l.items, kv.second.size() * sizeof(cs::RegItem)));
+      s->launches.push_back(l);  // freed with the sequence from here on
+      CS_HIP(hipMemcpy(l.items, kv.second.data(), kv.second.size() * sizeof(cs::RegItem), hipMemcpyHostToDevice));
+    }
+  }
+  s->chains = (int)classes.size();
+  return CS_OK;
+}
+
+// Side streams and the fork / join events for `chains` chains (created once).
+cs_status ensure_side(cs_batch *b, int chains) {
+  while ((int)b->side.size() < chains - 1) {
+    hipStream_t t = nullptr;
+    CS_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    b->side.push_back(t);
+  }
+  while ((int)b->events.size() < chains) {
+    hipEvent_t e = nullptr;
+    CS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    b->events.push_back(e);
+  }
+  return CS_OK;
+}
+
+// Chain 0 on the batch's stream, chain c on side stream c - 1: forked after the grid table
+// upload, joined back before the run ends (also under graph capture).
+cs_status issue_reg(cs_batch *b, RegSeq &s) {
+  const int nc = s.chains;
+  if (nc > 1) {
+    CS_HIP(hipEventRecord(b->events[0], b->stream));
+    for (int c = 1; c < nc; ++c) CS_HIP(hipStreamWaitEvent(b->side[(size_t)c - 1], b->events[0], 0));
+  }
+  for (const RegLaunch &l : s.launches) {
+    const hipStream_t st = l.chain == 0 ? b->stream : b->side[(size_t)l.chain - 1];
+    CS_HIP(l.v->launch_multi(cs::RegMulti{b->d_rgrids, l.items, l.nitems, l.flip}, st));
+  }
+  for (int c = 1; c < nc; ++c) {
+    CS_HIP(hipEventRecord(b->events[(size_t)c], b->side[(size_t)c - 1]));
+    CS_HIP(hipStreamWaitEvent(b->stream, b->events[(size_t)c], 0));
+  }
+  return CS_OK;
+}
+
+cs_status run_reg(cs_batch *b, int64_t steps) {
+  auto it = b->rseqs.find(steps);
+  if (it == b->rseqs.end()) {
+    RegSeq s;
+    cs_status st = build_rseq(b, steps, &s);
+    if (st != CS_OK) {
+      free_rseq(s);
+      return st;
+    }
+    it = b->rseqs.emplace(steps, s).first;
+  }
+  RegSeq &s = it->second;
+  CS_TRY(ensure_side(b, s.chains));
+  // grid table: buffers, weights, layout and the buffer that holds each grid's state
+  b->h_rgrids.assign((size_t)b->count, cs::RegGrid{});
+  for (int g = 0; g < b->count; ++g) {
+    cs::RegGrid &G = b->h_rgrids[(size_t)g];
+    G.buf[0] = b->buf0[(size_t)g];
+    G.buf[1] = b->buf1[(size_t)g];
+    G.wts = b->wdense[(size_t)g];
+    const cs_layout &L = b->layouts[(size_t)g];
+    for (int d = 0; d < 3; ++d) {
+      G.geo.ext[d] = L.ext[d];
+      G.geo.org[d] = L.org[d];
+      G.geo.n[d] = L.n[d];
+    }
+    G.geo.tiles[0] = G.geo.tiles[1] = G.geo.tiles[2] = 1;
+    G.cur = b->cur[(size_t)g];
+  }
+  CS_HIP(hipMemcpyAsync(b->d_rgrids, b->h_rgrids.data(), b->h_rgrids.size() * sizeof(cs::RegGrid),
+                        hipMemcpyHostToDevice, b->stream));
+  if (b->opt.use_graph) {
+    if (!s.graph) {
+      hipGraph_t graph = nullptr;
+      CS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      cs_status st = issue_reg(b, s);
+      hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (st != CS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+      }
+      CS_HIP(e);
+      hipError_t ei = hipGraphInstantiate(&s.graph, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      CS_HIP(ei);
+    }
+    CS_HIP(hipGraphLaunch(s.graph, b->stream));
+  } else {
+    CS_TRY(issue_reg(b, s));
+  }
+  // the host table must stay unchanged until the copy above has run
+  CS_HIP(hipStreamSynchronize(b->stream));
+  b->launches += (int64_t)s.launches.size();
+  for (int g = 0; g < b->count; ++g) b->cur[(size_t)g] ^= s.parity_flip[(size_t)g];
+  return CS_OK;
+}
+
+bool reg_path(const cs_batch *b) {
+  if (b->prec != 0 || !b->reg) return false;
+  for (char u : b->uniform)
+    if (!u) return false;
+  return true;
+}
+
 cs_status issue(cs_batch *b, Seq &s) {
   const unsigned nwg = (unsigned)b->max_wg;
   CS_HIP(cs::batch_launch(b->prec, b->d_grids, s.items, s.block_start, s.nblocks, s.sync, s.nsync, b->d_error, nwg,
@@ -149,6 +361,11 @@ cs_status run(cs_batch *b, int64_t steps) {
       return CS_ERR_INVALID;
     }
   CS_HIP(hipSetDevice(b->device));
+  if (reg_path(b)) {
+    CS_TRY(run_reg(b, steps));
+    b->amax = bound;
+    return check_error_flag(b);
+  }
   auto it = b->seqs.find(steps);
   if (it != b->seqs.end() && it->second.uniform != b->uniform) {
     free_seq(it->second);
@@ -270,6 +487,16 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
         s = CS_ERR_ALLOC;
     }
     b->bfrag.push_back(d);
+    // dense weights for the register-streaming path (FP64, when the family has the stencil)
+    double *wd = nullptr;
+    if (s == CS_OK && b->prec == 0) {
+      std::vector<double> h(64, 0.0);
+      std::copy(w.begin(), w.end(), h.begin());
+      if (hipMalloc(&wd, h.size() * sizeof(double)) != hipSuccess ||
+          hipMemcpy(wd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        s = CS_ERR_ALLOC;
+    }
+    b->wdense.push_back(wd);
     double *p0 = nullptr, *p1 = nullptr;
     const size_t bytes = (size_t)cell_count(&layouts[i]) * sizeof(double);
     if (s == CS_OK && (hipMalloc(&p0, bytes) != hipSuccess || hipMalloc(&p1, bytes) != hipSuccess)) {
@@ -287,6 +514,15 @@ cs_status cs_batch_create(cs_batch **out, int32_t count, const cs_layout *layout
     b->wabs.push_back(abs_sum(w.data(), w.size()));
   }
   if (s == CS_OK && hipMalloc(&b->d_grids, (size_t)count * sizeof(cs::BatchGrid)) != hipSuccess) s = CS_ERR_ALLOC;
+  if (s == CS_OK && b->prec == 0) {
+    b->reg = true;
+    std::vector<const cs::Variant *> one;
+    for (int i = 0; i < count && b->reg; ++i) {
+      cs::reg_sequence(b->R[(size_t)i], b->shape[(size_t)i], 1, &one);
+      b->reg = !one.empty();
+    }
+    if (b->reg && hipMalloc(&b->d_rgrids, (size_t)count * sizeof(cs::RegGrid)) != hipSuccess) s = CS_ERR_ALLOC;
+  }
   if (s == CS_OK && (hipMalloc(&b->d_error, sizeof(unsigned)) != hipSuccess ||
                      hipMemset(b->d_error, 0, sizeof(unsigned)) != hipSuccess))
     s = CS_ERR_ALLOC;
@@ -305,6 +541,14 @@ cs_status cs_batch_destroy(cs_batch *b) {
   if (!b) return CS_ERR_INVALID;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (auto &kv : b->seqs) free_seq(kv.second);
+  for (auto &kv : b->rseqs) free_rseq(kv.second);
+  for (double *p : b->wdense)
+    if (p) (void)hipFree(p);
+  if (b->d_rgrids) (void)hipFree(b->d_rgrids);
+  for (hipStream_t t : b->side)
+    if (t) (void)hipStreamDestroy(t);
+  for (hipEvent_t e : b->events)
+    if (e) (void)hipEventDestroy(e);
   for (double *p : b->buf0)
     if (p) (void)hipFree(p);
   for (double *p : b->buf1)
@@ -408,8 +652,9 @@ cs_status cs_batch_describe(cs_batch *b, char *buf, int32_t len) {
   if (!b || !buf || len <= 0) return CS_ERR_INVALID;
   int64_t tiles = 0;
   for (const cs_layout &L : b->layouts) tiles += ((L.n[1] + 63) / 64) * ((L.n[2] + 63) / 64);
-  snprintf(buf, (size_t)len, "batch grids=%d precision=%s tiles=%lld workgroups=%d lds=%zu", b->count,
-           b->prec == 0 ? "fp64" : b->prec == 1 ? "bf16x3" : "fp16x2", (long long)tiles, b->max_wg,
+  snprintf(buf, (size_t)len, "batch grids=%d precision=%s path=%s tiles=%lld workgroups=%d lds=%zu", b->count,
+           b->prec == 0 ? "fp64" : b->prec == 1 ? "bf16x3" : "fp16x2",
+           reg_path(b) ? "register-streaming" : "persistent", (long long)tiles, b->max_wg,
            cs::batch_lds_bytes(b->prec));
   return CS_OK;
 }

@@ -529,6 +529,31 @@ namespace cs {
 // The FP64 kernel choice of a plan, made without a plan (nothing allocated, no device
 // touched): the main variant, the single-step variant and one variant per remainder depth
 // 1 .. kf - 1 (vrem[r], null where the family has none), as select_variants picks them.
+// The launch sequence of the register-streaming 2-D family for `steps` steps of a radius-R
+// stencil of this shape class: the plan's default depth as often as it fits, then the
+// family's largest depth that fits the rest; empty when the plan would not take the family.
+void reg_sequence(int R, int shape, int64_t steps, std::vector<const Variant *> *seq) {
+  seq->clear();
+  const Variant *v = pick_main(2, R, shape, CS_PREC_FP64, 0, 0);
+  if (!v || v->mf != 7) return;
+  for (int64_t done = 0; done < steps;) {
+    const int64_t rem = steps - done;
+    const Variant *u = nullptr;
+    if (rem >= v->kf) {
+      u = v;
+    } else {
+      const int top = max_kf(2, 7, R, shape);  // the largest compiled depth that fits
+      for (int kf = (int)std::min<int64_t>(rem, top); kf >= 1 && !u; --kf) u = find_variant(2, 7, R, shape, kf, 0);
+    }
+    if (!u) {
+      seq->clear();
+      return;
+    }
+    seq->push_back(u);
+    done += u->kf;
+  }
+}
+
 cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int tile_variant, const Variant **vmain,
                              const Variant **vone, const Variant **vrem, int vrem_cap) {
   const Variant *v = pick_main(ndim, R, shape, CS_PREC_FP64, fusion_depth, tile_variant);

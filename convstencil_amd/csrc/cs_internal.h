@@ -173,6 +173,26 @@ inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int h, int64_t slots, 
 
 typedef hipError_t (*LaunchFn)(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s);
 
+// Several grids in one launch of a register-streaming 2-D kernel (the batched workload):
+// wave w sweeps rows [y0, y1) of strip `strip` of grid items[w].grid, from that grid's
+// buffer cur ^ flip into the other one.
+struct RegGrid {
+  double *buf[2];
+  const double *wts;  // dense weights
+  Geo geo;
+  int cur;  // buffer holding the grid's state at the start of the run
+  int pad;
+};
+struct RegItem {
+  int grid, strip, y0, y1;
+};
+struct RegMulti {
+  const RegGrid *grids;
+  const RegItem *items;  // null: a single-grid launch
+  int nitems, flip;
+};
+typedef hipError_t (*LaunchMultiFn)(const RegMulti &mm, hipStream_t s);
+
 // One kernel variant of the FP64 stencil2row families.
 struct Variant {
   int id;
@@ -187,6 +207,7 @@ struct Variant {
   hipError_t (*prepare)();  // one-time function attributes (dynamic LDS above 64 KiB)
   const char *name;
   int cs = 0;  // 2-D 4x4x4: LDS storage column of tile column 0
+  LaunchMultiFn launch_multi = nullptr;  // register-streaming 2-D: several grids per launch
 };
 
 const Variant *variants_1d_f64(int *count);

@@ -219,12 +219,27 @@ __device__ __forceinline__ void sweep(const double *__restrict__ in, double *__r
 
 template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
 __global__ __launch_bounds__(NW * 64, WPE) void k2d_reg(const double *__restrict__ in, double *__restrict__ out,
-                                                        const double *__restrict__ wts, Geo g, RegSplit sp) {
+                                                        const double *__restrict__ wts, Geo g, RegSplit sp,
+                                                        RegMulti mm) {
   using C = Reg2D<R, PX, KF>;
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * NW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int64_t strip, y0, y1;
-  if (!reg_chunk(sp, g.n[1], wid, &strip, &y0, &y1)) return;
+  if (mm.items) {
+    // one item of a multi-grid launch (wave-uniform loads)
+    if (wid >= mm.nitems) return;
+    const RegItem it = mm.items[wid];
+    const RegGrid &G = mm.grids[it.grid];
+    g = G.geo;
+    in = G.buf[G.cur ^ mm.flip];
+    out = G.buf[G.cur ^ mm.flip ^ 1];
+    wts = G.wts;
+    strip = it.strip;
+    y0 = it.y0;
+    y1 = it.y1;
+  } else if (!reg_chunk(sp, g.n[1], wid, &strip, &y0, &y1)) {
+    return;
+  }
   const int64_t xs = strip * C::OX - C::H;  // interior column of the strip's first cell
   // the plain path: every row the sweep freezes or loads (rows y0 - R KF .. y1 + R KF +
   // period + RING - 3) and every column of the strip interior
@@ -264,7 +279,15 @@ static hipError_t launch_reg2d(const double *in, double *out, const double *wts,
   const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, C::H, slots, kRegEdgeCost, reg_least(R, KF, RING));
   const int64_t waves = reg_waves(sp);
   hipLaunchKernelGGL((k2d_reg<R, SH, PX, KF, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
-                     s, in, out, wts, g, sp);
+                     s, in, out, wts, g, sp, RegMulti{});
+  return hipGetLastError();
+}
+
+template <int R, int SH, int PX, int KF, int RING, int NW, int WPE>
+static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
+  if (mm.nitems <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k2d_reg<R, SH, PX, KF, RING, NW, WPE>), dim3((unsigned)((mm.nitems + NW - 1) / NW)),
+                     dim3(NW * 64), 0, s, nullptr, nullptr, nullptr, Geo{}, RegSplit{}, mm);
   return hipGetLastError();
 }
 
@@ -275,7 +298,8 @@ static hipError_t launch_reg2d(const double *in, double *out, const double *wts,
   Variant {                                                                                                   \
     ID, 7, R, SH, KF, 1, Reg2D<R, PX, KF>::OX, 4, 0, PX, RING, &launch_reg2d<R, SH, PX, KF, RING, 4, WPE>,    \
         &prepare_reg2d<R, SH, PX, KF, RING, 4, WPE>,                                                         \
-        "k2d_reg<r" #R "," #SH ",p" #PX ",kf" #KF ",ring" #RING ",wpe" #WPE ">"                             \
+        "k2d_reg<r" #R "," #SH ",p" #PX ",kf" #KF ",ring" #RING ",wpe" #WPE ">", 0,                         \
+        &launch_multi_reg2d<R, SH, PX, KF, RING, 4, WPE>                                                     \
   }
 
 // ids 700000 + (ring of 2 (2R + 1) rows) * 50000 + (star) * 10000 + (R > 1) * R * 1000 + PX * 100
