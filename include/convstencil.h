@@ -147,8 +147,9 @@ cs_status cs_batch_destroy(cs_batch *batch);
 cs_status cs_batch_upload(cs_batch *batch, int32_t grid, const double *host_in);
 /* Seeds grid i with seeds[i] (as cs_plan_fill_random). */
 cs_status cs_batch_fill_random(cs_batch *batch, const uint64_t *seeds);
-/* The persistent launch needs every workgroup resident at once; its grid-wide barriers wait
- * a bounded time.  A barrier that times out sets the batch's sticky error word: that run,
+/* FP64 batches whose grids all carry a uniform halo advance by multi-grid launches of the
+ * register-streaming kernels; the others by one persistent launch, which needs every
+ * workgroup resident at once and whose grid-wide barriers wait a bounded time.  A barrier that times out sets the batch's sticky error word: that run,
  * and every later cs_batch_run / cs_batch_time / cs_batch_download of this batch, returns
  * CS_ERR_DEVICE (the grids' contents are undefined from then on). */
 cs_status cs_batch_run(cs_batch *batch, int64_t steps, cs_report *report);
