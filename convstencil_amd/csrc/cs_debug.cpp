@@ -19,6 +19,7 @@ static const cs::Variant *family(int32_t ndim, int *n) {
   if (ndim == 6) return cs::variants_2d_stream(n);
   if (ndim == 7) return cs::variants_1d_reg(n);
   if (ndim == 8) return cs::variants_2d_reg(n);
+  if (ndim == 9) return cs::variants_3d_reg(n);
   return nullptr;
 }
 
@@ -157,6 +158,15 @@ int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t radius,
   if (ny < 1 || nx < 1 || ox < 1 || radius < 1 || kf < 1 || ring < 2 || slots < 1 || !split) return -1;
   const cs::RegSplit sp = cs::reg_split(ny, nx, ox, radius * kf, slots, cs::kRegEdgeCost, cs::reg_least(radius, kf, ring));
   const int64_t v[5] = {sp.strips, sp.he, sp.hm, sp.we, sp.wi};
+  memcpy(split, v, sizeof(v));
+  return 0;
+}
+
+int32_t cs_debug_reg3d_split(int64_t nz, int64_t ny, int64_t nx, int32_t ox, int32_t by, int64_t slots,
+                             int64_t *split) {
+  if (nz < 1 || ny < 1 || nx < 1 || ox < 1 || by < 1 || slots < 1 || !split) return -1;
+  const cs::Reg3Split sp = cs::reg3_split(nz, ny, nx, ox, by, slots);
+  const int64_t v[4] = {sp.strips, sp.ytiles, sp.zchunks, sp.zc};
   memcpy(split, v, sizeof(v));
   return 0;
 }

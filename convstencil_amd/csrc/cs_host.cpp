@@ -115,7 +115,7 @@ const Variant *family(int ndim, int *cnt, int prec = CS_PREC_FP64) {
   static const std::vector<Variant> all[3] = {
       joined3(cs::variants_1d_reg, cs::variants_1d_f64, cs::variants_1d_ablation),
       joined4(cs::variants_2d_f64, cs::variants_2d_reg, cs::variants_2d_stream, cs::variants_2d_ablation),
-      joined3(cs::variants_3d_f64, cs::variants_3d_fma, cs::variants_3d_ablation)};
+      joined4(cs::variants_3d_reg, cs::variants_3d_f64, cs::variants_3d_fma, cs::variants_3d_ablation)};
   *cnt = (int)all[ndim - 1].size();
   return all[ndim - 1].data();
 }
@@ -247,11 +247,11 @@ cs_status select_variants(cs_plan *p) {
     CS_HIP(hipMemcpy(p->bfrag, hb.data(), bytes, hipMemcpyHostToDevice));
     return CS_OK;
   }
-  // the FMA kernels (the ladder's, the 3-D mf 3 family and the 2-D register-streaming
-  // mf 7 family) read the dense weights (zero-padded to whole 64-entry rows); an mf 7 plan
-  // whose single steps run on the MFMA kernel appends that kernel's B fragments
+  // the FMA kernels (the ladder's, the 3-D mf 3 family and the register-streaming mf 7 and
+  // mf 9 families) read the dense weights (zero-padded to whole 64-entry rows); an mf 7
+  // plan whose single steps run on the MFMA kernel appends that kernel's B fragments
   p->frag_off = 0;
-  if (v->mf == 1 || v->mf == 2 || v->mf == 3 || v->mf == 7) {
+  if (v->mf == 1 || v->mf == 2 || v->mf == 3 || v->mf == 7 || v->mf == 9) {
     std::vector<double> h((p->w.size() + 63) / 64 * 64, 0.0);
     std::copy(p->w.begin(), p->w.end(), h.begin());
     if (v->mf == 7 && p->vone->mf == 4) {

@@ -173,6 +173,27 @@ inline RegSplit reg_split(int64_t ny, int64_t nx, int ox, int h, int64_t slots, 
 
 typedef hipError_t (*LaunchFn)(const double *in, double *out, const double *bfrag, const Geo &g, hipStream_t s);
 
+// One launch of the register-streaming 3-D kernels (k3d_reg.hip): strips x y-tiles x
+// z-chunks of zc planes; wave w takes y-tile w mod ytiles (neighbouring tiles share two
+// input rows and run together), then strip, then z-chunk.  As many waves as are resident
+// at once, with z-chunks of at least 8 planes.
+struct Reg3Split {
+  int64_t strips, ytiles, zchunks, zc;
+};
+
+inline Reg3Split reg3_split(int64_t nz, int64_t ny, int64_t nx, int ox, int by, int64_t slots) {
+  Reg3Split sp{};
+  sp.strips = (nx + ox - 1) / ox;
+  sp.ytiles = (ny + by - 1) / by;
+  int64_t zch = slots / (sp.strips * sp.ytiles);
+  const int64_t most = nz / 8;
+  if (zch > most) zch = most;
+  if (zch < 1) zch = 1;
+  sp.zc = (nz + zch - 1) / zch;
+  sp.zchunks = (nz + sp.zc - 1) / sp.zc;
+  return sp;
+}
+
 // Several grids in one launch of a register-streaming 2-D kernel (the batched workload):
 // wave w sweeps rows [y0, y1) of strip `strip` of grid items[w].grid, from that grid's
 // buffer cur ^ flip into the other one.
@@ -215,7 +236,8 @@ const Variant *variants_1d_reg(int *count);   // register-resident FP64 1-D (k1d
 const Variant *variants_2d_f64(int *count);
 const Variant *variants_2d_reg(int *count);     // register-streaming FP64 2-D, radius 1 (k2d_reg.hip; mf 7)
 const Variant *variants_2d_stream(int *count);  // pipelined persistent 2-D FP64 (k2d_stream.hip)
-const Variant *variants_3d_f64(int *count);   // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
+const Variant *variants_3d_f64(int *count);  // all 3-D FP64 variants: k3d2 (mf 8) first, then k3d (mf 4)
+const Variant *variants_3d_reg(int *count);  // register-streaming FP64 3-D single steps (k3d_reg.hip; mf 9)
 const Variant *variants_3d2_f64(int *count);  // 8 x 8 block family only
 const Variant *variants_2d_split(int *count);
 const Variant *variants_2d_ablation(int *count);  // --breakdown rungs 1-2 (mf 1, 2; forced by id only)

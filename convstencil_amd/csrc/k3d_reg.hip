@@ -1,0 +1,282 @@
+// k3d_reg.hip -- register-streaming FP64 3-D single-step kernels for gfx950 (radius 1, the
+// 27-point box and the 7-point star).
+//
+// Replaces gpu_box_3d1r_kernel and gpu_star_3d1r_step3_kernel (reference
+// src/3d/gpu_box.cu:186-238, src/3d/gpu_star.cu) for physical single steps.  A wave owns a
+// tile of 64 PX columns by BY rows and streams its z range, one input plane per iteration,
+// with no LDS and no barrier: lane l holds columns l PX .. l PX + PX - 1 of the BY + 2 input
+// rows a plane contributes (the tile's rows and one on either side), the column
+// neighbours beyond a lane's run come from lanes l -+ 1 (v_mov_b32_dpp wave shifts), and
+// the columns just outside the tile from one extra load per row that lanes 0 and 63 hold
+// (the shift's "old" operand, taken where the shifted lane does not exist).  An incoming
+// plane z adds its dz = +1 taps to output plane z - 1 (then complete and stored), its dz = 0
+// taps to plane z and starts plane z + 1 with its dz = -1 taps: three partial planes per
+// wave rotate every three iterations.  One step per launch, so cells outside the interior
+// are simply not stored.  Sum order per output: dz, dy, dx ascending (the reference's).
+#include "cs_internal.h"
+
+namespace cs {
+
+// lane l receives lane l - 1's value, lane 0 keeps `old`
+__device__ __forceinline__ double lane_from_lower_or(double v, double old) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// lane l receives lane l + 1's value, lane 63 keeps `old`
+__device__ __forceinline__ double lane_from_upper_or(double v, double old) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int PX, int BY>
+struct Reg3D {
+  static constexpr int OX = 64 * PX;  // output columns per wave
+  static constexpr int NR = BY + 2;   // input rows per plane
+};
+
+template <int SH, int PX, int BY, int RING, bool EDGE>
+struct Sweep3D {
+  using C = Reg3D<PX, BY>;
+  static constexpr int NR = C::NR;
+  static constexpr bool BOX = SH == SHAPE3_BOX;
+  static constexpr int PER = RING % 3 == 0 ? RING : 3 * RING;
+
+  const double *__restrict__ in;
+  double *__restrict__ out;
+  double w[27];
+  int64_t nz, ny, nx, ez, ey, ex, oz, oy, ox;
+  int64_t y0, cx, xe;  // first output row; this lane's first column; its extra column
+  int lane;
+  int64_t colc[PX], xec;  // checked path: the columns clamped into the array
+  bool colout[PX];
+  double ring[RING][NR][PX], ext[RING][NR];
+  double acc[3][BY][PX];
+
+  // input plane z, rows y0 - 1 .. y0 + BY: the lane's run and its extra cell (lane 0: the
+  // column left of the tile, lane 63: the column right of it).  The checked path clamps
+  // plane, row and column into the array: a cell outside it is at least one cell from the
+  // interior (the halo is at least one cell wide), so whatever it holds reaches only outputs
+  // that are never stored -- and no select waits on the load.
+  __device__ __forceinline__ void load_plane(int64_t z, double (&q)[NR][PX], double (&e)[NR]) {
+    const int64_t az = oz + z;
+    if constexpr (!EDGE) {
+      const double *p = in + ((az * ey) + oy + y0 - 1) * ex + ox;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+#pragma unroll
+        for (int j = 0; j < PX; ++j) q[r][j] = p[r * ex + cx + j];
+        e[r] = p[r * ex + xe];
+      }
+    } else {
+      const int64_t cz = az < 0 ? 0 : az >= ez ? ez - 1 : az;  // wave-uniform
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int64_t ay = oy + y0 - 1 + r;
+        const int64_t cy = ay < 0 ? 0 : ay >= ey ? ey - 1 : ay;
+        const double *p = in + (cz * ey + cy) * ex;
+#pragma unroll
+        for (int j = 0; j < PX; ++j) q[r][j] = p[colc[j]];
+        e[r] = p[xec];
+      }
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void phase(int64_t it, int64_t zs, int64_t z0, int64_t z1) {
+    const int64_t z = zs + it + K;
+    load_plane(z + RING - 1, ring[(K + RING - 1) % RING], ext[(K + RING - 1) % RING]);
+    const double(&q)[NR][PX] = ring[K % RING];
+    const double(&e)[NR] = ext[K % RING];
+    // the rows extended by one cell on either side: qe[r][1 + c], c = -1 .. PX
+    double qe[NR][PX + 2];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+#pragma unroll
+      for (int j = 0; j < PX; ++j) qe[r][1 + j] = q[r][j];
+      qe[r][0] = lane_from_lower_or(q[r][PX - 1], e[r]);
+      qe[r][PX + 1] = lane_from_upper_or(q[r][0], e[r]);
+    }
+    // output plane z - dz sits in slot (z - dz - zs) mod 3 = (K - dz) mod 3
+#pragma unroll
+    for (int dz = 1; dz >= -1; --dz) {
+      double(&T)[BY][PX] = acc[((K - dz) % 3 + 3) % 3];
+#pragma unroll
+      for (int i = 0; i < BY; ++i)
+#pragma unroll
+        for (int j = 0; j < PX; ++j) {
+          double t = dz == -1 ? 0.0 : T[i][j];
+          bool first = dz == -1;
+#pragma unroll
+          for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+              if (!BOX && (dz != 0) + (dy != 0) + (dx != 0) > 1) continue;
+              const double wv = w[((dz + 1) * 3 + dy + 1) * 3 + dx + 1], v = qe[i + 1 + dy][1 + j + dx];
+              t = first ? wv * v : __builtin_fma(wv, v, t);
+              first = false;
+            }
+          T[i][j] = t;
+        }
+      if (dz < 1) {
+#pragma unroll
+        for (int i = 0; i < BY; ++i)
+#pragma unroll
+          for (int j = 0; j < PX; ++j) asm volatile("" : "+v"(T[i][j]));
+      }
+    }
+    // plane z - 1 is complete
+    const int64_t zo = z - 1;
+    if (zo >= z0 && zo < z1) {
+      const double(&A)[BY][PX] = acc[(K - 1 + 3) % 3];
+      double *p = out + ((oz + zo) * ey + oy + y0) * ex + ox + cx;
+#pragma unroll
+      for (int i = 0; i < BY; ++i) {
+        if (EDGE && y0 + i >= ny) continue;  // wave-uniform
+#pragma unroll
+        for (int j = 0; j < PX; ++j)
+          if (!EDGE || colout[j]) p[i * ex + j] = A[i][j];
+      }
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void phases(int64_t it, int64_t zs, int64_t z0, int64_t z1) {
+    if constexpr (K < PER) {
+      phase<K>(it, zs, z0, z1);
+      phases<K + 1>(it, zs, z0, z1);
+    }
+  }
+
+  __device__ __forceinline__ void run(int64_t z0, int64_t z1) {
+    const int64_t zs = z0 - 1;
+    const int64_t iters = (z1 - z0) + 2;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int i = 0; i < BY; ++i)
+#pragma unroll
+        for (int j = 0; j < PX; ++j) acc[k][i][j] = 0.0;
+    // the planes are requested in order (no reordering across the scheduling barrier), so
+    // the loop's first wait counts the loads issued after its plane rather than none
+#pragma unroll
+    for (int k = 0; k < RING - 1; ++k) {
+      load_plane(zs + k, ring[k], ext[k]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int64_t it = 0; it < iters; it += PER) phases<0>(it, zs, z0, z1);
+  }
+};
+
+template <int SH, int PX, int BY, int RING, bool EDGE>
+__device__ __forceinline__ void sweep3(const double *__restrict__ in, double *__restrict__ out,
+                                       const double *__restrict__ wts, const Geo &g, int64_t x0, int64_t y0,
+                                       int64_t z0, int64_t z1, int lane) {
+  Sweep3D<SH, PX, BY, RING, EDGE> S;
+  S.in = in;
+  S.out = out;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) S.w[i] = wts[i];
+  S.nz = g.n[0];
+  S.ny = g.n[1];
+  S.nx = g.n[2];
+  S.ez = g.ext[0];
+  S.ey = g.ext[1];
+  S.ex = g.ext[2];
+  S.oz = g.org[0];
+  S.oy = g.org[1];
+  S.ox = g.org[2];
+  S.y0 = y0;
+  S.lane = lane;
+  S.cx = x0 + (int64_t)lane * PX;
+  // lane 0 holds the column left of the tile, lane 63 the one right of it, the others
+  // their own first column (never used)
+  S.xe = lane == 0 ? x0 - 1 : lane == 63 ? x0 + 64 * PX : S.cx;
+  S.xec = S.ox + S.xe < 0 ? 0 : S.ox + S.xe >= S.ex ? S.ex - 1 : S.ox + S.xe;
+#pragma unroll
+  for (int j = 0; j < PX; ++j) {
+    const int64_t c = S.cx + j;
+    S.colc[j] = S.ox + c < 0 ? 0 : S.ox + c >= S.ex ? S.ex - 1 : S.ox + c;
+    S.colout[j] = c >= 0 && c < S.nx;
+  }
+  S.run(z0, z1);
+}
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+__global__ __launch_bounds__(NW * 64, WPE) void k3d_reg(const double *__restrict__ in, double *__restrict__ out,
+                                                        const double *__restrict__ wts, Geo g, Reg3Split sp) {
+  using C = Reg3D<PX, BY>;
+  constexpr int PER = RING % 3 == 0 ? RING : 3 * RING;
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * NW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t yt = wid % sp.ytiles, rest = wid / sp.ytiles;
+  const int64_t strip = rest % sp.strips, zch = rest / sp.strips;
+  if (zch >= sp.zchunks) return;
+  const int64_t z0 = zch * sp.zc;
+  if (z0 >= g.n[0]) return;
+  const int64_t z1 = z0 + sp.zc < g.n[0] ? z0 + sp.zc : g.n[0];
+  const int64_t y0 = yt * BY, x0 = strip * C::OX;
+  // the plain path: every cell the sweep loads (planes z0 - 1 .. z1 + period + RING - 2,
+  // rows y0 - 1 .. y0 + BY, columns x0 - 1 .. x0 + 64 PX) and stores is interior
+  if (x0 >= 1 && x0 + C::OX + 1 <= g.n[2] && y0 >= 1 && y0 + BY + 1 <= g.n[1] && z0 >= 1 &&
+      z1 + PER + RING <= g.n[0])
+    sweep3<SH, PX, BY, RING, false>(in, out, wts, g, x0, y0, z0, z1, lane);
+  else
+    sweep3<SH, PX, BY, RING, true>(in, out, wts, g, x0, y0, z0, z1, lane);
+}
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+struct Reg3Slots {
+  static int64_t value;
+};
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+int64_t Reg3Slots<SH, PX, BY, RING, NW, WPE>::value = 0;
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+static hipError_t prepare_reg3d() {
+  int nb = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, (const void *)k3d_reg<SH, PX, BY, RING, NW, WPE>, NW * 64, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  Reg3Slots<SH, PX, BY, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
+  return hipSuccess;
+}
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+static hipError_t launch_reg3d(const double *in, double *out, const double *wts, const Geo &g, hipStream_t s) {
+  using C = Reg3D<PX, BY>;
+  using S = Reg3Slots<SH, PX, BY, RING, NW, WPE>;
+  const int64_t slots = S::value > 0 ? S::value : 2048;
+  const Reg3Split sp = reg3_split(g.n[0], g.n[1], g.n[2], C::OX, BY, slots);
+  const int64_t waves = sp.strips * sp.ytiles * sp.zchunks;
+  hipLaunchKernelGGL((k3d_reg<SH, PX, BY, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
+                     s, in, out, wts, g, sp);
+  return hipGetLastError();
+}
+
+// Variant::mf 9 marks this family: by = rows per wave, bx = columns per wave, px = columns
+// per lane, st = planes in the load ring; the weights are the dense 3 x 3 x 3 block.
+#define CS_VR3D(ID, SH, PX, BY, RING, WPE)                                                                   \
+  Variant {                                                                                                  \
+    ID, 9, 1, SH, 1, BY, Reg3D<PX, BY>::OX, 4, 0, PX, RING, &launch_reg3d<SH, PX, BY, RING, 4, WPE>,        \
+        &prepare_reg3d<SH, PX, BY, RING, 4, WPE>,                                                           \
+        "k3d_reg<r1," #SH ",p" #PX ",by" #BY ",ring" #RING ",wpe" #WPE ">"                                  \
+  }
+
+// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING
+static const Variant kVariantsReg3D[] = {
+    CS_VR3D(902023, SHAPE3_BOX, 2, 2, 3, 3),  CS_VR3D(904023, SHAPE3_BOX, 4, 2, 3, 2),
+    CS_VR3D(902043, SHAPE3_BOX, 2, 4, 3, 2),  CS_VR3D(912023, SHAPE3_STAR, 2, 2, 3, 4),
+    CS_VR3D(914023, SHAPE3_STAR, 4, 2, 3, 2), CS_VR3D(912043, SHAPE3_STAR, 2, 4, 3, 2),
+};
+
+const Variant *variants_3d_reg(int *count) {
+  *count = (int)(sizeof(kVariantsReg3D) / sizeof(kVariantsReg3D[0]));
+  return kVariantsReg3D;
+}
+
+}  // namespace cs

@@ -157,6 +157,45 @@ def test_every_variant(gpu, vid, R, shape, vname):
     _check(L, got, oracle.run(L, offs, w, x, 5))
 
 
+def _reg_variants_3d():
+    import ctypes
+    from convstencil_amd import _lib
+    lib = _lib.load()
+    lib.cs_debug_variant_count.restype = ctypes.c_int32
+    lib.cs_debug_variant_info.restype = ctypes.c_int32
+    lib.cs_debug_variant_info.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.c_char_p, ctypes.c_int32]
+    out = []
+    for i in range(lib.cs_debug_variant_count(9)):
+        info = (ctypes.c_int64 * 12)()
+        name = ctypes.create_string_buffer(128)
+        lib.cs_debug_variant_info(9, i, info, name, 128)
+        out.append((int(info[0]), int(info[3]), name.value.decode()))
+    return out
+
+
+REG_3D = _reg_variants_3d()
+
+
+# the register-streaming single-step kernels (k3d_reg.hip, Variant::mf 9): plain-path
+# waves inside the large grid, ragged strips and tiles, halos of odd widths
+@pytest.mark.parametrize("vid,shape,vname", REG_3D, ids=[v[2] for v in REG_3D])
+def test_every_reg_variant(gpu, vid, shape, vname):
+    name = "box3d1r" if shape == 0 else "star3d1r"
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), vid)
+    for L, steps in [(Layout.with_halo([50, 21, 600], 1), 1), (Layout.with_halo([37, 45, 140], 1), 5),
+                     (Layout(3, [29, 14, 300], [2, 1, 3], [25, 11, 290]), 3), (Layout.with_halo([3, 1, 5], 1), 2)]:
+        print(f"case {vname} {L.n} T={steps}", flush=True)
+        x = oracle.fill_data(L, vid + steps)
+        with Plan(L, 1, dense_weights(3, 1, offs, w), variant=vid) as p:
+            assert f"variant={vid}" in p.describe()
+            p.upload(x)
+            p.run(steps)
+            got = p.download()
+        _check(L, got, oracle.run(L, offs, w, x, steps))
+
+
 # the register-blocked FMA kernels (k3d_fma.hip, Variant::mf 3): radius 2 and 3, each
 # shape class, partial tiles and z-chunks
 FMA_3D = [(35302, 3, "box"), (35322, 3, "diamond"), (35312, 3, "star"), (35303, 3, "box"), (35323, 3, "diamond"),

@@ -1803,3 +1803,140 @@ def test_reg2d_split_covers_once(ny, nx, ox, R, kf, ring, slots):
     assert (cover == 1).all()
     # a plain-path chunk needs the rows it freezes and loads inside the grid
     assert he >= R * kf + reg_period(R, ring) + ring - 2
+
+
+# ---------------------------------------------------------------- 3-D register-streaming family
+
+def reg3d_split(nz, ny, nx, ox, by, slots):
+    lib = _dbg()
+    lib.cs_debug_reg3d_split.restype = ctypes.c_int32
+    lib.cs_debug_reg3d_split.argtypes = [ctypes.c_int64] * 3 + [ctypes.c_int32, ctypes.c_int32, ctypes.c_int64,
+                                                                ctypes.POINTER(ctypes.c_int64)]
+    split = (ctypes.c_int64 * 4)()
+    assert lib.cs_debug_reg3d_split(nz, ny, nx, ox, by, slots, split) == 0
+    return [int(s) for s in split]
+
+
+def emulate_3d_reg(v, layout, dense, x, slots):
+    """k3d_reg: wave w takes y-tile w mod y-tiles, then strip, then z-chunk; it owns BY rows
+    by 64 PX columns (lane-major: the lanes' runs make one flat row, extended by the column
+    on either side that lanes 0 and 63 load) and streams input planes zs = z0 - 1 ..,
+    loaded RING - 1 iterations ahead; plane z adds its dz = +1 taps to output plane z - 1
+    (then stored), its dz = 0 taps to plane z and starts plane z + 1 with its dz = -1 taps,
+    in (dy, dx) ascending order per plane.  Loads outside the array read zero on the checked
+    path; the plain path must stay inside the interior's one-cell neighbourhood.  Checks
+    every load and store against the array and counts the stores per cell."""
+    PX, BY, RING = v["px"], v["by"], v["st"]
+    OX = 64 * PX
+    PER = RING if RING % 3 == 0 else 3 * RING
+    box = v["shape"] == 0
+    assert v["mf"] == 9 and v["R"] == 1 and v["kf"] == 1 and v["bx"] == OX and v["lds"] == 0
+    ez, ey, ex = layout.ext
+    oz, oy, ox = layout.org
+    nz, ny, nx = layout.n
+    X = x.reshape(ez, ey, ex)
+    w = np.asarray(dense, dtype=np.float64).reshape(3, 3, 3)
+    out = np.zeros((ez, ey, ex))
+    hits = np.zeros((ez, ey, ex), dtype=np.int64)
+    strips, ytiles, zchunks, zc = reg3d_split(nz, ny, nx, OX, BY, slots)
+    assert strips * ytiles * zchunks <= max(slots, strips * ytiles)
+    plain = 0
+    for wid in range(strips * ytiles * zchunks):
+        yt, rest = wid % ytiles, wid // ytiles
+        strip, zch = rest % strips, rest // strips
+        z0 = zch * zc
+        if z0 >= nz:
+            continue
+        z1 = min(z0 + zc, nz)
+        y0, x0 = yt * BY, strip * OX
+        fast = x0 >= 1 and x0 + OX + 1 <= nx and y0 >= 1 and y0 + BY + 1 <= ny and z0 >= 1 and z1 + PER + RING <= nz
+        plain += fast
+        ecols = x0 - 1 + np.arange(OX + 2)  # columns x0 - 1 .. x0 + OX
+        erows = y0 - 1 + np.arange(BY + 2)
+
+        def load(z):
+            if fast:
+                assert 0 <= z < nz and erows[0] >= 0 and erows[-1] < ny and ecols[0] >= 0 and ecols[-1] < nx
+                return X[oz + z][np.ix_(oy + erows, ox + ecols)].copy()
+            ok = ((oy + erows >= 0) & (oy + erows < ey))[:, None] & ((ox + ecols >= 0) & (ox + ecols < ex))[None, :]
+            if not 0 <= oz + z < ez:
+                ok[:] = False
+            ry = np.clip(oy + erows, 0, ey - 1)
+            cx = np.clip(ox + ecols, 0, ex - 1)
+            return np.where(ok, X[min(max(oz + z, 0), ez - 1)][np.ix_(ry, cx)], 0.0)
+
+        zs, iters = z0 - 1, (z1 - z0) + 2
+        acc = [np.zeros((BY, OX)) for _ in range(3)]
+        ring = [load(zs + k) for k in range(RING - 1)] + [None]
+        colout = x0 + np.arange(OX) < nx
+        for it in range(0, iters, PER):
+            for K in range(PER):
+                z = zs + it + K
+                ring[(K + RING - 1) % RING] = load(z + RING - 1)
+                qe = ring[K % RING]
+                for dz in (1, 0, -1):
+                    sl = (K - dz) % 3
+                    t = None if dz == -1 else acc[sl]
+                    for dy in (-1, 0, 1):
+                        for dx in (-1, 0, 1):
+                            if not box and (dz != 0) + (dy != 0) + (dx != 0) > 1:
+                                continue
+                            term = w[dz + 1, dy + 1, dx + 1] * qe[1 + dy:1 + dy + BY, 1 + dx:1 + dx + OX]
+                            t = term if t is None else t + term
+                    acc[sl] = t
+                zo = z - 1
+                if z0 <= zo < z1:
+                    A = acc[(K - 1) % 3]
+                    for i in range(BY):
+                        if y0 + i >= ny:
+                            continue
+                        c = ox + x0 + np.arange(OX)[colout]
+                        assert 0 <= oz + zo < ez and 0 <= oy + y0 + i < ey and (c < ex).all()
+                        out[oz + zo, oy + y0 + i, c] = A[i, colout]
+                        hits[oz + zo, oy + y0 + i, c] += 1
+    inner = hits[oz:oz + nz, oy:oy + ny, ox:ox + nx]
+    assert (inner == 1).all(), (int(inner.min()), int(inner.max()))
+    assert hits.sum() == inner.sum()
+    emulate_3d_reg.plain_waves = plain
+    return out.reshape(-1)
+
+
+VARIANTS_REG3D = variants(9)
+
+
+@pytest.mark.parametrize("v", VARIANTS_REG3D, ids=[v["name"] for v in VARIANTS_REG3D])
+def test_variant_emulation_3d_reg(v):
+    # three strips, y-tiles and z-chunks (a plain-path wave in the middle), halos of odd
+    # widths (masks at the array's edges), one ragged strip and tile, and a tiny grid
+    OX, BY = v["bx"], v["by"]
+    rng = np.random.default_rng(v["id"])
+    w = rng.uniform(0.05, 1.0, (3, 3, 3))
+    zz, yy, xx = np.meshgrid(*[np.arange(-1, 2)] * 3, indexing="ij")
+    if v["shape"] == 1:
+        w[((zz != 0).astype(int) + (yy != 0) + (xx != 0)) > 1] = 0.0
+    w /= w.sum()
+    dense = w.reshape(-1)
+    ny = 3 * BY - 1
+    cases = [(Layout.with_halo([40, ny, 3 * OX - 5], 1), 3 * 3 * 3, True),
+             (Layout(3, [29, ny + 3, 2 * OX + 9], [2, 1, 3], [25, ny, 2 * OX + 3]), 4, False),
+             (Layout.with_halo([3, 1, 5], 1), 64, False)]
+    for layout, slots, want_plain in cases:
+        x = oracle.fill_data(layout, v["id"] + slots)
+        got = emulate_3d_reg(v, layout, dense, x, slots)
+        if want_plain:
+            assert emulate_3d_reg.plain_waves > 0
+        offs, ww = oracle.dense_to_taps(3, 1, dense)
+        want = oracle.run(layout, offs, ww, x, 1)
+        gi, wi = layout.interior(got), layout.interior(want)
+        err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+        assert err < 1e-13, (v["name"], slots, err)
+
+
+@pytest.mark.parametrize("nz,ny,nx,ox,by,slots", [(512, 512, 512, 128, 2, 2048), (512, 512, 512, 256, 2, 1024),
+                                                  (3, 1, 5, 128, 4, 64), (100, 30, 1000, 128, 2, 100000),
+                                                  (64, 7, 129, 256, 4, 1)])
+def test_reg3d_split_covers_once(nz, ny, nx, ox, by, slots):
+    strips, ytiles, zchunks, zc = reg3d_split(nz, ny, nx, ox, by, slots)
+    assert strips * ox >= nx > (strips - 1) * ox and ytiles * by >= ny > (ytiles - 1) * by
+    assert zchunks * zc >= nz > (zchunks - 1) * zc
+    assert zc >= min(8, nz) and strips * ytiles * zchunks <= max(slots, strips * ytiles)
