@@ -31,6 +31,33 @@ __device__ __forceinline__ double lane_from_upper(double v) {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+// Raw buffer resource over `bytes` bytes from p (no stride, no format; word 3 as for the
+// gfx9 family).  The hardware drops a buffer store whose offset lies past `bytes`, so a
+// store that only some rows or lanes want can still be issued by every lane on every row:
+// no branch, and the compiler's waits count the stores exactly.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_over(double *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+}
+constexpr int kBufferDrop = 0x40000000;  // a store offset past every range used
+
+__device__ __forceinline__ void buffer_store2(__amdgpu_buffer_rsrc_t r, int off, double a, double b) {
+  u4 v;
+  v.x = (unsigned)__double2loint(a);
+  v.y = (unsigned)__double2hiint(a);
+  v.z = (unsigned)__double2loint(b);
+  v.w = (unsigned)__double2hiint(b);
+  __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, 0);
+}
+
+__device__ __forceinline__ void buffer_store1(__amdgpu_buffer_rsrc_t r, int off, double a) {
+  u2 v;
+  v.x = (unsigned)__double2loint(a);
+  v.y = (unsigned)__double2hiint(a);
+  __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+}
 
 // Geometry of one padded grid as the kernels see it.  All coordinates slowest first.
 struct Geo {

@@ -45,12 +45,20 @@ struct Sweep2D {
   static constexpr bool BOX = SH == SHAPE_BOX;
   static constexpr int PER = reg_period(R, RING);
   static constexpr int NL = (R + PX - 1) / PX;  // lanes a halo reaches across
+  // The box radius-3 kernels hold more partial sums than the register file and spill; they
+  // keep plain conditional stores, zero-selected checked loads and a prologue in free
+  // order, which measured faster for them (box2d3r 8192^2, depth 2: 348 GStencil/s against
+  // 102-182 with the buffer stores, clamped loads and ordered prologue of the others)
+  static constexpr bool SPILLS = R > 1 && BOX;
 
   const double *__restrict__ in;
   double *__restrict__ out;
   double w[D * D];
-  int64_t ny, nx, pitch, ey, oy, ox, cx;
-  bool colin[PX], colarr[PX], colout[PX];
+  int64_t ny, nx, pitch, ey, oy, ox, cx, xs;
+  int lane;
+  int dcol[PX];  // checked path: column j + dcol[j] of the lane's run is column j clamped into
+                 // the array (|dcol| <= 64 PX: a strip ends within 64 PX columns of the interior)
+  bool colin[PX], colout[PX];
   double ring[RING][PX];
   double acc[KF][NS][PX];
 
@@ -61,15 +69,50 @@ struct Sweep2D {
       const double *p = in + ay * pitch + ox + cx;
 #pragma unroll
       for (int j = 0; j < PX; ++j) dst[j] = p[j];
-    } else {
+    } else if constexpr (SPILLS) {
       const bool rowok = ay >= 0 && ay < ey;  // wave-uniform
       const double *p = in + (rowok ? ay : 0) * pitch + ox + cx;
 #pragma unroll
       for (int j = 0; j < PX; ++j) {
-        const bool ok = rowok && colarr[j];
+        const bool ok = rowok && dcol[j] == 0;
         const double v = *(ok ? p + j : in);
         dst[j] = ok ? v : 0.0;
       }
+    } else {
+      // row and columns clamped into the array: a cell outside it lies more than R cells
+      // from the interior (the halo is at least R wide) and outside the interior every
+      // cell is frozen, so what it holds reaches no stored output -- and no select waits
+      // on the load
+      const int64_t cy = ay < 0 ? 0 : ay >= ey ? ey - 1 : ay;  // wave-uniform
+      const double *p = in + cy * pitch + ox + cx;
+#pragma unroll
+      for (int j = 0; j < PX; ++j) dst[j] = p[j + dcol[j]];
+    }
+  }
+
+  // Output row yo of level KF from q, by buffer stores over the strip's row: a row outside
+  // [y0, y1) gets an empty range and a lane's cells outside the strip's outputs an offset
+  // past it, so every lane issues the same stores on every row (no branch in the loop).
+  // The plain path stores a lane's cells in pairs (with H even both cells of a pair are
+  // outputs or neither).
+  __device__ __forceinline__ void store_row(int64_t yo, bool ok, const double (&q)[PX]) {
+    if constexpr (SPILLS) {
+      if (ok) {
+        double *p = out + (oy + yo) * pitch + ox + cx;
+#pragma unroll
+        for (int j = 0; j < PX; ++j)
+          if (colout[j]) p[j] = q[j];
+      }
+      return;
+    }
+    const __amdgpu_buffer_rsrc_t rs = buffer_over(out + (oy + yo) * pitch + ox + xs, ok ? 64 * PX * 8 : 0);
+    const int base = lane * PX * 8;
+    if constexpr (!EDGE && C::H % 2 == 0 && PX % 2 == 0) {
+#pragma unroll
+      for (int j = 0; j < PX; j += 2) buffer_store2(rs, colout[j] ? base + j * 8 : kBufferDrop, q[j], q[j + 1]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < PX; ++j) buffer_store1(rs, colout[j] ? base + j * 8 : kBufferDrop, q[j]);
     }
   }
 
@@ -159,12 +202,7 @@ struct Sweep2D {
     }
     // q: level KF, row r - R KF
     const int64_t yo = r - C::H;
-    if (yo >= y0 && yo < y1) {
-      double *p = out + (oy + yo) * pitch + ox + cx;
-#pragma unroll
-      for (int j = 0; j < PX; ++j)
-        if (colout[j]) p[j] = q[j];
-    }
+    store_row(yo, yo >= y0 && yo < y1, q);
   }
 
   template <int K>
@@ -184,15 +222,19 @@ struct Sweep2D {
       for (int k = 0; k < NS; ++k)
 #pragma unroll
         for (int j = 0; j < PX; ++j) acc[s][k][j] = 0.0;
-    // the rows are requested in order (no reordering across the scheduling barrier), so the
-    // loop's waits count the loads issued after each row: with the prologue's rows issued
-    // youngest-first, the first wait of every period drains the whole ring.  The box
-    // radius-3 kernels, which spill, keep the free order (measured at 8192^2: 348 GStencil/s
-    // against 178 with the barrier).
+    // The loop head's wait is one instruction for the first entry and every later one, so
+    // the prologue issues the loop's own pattern: each row's load, in order (no reordering
+    // across the scheduling barrier), then one row of dropped stores.  The wait then counts
+    // as many operations after the row it needs as in the steady state (with the rows
+    // issued youngest-first it drained the whole ring once per period).
 #pragma unroll
     for (int k = 0; k < RING - 1; ++k) {
       load_row(ys + k, ring[k]);
-      if constexpr (!(R > 1 && SH == SHAPE_BOX)) __builtin_amdgcn_sched_barrier(0);
+      if constexpr (!SPILLS) {
+        const double none[PX] = {};
+        store_row(ys, false, none);
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     // the partial-sum slots repeat every NS iterations, the ring every RING
     for (int64_t it = 0; it < iters; it += PER) phases<0>(it, ys, y0, y1);
@@ -214,12 +256,14 @@ __device__ __forceinline__ void sweep(const double *__restrict__ in, double *__r
   S.ey = g.ext[1];
   S.oy = g.org[1];
   S.ox = g.org[2];
+  S.xs = xs;
+  S.lane = lane;
   S.cx = xs + (int64_t)lane * PX;
 #pragma unroll
   for (int j = 0; j < PX; ++j) {
     const int64_t c = S.cx + j, d = (int64_t)lane * PX + j;
     S.colin[j] = c >= 0 && c < S.nx;
-    S.colarr[j] = S.ox + c >= 0 && S.ox + c < S.pitch;
+    S.dcol[j] = (int)(S.ox + c < 0 ? -(S.ox + c) : S.ox + c >= S.pitch ? S.pitch - 1 - (S.ox + c) : 0);
     S.colout[j] = d >= C::H && d < C::H + C::OX && c < S.nx;  // c >= 0 wherever d >= H
   }
   S.run(y0, y1);
@@ -321,10 +365,14 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
 static const Variant kVariantsReg2D[] = {
     CS_VR2D(750212, 1, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, 1, SHAPE_BOX, 2, 12, 3, 3),
     CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),   CS_VR2D(750216, 1, SHAPE_BOX, 2, 16, 6, 2),
-    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),
-    CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
+    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700407, 1, SHAPE_BOX, 4, 7, 3, 2),
+    CS_VR2D(700406, 1, SHAPE_BOX, 4, 6, 3, 2),   CS_VR2D(750406, 1, SHAPE_BOX, 4, 6, 6, 2),
+    CS_VR2D(700308, 1, SHAPE_BOX, 3, 8, 3, 2),   CS_VR2D(700306, 1, SHAPE_BOX, 3, 6, 3, 2),
+    CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),   CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),
+    CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
     CS_VR2D(710212, 1, SHAPE_STAR, 2, 12, 3, 3), CS_VR2D(760212, 1, SHAPE_STAR, 2, 12, 6, 2),
     CS_VR2D(760216, 1, SHAPE_STAR, 2, 16, 6, 2), CS_VR2D(760208, 1, SHAPE_STAR, 2, 8, 6, 3),
+    CS_VR2D(710406, 1, SHAPE_STAR, 4, 6, 3, 2),  CS_VR2D(710308, 1, SHAPE_STAR, 3, 8, 3, 2),
     CS_VR2D(710404, 1, SHAPE_STAR, 4, 4, 3, 2),  CS_VR2D(710402, 1, SHAPE_STAR, 4, 2, 3, 2),
     CS_VR2D(710401, 1, SHAPE_STAR, 4, 1, 3, 2),
     // radius 3 (star2d3r, box2d3r): rings of 7 rows

@@ -129,15 +129,26 @@ struct Sweep3D {
     }
     // plane z - 1 is complete
     const int64_t zo = z - 1;
-    if (zo >= z0 && zo < z1) {
-      const double(&A)[BY][PX] = acc[(K - 1 + 3) % 3];
-      double *p = out + ((oz + zo) * ey + oy + y0) * ex + ox + cx;
+    store_plane(zo, zo >= z0 && zo < z1, acc[(K - 1 + 3) % 3]);
+  }
+
+  // Output plane zo's rows by buffer stores over each row of the tile: a plane outside
+  // [z0, z1) or a row outside the interior gets an empty range and a cell outside it an
+  // offset past the range, so every lane issues the same stores on every plane (no branch
+  // in the loop, and the compiler's waits count the stores exactly).
+  __device__ __forceinline__ void store_plane(int64_t zo, bool ok, const double (&A)[BY][PX]) {
+    const int base = lane * PX * 8;
 #pragma unroll
-      for (int i = 0; i < BY; ++i) {
-        if (EDGE && y0 + i >= ny) continue;  // wave-uniform
+    for (int i = 0; i < BY; ++i) {
+      const bool rok = ok && (!EDGE || y0 + i < ny);
+      const __amdgpu_buffer_rsrc_t rs =
+          buffer_over(out + ((oz + zo) * ey + oy + y0 + i) * ex + ox + cx - lane * PX, rok ? C::OX * 8 : 0);
+      if constexpr (!EDGE && PX % 2 == 0) {
 #pragma unroll
-        for (int j = 0; j < PX; ++j)
-          if (!EDGE || colout[j]) p[i * ex + j] = A[i][j];
+        for (int j = 0; j < PX; j += 2) buffer_store2(rs, base + j * 8, A[i][j], A[i][j + 1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < PX; ++j) buffer_store1(rs, !EDGE || colout[j] ? base + j * 8 : kBufferDrop, A[i][j]);
       }
     }
   }
@@ -159,11 +170,14 @@ struct Sweep3D {
       for (int i = 0; i < BY; ++i)
 #pragma unroll
         for (int j = 0; j < PX; ++j) acc[k][i][j] = 0.0;
-    // the planes are requested in order (no reordering across the scheduling barrier), so
-    // the loop's first wait counts the loads issued after its plane rather than none
+    // The loop head's wait is one instruction for the first entry and every later one, so
+    // the prologue issues the loop's own pattern: each plane's loads, in order (no
+    // reordering across the scheduling barrier), then one plane of dropped stores.
 #pragma unroll
     for (int k = 0; k < RING - 1; ++k) {
       load_plane(zs + k, ring[k], ext[k]);
+      const double none[BY][PX] = {};
+      store_plane(zs, false, none);
       __builtin_amdgcn_sched_barrier(0);
     }
     for (int64_t it = 0; it < iters; it += PER) phases<0>(it, zs, z0, z1);
