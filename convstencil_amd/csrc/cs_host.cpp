@@ -170,8 +170,11 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
     // 1-D: the register-resident kernels (mf 6) at their deepest compiled fusion, else the
     // 4 x 4 x 4 blocks; 2-D: the register-streaming kernels (mf 7), else the dilated 8 x 8
     // blocks; 3-D: the 8 x 8 block family (mf 8) where it exists,
-    // else the register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips
-    const int fams[3] = {nd == 3 ? 8 : nd == 1 ? 6 : 7, nd == 3 ? 3 : mf, mf};
+    // else the register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips.
+    // The 2-D box radius-3 register kernels spill; the dilated tables are faster there
+    // (8192^2, 64 steps: 304-308 GStencil/s at depth 2-3 against 184-195).
+    const bool reg2d = !(nd == 2 && R > 1 && shape == cs::SHAPE_BOX);
+    const int fams[3] = {nd == 3 ? 8 : nd == 1 ? 6 : reg2d ? 7 : mf, nd == 3 ? 3 : mf, mf};
     // a requested depth that some family compiles exactly is honoured exactly
     if (fusion_depth > 0)
       for (int fmf : fams)
@@ -181,8 +184,8 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       const int top = max_kf(nd, fmf, R, shape, prec);
       if (top == 0) continue;
       // register kernels (measured at 10240^2 / 8192^2, 64 steps): 2-D radius 1 depth 8
-      // (1662-1745 GStencil/s; depth 12 1617-1902), star2d3r depth 4 (773; 3: 627, 6: 499),
-      // box2d3r depth 2 (330; 3: 315, the dilated tables 300); 1-D the deepest compiled
+      // (1917-1940 GStencil/s; depth 12 1890-1908), star2d3r depth 4 (970-987; 3: 642, 6:
+      // 607); 1-D the deepest compiled
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
                : fmf == 6       ? top
                : fmf == 7       ? std::min(top, R == 1 ? 8 : shape == cs::SHAPE_STAR ? 4 : 2)
