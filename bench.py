@@ -303,7 +303,7 @@ def extras_all(args):
     box2d3r's forms are autotuned (as the headline); the rest run the library's defaults."""
     ex = {}
     g = not args.no_graph
-    u, _ = measure("box2d1r", "fp64", args.seed, reps=3, warmup=1, graph=g, autotune=0, fusion=1, check=False)
+    u, _ = measure("box2d1r", "fp64", args.seed, reps=5, warmup=2, graph=g, autotune=0, fusion=1, check=False)
     ex["box2d1r_single_step"] = {k: u[k] for k in ("kernel", "gstencil_per_s", "ms_per_launch", "hbm_gbs_alg",
                                                    "hbm_frac")}
     for name, precs, tune in (("box2d3r", ("fp64", "bf16x3", "fp16w3", "fp16x2", "fp32x2"), 1),
@@ -313,7 +313,7 @@ def extras_all(args):
                               ("1d1r", ("fp64",), 0)):
         entry = {}
         for prec in precs:
-            r, _ = measure(name, prec, args.seed, reps=3, warmup=1, graph=g,
+            r, _ = measure(name, prec, args.seed, reps=5, warmup=2, graph=g,
                            autotune=tune if prec != "fp32x2" else 0)
             entry[prec] = {k: r[k] for k in ("gstencil_per_s", "kernel", "fusion_depth", "ms_per_launch", "hbm_frac",
                                              "rel_max_err", "rel_l2_err", "within_bar")}
@@ -329,8 +329,8 @@ def extras_all(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="box2d1r", choices=sorted(WORKLOADS) + ["batch"])
     ap.add_argument("--batch-grids", type=int, default=256)
     ap.add_argument("--fusion", type=int, default=0, help="fusion depth (0 = library default)")

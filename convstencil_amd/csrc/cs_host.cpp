@@ -483,15 +483,21 @@ namespace {
 // Times every candidate variant for at least kTuneSteps steps, a whole number of its
 // fused launches (a remainder launch would come from another variant), with the graph
 // as the plan asks, on the plan's own buffers and seeded synthetic data, and keeps the
-// fastest; the buffers are zeroed afterwards (the caller uploads or fills next).
-constexpr int64_t kTuneSteps = 24;
+// fastest -- unless it beats the library's default choice by less than kTuneMargin, a
+// difference that timings on one device repeat within (box2d1r 10240^2: depth 8 and 12
+// within 2%, picked either way run to run); the buffers are zeroed afterwards (the caller
+// uploads or fills next).
+constexpr int64_t kTuneSteps = 96;
+constexpr double kTuneMargin = 0.03;
 
 cs_status autotune_plan(cs_plan *p, int classified) {
   const std::vector<const Variant *> cand = tune_candidates(p, classified);
   if (cand.empty()) return CS_OK;
   const cs_options user = p->opt;
-  int best = 0;
-  double best_ms = 1e300;
+  int best = 0, dflt = 0;
+  double best_ms = 1e300, dflt_ms = 1e300;
+  p->shape = classified;
+  if (select_variants(p) == CS_OK) dflt = p->vmain->id;
   for (const Variant *v : cand) {
     destroy_graphs(p);
     p->opt = user;
@@ -508,6 +514,11 @@ cs_status autotune_plan(cs_plan *p, int classified) {
       best_ms = ms;
       best = v->id;
     }
+    if (v->id == dflt) dflt_ms = ms;
+  }
+  if (dflt != 0 && best_ms > dflt_ms * (1.0 - kTuneMargin)) {
+    best = dflt;
+    best_ms = dflt_ms;
   }
   destroy_graphs(p);
   p->opt = user;
