@@ -356,23 +356,20 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
 
 // ids 700000 + (ring of 2 (2R + 1) rows) * 50000 + (star) * 10000 + (R > 1) * R * 1000 + PX * 100
 // + KF.  Measured (GStencil/s over 64-step runs; boxes differ by up to 10%): box2d1r 10240^2
-// p2 kf8 ring 6 (3 waves per SIMD) 1662-1745, p2 kf12 ring 6 (2 waves) 1617-1902, p2 kf12
-// ring 3 (3 waves) 1827, p4 kf8 ring 3 (2 waves) 1352-1428; star2d1r p2 kf12 ring 3 2428;
-// star2d3r 8192^2 p2 kf4 ring 7 773 (kf3 627, kf6 499, kf2 428); box2d3r p2 kf2 330 (kf3
-// 315).  Fewer columns per lane cost more recomputed columns and lane shifts per cell but
-// leave registers for more waves and a deeper load ring, which the FP64 FMA issue needs:
-// one wave issues an FP64 FMA about every 12 cycles, two waves about every 6.
+// p2 kf8 ring 6 (3 waves per SIMD) 1868-1940, p4 kf8 ring 3 (2 waves) 1916-1988, p2 kf12
+// ring 6 (2 waves) 1849-1908, p2 kf10 ring 6 1840, p2 kf8 ring 3 1812-1830, p3 kf8 1715, p2
+// kf6 ring 3 (4 waves) 1546; star2d3r 8192^2 p2 kf4 ring 7 970-987 (kf3 642, kf6 607);
+// box2d3r p2 kf3 195, kf2 184 (these spill; the MFMA tables run it at 304-308).  Fewer
+// columns per lane cost more recomputed columns and lane shifts per cell but leave
+// registers for more waves and a deeper load ring, which the FP64 FMA issue needs: one wave
+// issues an FP64 FMA about every 12 cycles, two waves about every 6.
 static const Variant kVariantsReg2D[] = {
     CS_VR2D(750212, 1, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, 1, SHAPE_BOX, 2, 12, 3, 3),
     CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),   CS_VR2D(750216, 1, SHAPE_BOX, 2, 16, 6, 2),
-    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700407, 1, SHAPE_BOX, 4, 7, 3, 2),
-    CS_VR2D(700406, 1, SHAPE_BOX, 4, 6, 3, 2),   CS_VR2D(750406, 1, SHAPE_BOX, 4, 6, 6, 2),
-    CS_VR2D(700308, 1, SHAPE_BOX, 3, 8, 3, 2),   CS_VR2D(700306, 1, SHAPE_BOX, 3, 6, 3, 2),
-    CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),   CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),
-    CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
+    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),
+    CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
     CS_VR2D(710212, 1, SHAPE_STAR, 2, 12, 3, 3), CS_VR2D(760212, 1, SHAPE_STAR, 2, 12, 6, 2),
     CS_VR2D(760216, 1, SHAPE_STAR, 2, 16, 6, 2), CS_VR2D(760208, 1, SHAPE_STAR, 2, 8, 6, 3),
-    CS_VR2D(710406, 1, SHAPE_STAR, 4, 6, 3, 2),  CS_VR2D(710308, 1, SHAPE_STAR, 3, 8, 3, 2),
     CS_VR2D(710404, 1, SHAPE_STAR, 4, 4, 3, 2),  CS_VR2D(710402, 1, SHAPE_STAR, 4, 2, 3, 2),
     CS_VR2D(710401, 1, SHAPE_STAR, 4, 1, 3, 2),
     // radius 3 (star2d3r, box2d3r): rings of 7 rows
