@@ -179,7 +179,8 @@ void reg_items(const cs_batch *b, int g, const cs::Variant *v, std::vector<cs::R
   const int ox = v->bx, h = v->R * v->kf;
   const int64_t strips = (nx + ox - 1) / ox;
   const int64_t slots = strips * std::max<int64_t>(1, (ny + 63) / 64);
-  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, h, slots, cs::kRegEdgeCost, cs::reg_least(v->R, v->kf, v->st));
+  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, h, slots, cs::reg_edge_cost(v->px, v->R),
+                                         cs::reg_least(v->R, v->kf, v->st));
   const int64_t waves = cs::reg_waves(sp);
   for (int64_t w = 0; w < waves; ++w) {
     int64_t strip = 0, y0 = 0, y1 = 0;

@@ -156,7 +156,9 @@ int32_t cs_debug_ctable(int32_t radius, int32_t shape, int32_t *c0, int32_t *ty,
 int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t radius, int32_t kf, int32_t ring,
                              int64_t slots, int64_t *split) {
   if (ny < 1 || nx < 1 || ox < 1 || radius < 1 || kf < 1 || ring < 2 || slots < 1 || !split) return -1;
-  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, radius * kf, slots, cs::kRegEdgeCost, cs::reg_least(radius, kf, ring));
+  const int px = (ox + 2 * radius * kf) / 64;  // strips of 64 px columns
+  const cs::RegSplit sp = cs::reg_split(ny, nx, ox, radius * kf, slots, cs::reg_edge_cost(px, radius),
+                                        cs::reg_least(radius, kf, ring));
   const int64_t v[5] = {sp.strips, sp.he, sp.hm, sp.we, sp.wi};
   memcpy(split, v, sizeof(v));
   return 0;

@@ -175,10 +175,14 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
     // (8192^2, 64 steps: 304-308 GStencil/s at depth 2-3 against 184-195).
     const bool reg2d = !(nd == 2 && R > 1 && shape == cs::SHAPE_BOX);
     const int fams[3] = {nd == 3 ? 8 : nd == 1 ? 6 : reg2d ? 7 : mf, nd == 3 ? 3 : mf, mf};
-    // a requested depth that some family compiles exactly is honoured exactly
+    // a requested depth that some family compiles exactly is honoured exactly; single
+    // steps of the register families run on the MFMA single-step kernel, as a plan's
+    // remainders do (see single_step)
     if (fusion_depth > 0)
-      for (int fmf : fams)
+      for (int fmf : fams) {
+        if (fusion_depth == 1 && (fmf == 6 || fmf == 7)) continue;
         if (!v) v = find_variant(nd, fmf, R, shape, fusion_depth, 0, prec);
+      }
     for (int fmf : fams) {
       if (v) break;
       const int top = max_kf(nd, fmf, R, shape, prec);

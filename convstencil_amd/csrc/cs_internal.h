@@ -153,8 +153,11 @@ __host__ __device__ inline bool reg_chunk(const RegSplit &sp, int64_t ny, int64_
 }
 
 // Time of a wave on the checked path (edge strips, first and last chunk of a strip)
-// relative to the plain path, per row (measured: 2.3-2.6 balances the two at 10240^2).
-constexpr double kRegEdgeCost = 2.5;
+// relative to the plain path, per row, for strips of 64 px columns.  Measured at 10240^2
+// (box2d1r depth 8, GStencil/s): 2 columns per lane (whose checked path spills at 3 waves
+// per SIMD) 1920-1928 at 2.5, 1868 at 1.75, 1752 at 1.25; 4 columns per lane 1948-1950 at
+// 2.5, 1979-1985 at 1.75, 1888 at 1.25; star2d3r (2 columns) 968 at 2.5, 980 at 1.75.
+inline double reg_edge_cost(int px, int R) { return px >= 4 || R > 1 ? 1.75 : 2.5; }
 
 // Iterations per unrolled block of a radius-R sweep with a load ring of `ring` rows (the
 // 2R + 1 partial-sum slots and the ring both repeat), and the rows a plain-path chunk needs

@@ -328,7 +328,7 @@ static hipError_t launch_reg2d(const double *in, double *out, const double *wts,
   using C = Reg2D<R, PX, KF>;
   using S = RegSlots<R, SH, PX, KF, RING, NW, WPE>;
   const int64_t slots = S::value > 0 ? S::value : 2048;
-  const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, C::H, slots, kRegEdgeCost, reg_least(R, KF, RING));
+  const RegSplit sp = reg_split(g.n[1], g.n[2], C::OX, C::H, slots, reg_edge_cost(PX, R), reg_least(R, KF, RING));
   const int64_t waves = reg_waves(sp);
   hipLaunchKernelGGL((k2d_reg<R, SH, PX, KF, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
                      s, in, out, wts, g, sp, RegMulti{});
@@ -364,9 +364,9 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
 // registers for more waves and a deeper load ring, which the FP64 FMA issue needs: one wave
 // issues an FP64 FMA about every 12 cycles, two waves about every 6.
 static const Variant kVariantsReg2D[] = {
+    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),
     CS_VR2D(750212, 1, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, 1, SHAPE_BOX, 2, 12, 3, 3),
-    CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),   CS_VR2D(750216, 1, SHAPE_BOX, 2, 16, 6, 2),
-    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),
+    CS_VR2D(750216, 1, SHAPE_BOX, 2, 16, 6, 2),  CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),
     CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
     CS_VR2D(710212, 1, SHAPE_STAR, 2, 12, 3, 3), CS_VR2D(760212, 1, SHAPE_STAR, 2, 12, 6, 2),
     CS_VR2D(760216, 1, SHAPE_STAR, 2, 16, 6, 2), CS_VR2D(760208, 1, SHAPE_STAR, 2, 8, 6, 3),
