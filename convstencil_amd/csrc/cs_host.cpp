@@ -188,8 +188,8 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       const int top = max_kf(nd, fmf, R, shape, prec);
       if (top == 0) continue;
       // register kernels (measured at 10240^2 / 8192^2, 64 steps): 2-D radius 1 depth 8
-      // (1917-1940 GStencil/s; depth 12 1890-1908), star2d3r depth 4 (970-987; 3: 642, 6:
-      // 607); 1-D the deepest compiled
+      // (1980-2001 GStencil/s; depth 12 1890-1908), star2d3r depth 4 (1016-1018; 3: 797,
+      // 6: 761); 1-D the deepest compiled
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
                : fmf == 6       ? top
                : fmf == 7       ? std::min(top, R == 1 ? 8 : shape == cs::SHAPE_STAR ? 4 : 2)

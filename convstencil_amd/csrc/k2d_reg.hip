@@ -50,6 +50,10 @@ struct Sweep2D {
   // order, which measured faster for them (box2d3r 8192^2, depth 2: 348 GStencil/s against
   // 102-182 with the buffer stores, clamped loads and ordered prologue of the others)
   static constexpr bool SPILLS = R > 1 && BOX;
+  // The radius-3 star runs its first period ahead of the loop (see run); measured at 8192^2
+  // 1018 GStencil/s against 982.  For radius 1 the scheduling barriers this needs cost
+  // registers: 4 columns per lane 1831 against 1983 (31 spilled registers), 2 columns equal.
+  static constexpr bool PEEL = R > 1 && !BOX;
 
   const double *__restrict__ in;
   double *__restrict__ out;
@@ -203,6 +207,9 @@ struct Sweep2D {
     // q: level KF, row r - R KF
     const int64_t yo = r - C::H;
     store_row(yo, yo >= y0 && yo < y1, q);
+    // each phase's memory operations stay in their phase: the scheduler would otherwise
+    // sink the first period's loads below its stores (they are used only in the loop)
+    if constexpr (PEEL) __builtin_amdgcn_sched_barrier(0);
   }
 
   template <int K>
@@ -222,22 +229,31 @@ struct Sweep2D {
       for (int k = 0; k < NS; ++k)
 #pragma unroll
         for (int j = 0; j < PX; ++j) acc[s][k][j] = 0.0;
-    // The loop head's wait is one instruction for the first entry and every later one, so
-    // the prologue issues the loop's own pattern: each row's load, in order (no reordering
-    // across the scheduling barrier), then one row of dropped stores.  The wait then counts
-    // as many operations after the row it needs as in the steady state (with the rows
-    // issued youngest-first it drained the whole ring once per period).
+    // The partial-sum slots repeat every NS iterations, the ring every RING.  The loop head's
+    // wait is one instruction for the first entry and every later one.  After the prologue
+    // fewer operations follow each ring row than in the steady state (no stores), so that
+    // wait is sized for the first entry and drains more of the ring than the steady state
+    // needs.  With PEEL the first period runs before the loop (every sweep has at least one:
+    // iters >= 1) and the loop is entered in the steady state.  Otherwise the prologue's rows
+    // are at least issued in order (a scheduling barrier after each; issued youngest-first,
+    // the head's wait drained the whole ring once per period), each followed by a row of
+    // dropped stores, the loop's own pattern (the compiler merges those and moves them ahead
+    // of the loads; the form is kept as measured: 1979-1985 GStencil/s for box2d1r).
 #pragma unroll
     for (int k = 0; k < RING - 1; ++k) {
       load_row(ys + k, ring[k]);
-      if constexpr (!SPILLS) {
+      if constexpr (!SPILLS && !PEEL) {
         const double none[PX] = {};
         store_row(ys, false, none);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    // the partial-sum slots repeat every NS iterations, the ring every RING
-    for (int64_t it = 0; it < iters; it += PER) phases<0>(it, ys, y0, y1);
+    int64_t it0 = 0;
+    if constexpr (PEEL) {
+      phases<0>(0, ys, y0, y1);
+      it0 = PER;
+    }
+    for (int64_t it = it0; it < iters; it += PER) phases<0>(it, ys, y0, y1);
   }
 };
 
@@ -358,7 +374,7 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
 // + KF.  Measured (GStencil/s over 64-step runs; boxes differ by up to 10%): box2d1r 10240^2
 // p2 kf8 ring 6 (3 waves per SIMD) 1868-1940, p4 kf8 ring 3 (2 waves) 1916-1988, p2 kf12
 // ring 6 (2 waves) 1849-1908, p2 kf10 ring 6 1840, p2 kf8 ring 3 1812-1830, p3 kf8 1715, p2
-// kf6 ring 3 (4 waves) 1546; star2d3r 8192^2 p2 kf4 ring 7 970-987 (kf3 642, kf6 607);
+// kf6 ring 3 (4 waves) 1546; star2d3r 8192^2 p2 kf4 ring 7 1016-1018 (kf3 797, kf6 761);
 // box2d3r p2 kf3 195, kf2 184 (these spill; the MFMA tables run it at 304-308).  Fewer
 // columns per lane cost more recomputed columns and lane shifts per cell but leave
 // registers for more waves and a deeper load ring, which the FP64 FMA issue needs: one wave
