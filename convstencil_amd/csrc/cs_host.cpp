@@ -152,6 +152,10 @@ int max_kf(int ndim, int mf, int R, int shape, int prec = CS_PREC_FP64, int st =
 // 8192^2 286 against 253; box2d3r keeps its own (246 against 204).
 const Variant *single_step(int nd, int R, int shape, const Variant *v) {
   if (v->kf == 1) return v;
+  // the fused 3-D register kernels: the 8 x 8 block single-step kernel (512^3: 258-268
+  // GStencil/s against 242 for the register single step)
+  if (v->mf == 9 && nd == 3)
+    if (const Variant *m = find_variant(nd, 8, R, shape, 1, 0, CS_PREC_FP64)) return m;
   if ((v->mf == 6 || v->mf == 7) && !(nd == 2 && R > 1 && shape == cs::SHAPE_BOX))
     if (const Variant *m = find_variant(nd, 4, R, shape, 1, 0, CS_PREC_FP64)) return m;
   return find_variant(nd, v->mf, R, shape, 1, 0, CS_PREC_FP64);
@@ -169,18 +173,19 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
   } else if (prec == CS_PREC_FP64) {
     // 1-D: the register-resident kernels (mf 6) at their deepest compiled fusion, else the
     // 4 x 4 x 4 blocks; 2-D: the register-streaming kernels (mf 7), else the dilated 8 x 8
-    // blocks; 3-D: the 8 x 8 block family (mf 8) where it exists,
-    // else the register-blocked FMA kernels (mf 3, radius 2-3), else the 4 x 16 strips.
-    // The 2-D box radius-3 register kernels spill; the dilated tables are faster there
-    // (8192^2, 64 steps: 304-308 GStencil/s at depth 2-3 against 184-195).
+    // blocks; 3-D: the fused register-streaming kernels (mf 9, radius 1), else the 8 x 8
+    // block family (mf 8) where it exists, else the register-blocked FMA kernels (mf 3,
+    // radius 2-3), else the 4 x 16 strips.  The 2-D box radius-3 register kernels spill; the
+    // dilated tables are faster there (8192^2, 64 steps: 304-308 GStencil/s at depth 2-3
+    // against 184-195).
     const bool reg2d = !(nd == 2 && R > 1 && shape == cs::SHAPE_BOX);
-    const int fams[3] = {nd == 3 ? 8 : nd == 1 ? 6 : reg2d ? 7 : mf, nd == 3 ? 3 : mf, mf};
+    const int fams[4] = {nd == 3 ? 9 : nd == 1 ? 6 : reg2d ? 7 : mf, nd == 3 ? 8 : mf, nd == 3 ? 3 : mf, mf};
     // a requested depth that some family compiles exactly is honoured exactly; single
     // steps of the register families run on the MFMA single-step kernel, as a plan's
     // remainders do (see single_step)
     if (fusion_depth > 0)
       for (int fmf : fams) {
-        if (fusion_depth == 1 && (fmf == 6 || fmf == 7)) continue;
+        if (fusion_depth == 1 && (fmf == 6 || fmf == 7 || fmf == 9)) continue;
         if (!v) v = find_variant(nd, fmf, R, shape, fusion_depth, 0, prec);
       }
     for (int fmf : fams) {
@@ -190,9 +195,13 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       // register kernels (measured at 10240^2 / 8192^2, 64 steps): 2-D radius 1 depth 8
       // (1980-2001 GStencil/s; depth 12 1890-1908), star2d3r depth 4 (1016-1018; 3: 797,
       // 6: 761); 1-D the deepest compiled
+      // the 3-D register kernels fuse two steps (their single steps are slower than the 8 x 8
+      // blocks', which a plan takes for its single steps instead)
+      if (fmf == 9 && top < 2) continue;
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
                : fmf == 6       ? top
                : fmf == 7       ? std::min(top, R == 1 ? 8 : shape == cs::SHAPE_STAR ? 4 : 2)
+               : fmf == 9       ? 2
                                 : default_kf(nd, R, top, shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
       if (v) break;
@@ -255,8 +264,9 @@ cs_status select_variants(cs_plan *p) {
     return CS_OK;
   }
   // the FMA kernels (the ladder's, the 3-D mf 3 family and the register-streaming mf 7 and
-  // mf 9 families) read the dense weights (zero-padded to whole 64-entry rows); an mf 7
-  // plan whose single steps run on the MFMA kernel appends that kernel's B fragments
+  // mf 9 families) read the dense weights (zero-padded to whole 64-entry rows); a plan of
+  // those whose single steps run on an MFMA kernel (mf 7: the 2-D 4 x 4 x 4 kernel, mf 9:
+  // the 3-D 8 x 8 blocks) appends that kernel's B fragments
   p->frag_off = 0;
   if (v->mf == 1 || v->mf == 2 || v->mf == 3 || v->mf == 7 || v->mf == 9) {
     std::vector<double> h((p->w.size() + 63) / 64 * 64, 0.0);
@@ -265,6 +275,10 @@ cs_status select_variants(cs_plan *p) {
       p->frag_off = (int64_t)h.size();
       h.resize(h.size() + (size_t)cs::bfrag_ksteps_2d_f64(4, p->R, p->shape) * 64);
       cs::build_bfrag_2d_f64(4, p->R, p->shape, p->w.data(), h.data() + p->frag_off);
+    } else if (v->mf == 9 && p->vone->mf == 8) {
+      p->frag_off = (int64_t)h.size();
+      h.resize(h.size() + (size_t)cs::bfrag_ksteps_3d2_f64(p->R, p->shape) * 64);
+      cs::build_bfrag_3d2_f64(p->R, p->shape, p->w.data(), h.data() + p->frag_off);
     }
     if (p->bfrag) (void)hipFree(p->bfrag);
     p->bfrag = nullptr;
@@ -295,7 +309,7 @@ cs_status select_variants(cs_plan *p) {
 
 cs_status launch_one(cs_plan *p, const Variant *v) {
   const Geo g = make_geo(p->L, v);
-  const double *wts = v->mf == 4 && p->frag_off > 0 ? p->bfrag + p->frag_off : p->bfrag;
+  const double *wts = (v->mf == 4 || v->mf == 8) && p->frag_off > 0 ? p->bfrag + p->frag_off : p->bfrag;
   CS_HIP(v->launch(p->buf[p->cur], p->buf[p->cur ^ 1], wts, g, p->stream));
   p->cur ^= 1;
   p->launches += 1;

@@ -241,6 +241,264 @@ __global__ __launch_bounds__(NW * 64, WPE) void k3d_reg(const double *__restrict
     sweep3<SH, PX, BY, RING, true>(in, out, wts, g, x0, y0, z0, z1, lane);
 }
 
+// ---- two fused steps (zero halo: the plan fuses only when both buffers carry a zero halo)
+//
+// A wave owns a strip of 64 PX columns whose outer two columns on either side are
+// recomputed by the neighbouring strips (64 PX - 4 outputs), and BY output rows.  Input
+// planes stream in z: rows y0 - 2 .. y0 + BY + 1 of plane z add their taps to three partial
+// planes of level 1 (rows y0 - 1 .. y0 + BY, as the single-step sweep does); the level-1
+// plane z - 1 then complete is masked to zero outside the interior (the halo of the
+// intermediate step) and adds its taps to three partial planes of level 2 (rows y0 ..
+// y0 + BY - 1), whose plane z - 2 is then complete and stored.  Column neighbours come
+// from the lanes l -+ 1; lane 0 and lane 63 read zero there, which only spoils the outer
+// columns' values.
+template <int PX, int BY>
+struct Fuse3D {
+  static constexpr int OX = 64 * PX - 4;  // output columns per wave
+  static constexpr int NR = BY + 4;       // input rows per plane
+  static constexpr int N1 = BY + 2;       // level-1 rows
+};
+
+template <int SH, int PX, int BY, int RING, bool EDGE>
+struct Sweep3F {
+  using C = Fuse3D<PX, BY>;
+  static constexpr int NR = C::NR, N1 = C::N1;
+  static constexpr bool BOX = SH == SHAPE3_BOX;
+  static constexpr int PER = RING % 3 == 0 ? RING : 3 * RING;
+
+  const double *__restrict__ in;
+  double *__restrict__ out;
+  double w[27];
+  int64_t nz, ny, nx, ez, ey, ex, oz, oy, ox;
+  int64_t y0, xs, cx;  // first output row; the strip's first column; this lane's first column
+  int lane;
+  int dcol[PX];  // checked path: column j + dcol[j] of the lane's run is column j clamped
+                 // into the array
+  bool colin[PX], colout[PX];
+  double ring[RING][NR][PX];
+  double a1[3][N1][PX];  // level-1 partial planes
+  double a2[3][BY][PX];  // level-2 partial planes
+
+  // input plane z, rows y0 - 2 .. y0 + BY + 1 (the checked path clamps plane, row and column
+  // into the array: a cell outside it is at least two cells from the interior, so it
+  // reaches only level-1 cells outside the interior, which are masked)
+  __device__ __forceinline__ void load_plane(int64_t z, double (&q)[NR][PX]) {
+    const int64_t az = oz + z;
+    if constexpr (!EDGE) {
+      const double *p = in + ((az * ey) + oy + y0 - 2) * ex + ox + cx;
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int j = 0; j < PX; ++j) q[r][j] = p[r * ex + j];
+    } else {
+      const int64_t cz = az < 0 ? 0 : az >= ez ? ez - 1 : az;  // wave-uniform
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int64_t ay = oy + y0 - 2 + r;
+        const int64_t cy = ay < 0 ? 0 : ay >= ey ? ey - 1 : ay;
+        const double *p = in + (cz * ey + cy) * ex + ox + cx;
+#pragma unroll
+        for (int j = 0; j < PX; ++j) q[r][j] = p[j + dcol[j]];
+      }
+    }
+  }
+
+  // the taps of one plane offset dz from rows src (extended: src[r][1 + c], c = -1 .. PX) to
+  // NO rows of the partial plane T (row i takes source rows i .. i + 2); start: the first
+  // contribution to T
+  template <int NO, int NI>
+  __device__ __forceinline__ void taps(const double (&src)[NI][PX + 2], double (&T)[NO][PX], int dz, bool start) {
+#pragma unroll
+    for (int i = 0; i < NO; ++i)
+#pragma unroll
+      for (int j = 0; j < PX; ++j) {
+        double t = start ? 0.0 : T[i][j];
+        bool first = start;
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+          for (int dx = -1; dx <= 1; ++dx) {
+            if (!BOX && (dz != 0) + (dy != 0) + (dx != 0) > 1) continue;
+            const double wv = w[((dz + 1) * 3 + dy + 1) * 3 + dx + 1], v = src[i + 1 + dy][1 + j + dx];
+            t = first ? wv * v : __builtin_fma(wv, v, t);
+            first = false;
+          }
+        T[i][j] = t;
+      }
+  }
+
+  template <int NO>
+  __device__ __forceinline__ void pin(double (&T)[NO][PX]) {
+#pragma unroll
+    for (int i = 0; i < NO; ++i)
+#pragma unroll
+      for (int j = 0; j < PX; ++j) asm volatile("" : "+v"(T[i][j]));
+  }
+
+  template <int NI>
+  __device__ __forceinline__ void extend(const double (&q)[NI][PX], double (&qe)[NI][PX + 2]) {
+#pragma unroll
+    for (int r = 0; r < NI; ++r) {
+#pragma unroll
+      for (int j = 0; j < PX; ++j) qe[r][1 + j] = q[r][j];
+      qe[r][0] = lane_from_lower(q[r][PX - 1]);
+      qe[r][PX + 1] = lane_from_upper(q[r][0]);
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void phase(int64_t it, int64_t zs, int64_t z0, int64_t z1) {
+    const int64_t z = zs + it + K;
+    load_plane(z + RING - 1, ring[(K + RING - 1) % RING]);
+    // level 1: input plane z; level-1 plane z - dz sits in slot (K - dz) mod 3
+    {
+      double qe[NR][PX + 2];
+      extend(ring[K % RING], qe);
+#pragma unroll
+      for (int dz = 1; dz >= -1; --dz) {
+        double(&T)[N1][PX] = a1[((K - dz) % 3 + 3) % 3];
+        taps<N1, NR>(qe, T, dz, dz == -1);
+        if (dz < 1) pin(T);
+      }
+    }
+    // level-1 plane p = z - 1 is complete: outside the interior it is the halo (zero)
+    double(&P)[N1][PX] = a1[((K - 1) % 3 + 3) % 3];
+    if constexpr (EDGE) {
+      const int64_t p = z - 1;
+      const bool zin = p >= 0 && p < nz;  // wave-uniform
+#pragma unroll
+      for (int i = 0; i < N1; ++i) {
+        const bool rin = zin && y0 - 1 + i >= 0 && y0 - 1 + i < ny;  // wave-uniform
+#pragma unroll
+        for (int j = 0; j < PX; ++j) P[i][j] = rin && colin[j] ? P[i][j] : 0.0;
+      }
+    }
+    // level 2: level-1 plane z - 1; level-2 plane z - 1 - dz sits in slot (K - 1 - dz) mod 3
+    {
+      double pe[N1][PX + 2];
+      extend(P, pe);
+#pragma unroll
+      for (int dz = 1; dz >= -1; --dz) {
+        double(&T)[BY][PX] = a2[((K - 1 - dz) % 3 + 3) % 3];
+        taps<BY, N1>(pe, T, dz, dz == -1);
+        if (dz < 1) pin(T);
+      }
+    }
+    // level-2 plane z - 2 is complete
+    const int64_t zo = z - 2;
+    store_plane(zo, zo >= z0 && zo < z1, a2[((K - 2) % 3 + 3) % 3]);
+  }
+
+  // Output plane zo's rows by buffer stores over each row of the strip: a plane outside
+  // [z0, z1) or a row outside the interior gets an empty range and a cell outside the
+  // outputs an offset past the range (no branch in the loop).
+  __device__ __forceinline__ void store_plane(int64_t zo, bool ok, const double (&A)[BY][PX]) {
+    const int base = lane * PX * 8;
+#pragma unroll
+    for (int i = 0; i < BY; ++i) {
+      const bool rok = ok && (!EDGE || y0 + i < ny);
+      const __amdgpu_buffer_rsrc_t rs =
+          buffer_over(out + ((oz + zo) * ey + oy + y0 + i) * ex + ox + xs, rok ? 64 * PX * 8 : 0);
+      if constexpr (!EDGE && PX % 2 == 0) {
+        // the plain path: the outputs start and end at even lane-run offsets (2 and
+        // 64 PX - 2), so both cells of a pair are outputs or neither (on the checked path the
+        // grid's last column can split a pair)
+#pragma unroll
+        for (int j = 0; j < PX; j += 2)
+          buffer_store2(rs, colout[j] ? base + j * 8 : kBufferDrop, A[i][j], A[i][j + 1]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < PX; ++j) buffer_store1(rs, colout[j] ? base + j * 8 : kBufferDrop, A[i][j]);
+      }
+    }
+  }
+
+  template <int K>
+  __device__ __forceinline__ void phases(int64_t it, int64_t zs, int64_t z0, int64_t z1) {
+    if constexpr (K < PER) {
+      phase<K>(it, zs, z0, z1);
+      phases<K + 1>(it, zs, z0, z1);
+    }
+  }
+
+  __device__ __forceinline__ void run(int64_t z0, int64_t z1) {
+    const int64_t zs = z0 - 2;
+    const int64_t iters = (z1 - z0) + 4;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+      for (int i = 0; i < N1; ++i)
+#pragma unroll
+        for (int j = 0; j < PX; ++j) a1[k][i][j] = 0.0;
+#pragma unroll
+      for (int i = 0; i < BY; ++i)
+#pragma unroll
+        for (int j = 0; j < PX; ++j) a2[k][i][j] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < RING - 1; ++k) {
+      load_plane(zs + k, ring[k]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    for (int64_t it = 0; it < iters; it += PER) phases<0>(it, zs, z0, z1);
+  }
+};
+
+template <int SH, int PX, int BY, int RING, bool EDGE>
+__device__ __forceinline__ void sweep3f(const double *__restrict__ in, double *__restrict__ out,
+                                        const double *__restrict__ wts, const Geo &g, int64_t xs, int64_t y0,
+                                        int64_t z0, int64_t z1, int lane) {
+  Sweep3F<SH, PX, BY, RING, EDGE> S;
+  S.in = in;
+  S.out = out;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) S.w[i] = wts[i];
+  S.nz = g.n[0];
+  S.ny = g.n[1];
+  S.nx = g.n[2];
+  S.ez = g.ext[0];
+  S.ey = g.ext[1];
+  S.ex = g.ext[2];
+  S.oz = g.org[0];
+  S.oy = g.org[1];
+  S.ox = g.org[2];
+  S.y0 = y0;
+  S.xs = xs;
+  S.lane = lane;
+  S.cx = xs + (int64_t)lane * PX;
+#pragma unroll
+  for (int j = 0; j < PX; ++j) {
+    const int64_t c = S.cx + j, d = (int64_t)lane * PX + j;
+    S.dcol[j] = (int)(S.ox + c < 0 ? -(S.ox + c) : S.ox + c >= S.ex ? S.ex - 1 - (S.ox + c) : 0);
+    S.colin[j] = c >= 0 && c < S.nx;
+    S.colout[j] = d >= 2 && d < 2 + Fuse3D<PX, BY>::OX && c < S.nx;  // c >= 0 wherever d >= 2
+  }
+  S.run(z0, z1);
+}
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+__global__ __launch_bounds__(NW * 64, WPE) void k3f_reg(const double *__restrict__ in, double *__restrict__ out,
+                                                        const double *__restrict__ wts, Geo g, Reg3Split sp) {
+  using C = Fuse3D<PX, BY>;
+  constexpr int PER = RING % 3 == 0 ? RING : 3 * RING;
+  const int lane = threadIdx.x & 63;
+  const int64_t wid = (int64_t)blockIdx.x * NW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t yt = wid % sp.ytiles, rest = wid / sp.ytiles;
+  const int64_t strip = rest % sp.strips, zch = rest / sp.strips;
+  if (zch >= sp.zchunks) return;
+  const int64_t z0 = zch * sp.zc;
+  if (z0 >= g.n[0]) return;
+  const int64_t z1 = z0 + sp.zc < g.n[0] ? z0 + sp.zc : g.n[0];
+  const int64_t y0 = yt * BY, xs = strip * C::OX - 2;
+  // the plain path: every cell the sweep loads (planes z0 - 2 .. z1 + period + RING - 2,
+  // rows y0 - 2 .. y0 + BY + 1, the strip's columns) and every level-1 cell is interior
+  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 >= 2 && y0 + BY + 2 <= g.n[1] && z0 >= 2 &&
+      z1 + PER + RING <= g.n[0])
+    sweep3f<SH, PX, BY, RING, false>(in, out, wts, g, xs, y0, z0, z1, lane);
+  else
+    sweep3f<SH, PX, BY, RING, true>(in, out, wts, g, xs, y0, z0, z1, lane);
+}
+
 template <int SH, int PX, int BY, int RING, int NW, int WPE>
 struct Reg3Slots {
   static int64_t value;
@@ -272,6 +530,37 @@ static hipError_t launch_reg3d(const double *in, double *out, const double *wts,
   return hipGetLastError();
 }
 
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+struct Fuse3Slots {
+  static int64_t value;
+};
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+int64_t Fuse3Slots<SH, PX, BY, RING, NW, WPE>::value = 0;
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+static hipError_t prepare_fuse3d() {
+  int nb = 0, dev = 0, cus = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &nb, (const void *)k3f_reg<SH, PX, BY, RING, NW, WPE>, NW * 64, 0);
+  if (e != hipSuccess) return e;
+  if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+  Fuse3Slots<SH, PX, BY, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
+  return hipSuccess;
+}
+
+template <int SH, int PX, int BY, int RING, int NW, int WPE>
+static hipError_t launch_fuse3d(const double *in, double *out, const double *wts, const Geo &g, hipStream_t s) {
+  using C = Fuse3D<PX, BY>;
+  using S = Fuse3Slots<SH, PX, BY, RING, NW, WPE>;
+  const int64_t slots = S::value > 0 ? S::value : 2048;
+  const Reg3Split sp = reg3_split(g.n[0], g.n[1], g.n[2], C::OX, BY, slots);
+  const int64_t waves = sp.strips * sp.ytiles * sp.zchunks;
+  hipLaunchKernelGGL((k3f_reg<SH, PX, BY, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
+                     s, in, out, wts, g, sp);
+  return hipGetLastError();
+}
+
 // Variant::mf 9 marks this family: by = rows per wave, bx = columns per wave, px = columns
 // per lane, st = planes in the load ring; the weights are the dense 3 x 3 x 3 block.
 #define CS_VR3D(ID, SH, PX, BY, RING, WPE)                                                                   \
@@ -281,8 +570,18 @@ static hipError_t launch_reg3d(const double *in, double *out, const double *wts,
         "k3d_reg<r1," #SH ",p" #PX ",by" #BY ",ring" #RING ",wpe" #WPE ">"                                  \
   }
 
-// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING
+#define CS_VR3F(ID, SH, PX, BY, RING, WPE)                                                                   \
+  Variant {                                                                                                  \
+    ID, 9, 1, SH, 2, BY, Fuse3D<PX, BY>::OX, 4, 0, PX, RING, &launch_fuse3d<SH, PX, BY, RING, 4, WPE>,      \
+        &prepare_fuse3d<SH, PX, BY, RING, 4, WPE>,                                                          \
+        "k3f_reg<r1," #SH ",p" #PX ",by" #BY ",kf2,ring" #RING ",wpe" #WPE ">"                              \
+  }
+
+// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING; the fused ones + 100000
 static const Variant kVariantsReg3D[] = {
+    CS_VR3F(1002023, SHAPE3_BOX, 2, 2, 3, 2),  CS_VR3F(1002022, SHAPE3_BOX, 2, 2, 2, 2),
+    CS_VR3F(1012042, SHAPE3_STAR, 2, 4, 2, 2), CS_VR3F(1012022, SHAPE3_STAR, 2, 2, 2, 2),
+    CS_VR3F(1012023, SHAPE3_STAR, 2, 2, 3, 2),
     CS_VR3D(902023, SHAPE3_BOX, 2, 2, 3, 3),  CS_VR3D(904023, SHAPE3_BOX, 4, 2, 3, 2),
     CS_VR3D(902043, SHAPE3_BOX, 2, 4, 3, 2),  CS_VR3D(912023, SHAPE3_STAR, 2, 2, 3, 4),
     CS_VR3D(914023, SHAPE3_STAR, 4, 2, 3, 2), CS_VR3D(912043, SHAPE3_STAR, 2, 4, 3, 2),

@@ -157,7 +157,7 @@ def test_every_variant(gpu, vid, R, shape, vname):
     _check(L, got, oracle.run(L, offs, w, x, 5))
 
 
-def _reg_variants_3d():
+def _reg_variants_3d(kf=1):
     import ctypes
     from convstencil_amd import _lib
     lib = _lib.load()
@@ -170,11 +170,13 @@ def _reg_variants_3d():
         info = (ctypes.c_int64 * 12)()
         name = ctypes.create_string_buffer(128)
         lib.cs_debug_variant_info(9, i, info, name, 128)
-        out.append((int(info[0]), int(info[3]), name.value.decode()))
+        if int(info[4]) == kf:
+            out.append((int(info[0]), int(info[3]), name.value.decode()))
     return out
 
 
 REG_3D = _reg_variants_3d()
+FUSED_3D = _reg_variants_3d(2)
 
 
 # the register-streaming single-step kernels (k3d_reg.hip, Variant::mf 9): plain-path
@@ -190,6 +192,27 @@ def test_every_reg_variant(gpu, vid, shape, vname):
         x = oracle.fill_data(L, vid + steps)
         with Plan(L, 1, dense_weights(3, 1, offs, w), variant=vid) as p:
             assert f"variant={vid}" in p.describe()
+            p.upload(x)
+            p.run(steps)
+            got = p.download()
+        _check(L, got, oracle.run(L, offs, w, x, steps))
+
+
+# the fused register-streaming kernels (two steps per launch; plans fuse with a zero halo,
+# which the seeded fill leaves): plain-path waves in the large grid, ragged strips and
+# tiles, halos of odd widths, even and odd step counts (a single-step remainder)
+@pytest.mark.parametrize("vid,shape,vname", FUSED_3D, ids=[v[2] for v in FUSED_3D])
+def test_every_fused_reg_variant(gpu, vid, shape, vname):
+    name = "box3d1r" if shape == 0 else "star3d1r"
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), vid)
+    for L, steps in [(Layout.with_halo([60, 30, 600], 1), 4), (Layout.with_halo([37, 45, 140], 1), 5),
+                     (Layout(3, [29, 14, 300], [2, 1, 3], [25, 11, 290]), 2), (Layout.with_halo([3, 1, 5], 1), 2),
+                     (Layout.with_halo([1, 2, 130], 2), 3), (Layout.with_halo([7, 9, 11], 1), 5)]:
+        print(f"case {vname} {L.n} T={steps}", flush=True)
+        x = oracle.fill_data(L, vid + steps)
+        with Plan(L, 1, dense_weights(3, 1, offs, w), variant=vid) as p:
+            assert f"variant={vid}" in p.describe() and "halo_uniform=1" in p.describe()
             p.upload(x)
             p.run(steps)
             got = p.download()

@@ -1901,7 +1901,7 @@ def emulate_3d_reg(v, layout, dense, x, slots):
     return out.reshape(-1)
 
 
-VARIANTS_REG3D = variants(9)
+VARIANTS_REG3D = [v for v in variants(9) if v["kf"] == 1]
 
 
 @pytest.mark.parametrize("v", VARIANTS_REG3D, ids=[v["name"] for v in VARIANTS_REG3D])
@@ -1940,3 +1940,146 @@ def test_reg3d_split_covers_once(nz, ny, nx, ox, by, slots):
     assert strips * ox >= nx > (strips - 1) * ox and ytiles * by >= ny > (ytiles - 1) * by
     assert zchunks * zc >= nz > (zchunks - 1) * zc
     assert zc >= min(8, nz) and strips * ytiles * zchunks <= max(slots, strips * ytiles)
+
+
+def emulate_3f_reg(v, layout, dense, x, slots):
+    """k3f_reg (two fused steps, zero halo): wave w takes y-tile w mod y-tiles, then strip,
+    then z-chunk; it owns BY output rows and a strip of 64 PX columns starting two columns
+    left of its outputs (lane-major flat row; the lane shifts bring zero past the wave's
+    ends), streams input planes z0 - 2 .., loaded RING - 1 iterations ahead, rows y0 - 2 ..
+    y0 + BY + 1; plane z adds its taps to level-1 planes z - 1, z, z + 1 (rows y0 - 1 ..
+    y0 + BY), level-1 plane z - 1 is masked to zero outside the interior on the checked path
+    and adds its taps to level-2 planes z - 2, z - 1, z, whose plane z - 2 is stored on the
+    output columns.  Checks loads and stores against the array, counts the stores per cell,
+    and that the plain path's level-1 cells are interior."""
+    PX, BY, RING = v["px"], v["by"], v["st"]
+    OX = 64 * PX - 4
+    NR, N1 = BY + 4, BY + 2
+    PER = RING if RING % 3 == 0 else 3 * RING
+    box = v["shape"] == 0
+    assert v["mf"] == 9 and v["R"] == 1 and v["kf"] == 2 and v["bx"] == OX and v["lds"] == 0
+    ez, ey, ex = layout.ext
+    oz, oy, ox = layout.org
+    nz, ny, nx = layout.n
+    X = x.reshape(ez, ey, ex)
+    w = np.asarray(dense, dtype=np.float64).reshape(3, 3, 3)
+    out = np.zeros((ez, ey, ex))
+    hits = np.zeros((ez, ey, ex), dtype=np.int64)
+    strips, ytiles, zchunks, zc = reg3d_split(nz, ny, nx, OX, BY, slots)
+    plain = 0
+    W = 64 * PX
+    d = np.arange(W)
+
+    def shifted(q):  # q[rows][W] -> extended rows [rows][W + 2] with zero past the wave's ends
+        return np.concatenate([np.zeros((q.shape[0], 1)), q, np.zeros((q.shape[0], 1))], axis=1)
+
+    def add_taps(src, T, dz, start, nout):
+        t = None if start else T
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if not box and (dz != 0) + (dy != 0) + (dx != 0) > 1:
+                    continue
+                term = w[dz + 1, dy + 1, dx + 1] * src[1 + dy:1 + dy + nout, 1 + dx:1 + dx + W]
+                t = term if t is None else t + term
+        return t
+
+    for wid in range(strips * ytiles * zchunks):
+        yt, rest = wid % ytiles, wid // ytiles
+        strip, zch = rest % strips, rest // strips
+        z0 = zch * zc
+        if z0 >= nz:
+            continue
+        z1 = min(z0 + zc, nz)
+        y0, xs = yt * BY, strip * OX - 2
+        fast = xs >= 0 and xs + W <= nx and y0 >= 2 and y0 + BY + 2 <= ny and z0 >= 2 and z1 + PER + RING <= nz
+        plain += fast
+        cols = xs + d
+        rows = y0 - 2 + np.arange(NR)
+        colin = (cols >= 0) & (cols < nx)
+        colout = (d >= 2) & (d < 2 + OX) & (cols < nx)
+
+        def load(z):
+            if fast:
+                assert 0 <= z < nz and rows[0] >= 0 and rows[-1] < ny and cols[0] >= 0 and cols[-1] < nx
+                return X[oz + z][np.ix_(oy + rows, ox + cols)].copy()
+            cz = min(max(oz + z, 0), ez - 1)
+            return X[cz][np.ix_(np.clip(oy + rows, 0, ey - 1), np.clip(ox + cols, 0, ex - 1))]
+
+        zs, iters = z0 - 2, (z1 - z0) + 4
+        a1 = [np.zeros((N1, W)) for _ in range(3)]
+        a2 = [np.zeros((BY, W)) for _ in range(3)]
+        ring = [load(zs + k) for k in range(RING - 1)] + [None]
+        for it in range(0, iters, PER):
+            for K in range(PER):
+                z = zs + it + K
+                ring[(K + RING - 1) % RING] = load(z + RING - 1)
+                qe = shifted(ring[K % RING])
+                for dz in (1, 0, -1):
+                    sl = (K - dz) % 3
+                    a1[sl] = add_taps(qe, a1[sl], dz, dz == -1, N1)
+                P = a1[(K - 1) % 3]
+                p = z - 1
+                if fast:
+                    if z0 - 1 <= p <= z1:  # the level-1 planes the stored outputs use
+                        r1 = y0 - 1 + np.arange(N1)
+                        assert 0 <= p < nz and r1[0] >= 0 and r1[-1] < ny and colin.all()
+                else:
+                    rin = (y0 - 1 + np.arange(N1) >= 0) & (y0 - 1 + np.arange(N1) < ny) & (0 <= p < nz)
+                    P = np.where(rin[:, None] & colin[None, :], P, 0.0)
+                    a1[(K - 1) % 3] = P
+                pe = shifted(P)
+                for dz in (1, 0, -1):
+                    sl = (K - 1 - dz) % 3
+                    a2[sl] = add_taps(pe, a2[sl], dz, dz == -1, BY)
+                zo = z - 2
+                if z0 <= zo < z1:
+                    A = a2[(K - 2) % 3]
+                    # the plain path stores pairs of a lane's cells, decided by the pair's first
+                    # cell; the checked path stores cell by cell
+                    st = colout.copy()
+                    if fast and PX % 2 == 0:
+                        st = np.repeat(colout[0::2], 2)
+                    for i in range(BY):
+                        if y0 + i >= ny:
+                            continue
+                        c = ox + cols[st]
+                        assert 0 <= oz + zo < ez and 0 <= oy + y0 + i < ey and (c >= 0).all() and (c < ex).all()
+                        out[oz + zo, oy + y0 + i, c] = A[i, st]
+                        hits[oz + zo, oy + y0 + i, c] += 1
+    inner = hits[oz:oz + nz, oy:oy + ny, ox:ox + nx]
+    assert (inner == 1).all(), (int(inner.min()), int(inner.max()))
+    assert hits.sum() == inner.sum()
+    emulate_3f_reg.plain_waves = plain
+    return out.reshape(-1)
+
+
+VARIANTS_REG3F = [v for v in variants(9) if v["kf"] == 2]
+
+
+@pytest.mark.parametrize("v", VARIANTS_REG3F, ids=[v["name"] for v in VARIANTS_REG3F])
+def test_variant_emulation_3f_reg(v):
+    # zero halo (the plans fuse only then): a plain-path wave in the middle of the large
+    # grid, halos of odd widths, ragged strips and tiles, and tiny grids
+    OX, BY = v["bx"], v["by"]
+    rng = np.random.default_rng(v["id"])
+    w = rng.uniform(0.05, 1.0, (3, 3, 3))
+    zz, yy, xx = np.meshgrid(*[np.arange(-1, 2)] * 3, indexing="ij")
+    if v["shape"] == 1:
+        w[((zz != 0).astype(int) + (yy != 0) + (xx != 0)) > 1] = 0.0
+    w /= w.sum()
+    dense = w.reshape(-1)
+    ny = 3 * BY + 4
+    three_chunks = 3 * (ny + BY - 1) // BY * 3  # strips x y-tiles x 3 z-chunks
+    cases = [(Layout.with_halo([40, ny, 3 * OX - 5], 1), three_chunks, True),
+             (Layout(3, [29, ny + 3, 2 * OX + 9], [2, 1, 3], [25, ny, 2 * OX + 3]), 4, False),
+             (Layout.with_halo([3, 1, 5], 1), 64, False), (Layout.with_halo([1, 2, 130], 2), 8, False)]
+    for layout, slots, want_plain in cases:
+        x = np.where(layout.interior_mask(), oracle.fill_data(layout, v["id"] + slots), 0.0)
+        got = emulate_3f_reg(v, layout, dense, x, slots)
+        if want_plain:
+            assert emulate_3f_reg.plain_waves > 0
+        offs, ww = oracle.dense_to_taps(3, 1, dense)
+        want = oracle.run(layout, offs, ww, x, 2)
+        gi, wi = layout.interior(got), layout.interior(want)
+        err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
+        assert err < 1e-13, (v["name"], slots, err)
