@@ -570,26 +570,29 @@ static hipError_t launch_fuse3d(const double *in, double *out, const double *wts
         "k3d_reg<r1," #SH ",p" #PX ",by" #BY ",ring" #RING ",wpe" #WPE ">"                                  \
   }
 
-#define CS_VR3F(ID, SH, PX, BY, RING, WPE)                                                                   \
+#define CS_VR3F(ID, SH, PX, BY, RING, WPE, NW)                                                               \
   Variant {                                                                                                  \
-    ID, 9, 1, SH, 2, BY, Fuse3D<PX, BY>::OX, 4, 0, PX, RING, &launch_fuse3d<SH, PX, BY, RING, 4, WPE>,      \
-        &prepare_fuse3d<SH, PX, BY, RING, 4, WPE>,                                                          \
-        "k3f_reg<r1," #SH ",p" #PX ",by" #BY ",kf2,ring" #RING ",wpe" #WPE ">"                              \
+    ID, 9, 1, SH, 2, BY, Fuse3D<PX, BY>::OX, NW, 0, PX, RING, &launch_fuse3d<SH, PX, BY, RING, NW, WPE>,    \
+        &prepare_fuse3d<SH, PX, BY, RING, NW, WPE>,                                                         \
+        "k3f_reg<r1," #SH ",p" #PX ",by" #BY ",kf2,ring" #RING ",wpe" #WPE ",w" #NW ">"                     \
   }
 
-// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING; the fused ones 1000000 + (star)
-// * 100000 + PX * 10000 + waves per SIMD * 1000 + BY * 10 + RING
+// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING; the fused ones 1000000 + (8-wave
+// workgroups) * 800000 + (star) * 100000 + PX * 10000 + waves per SIMD * 1000 + BY * 10 + RING
 // Measured at 512^3, T = 32 (GStencil/s; the first of each shape is the default): box p1
 // by8 ring 3 449-451 (ring 2 447, by10 437, by9 427, by6 373, p2 by4 383-386, p2 by3 349, p2
-// by2 276-294); star p1 by8 ring 3 448-472 (by10 449-453, by12 435, by6 371, p2 by4
+// by2 276-294), with 8-wave workgroups 440 against 428 on one box; star p1 by8 ring 3
+// 448-472 (8-wave workgroups 445 against 448, by10 449-453, by12 435, by6 371, p2 by4
 // 389-414, p2 by2 327-405).  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
 // load fewer halo rows ((BY + 4) / BY) per output row; one column per lane leaves the
 // registers for them.
 static const Variant kVariantsReg3D[] = {
-    CS_VR3F(1012083, SHAPE3_BOX, 1, 8, 3, 2),   CS_VR3F(1012082, SHAPE3_BOX, 1, 8, 2, 2),
-    CS_VR3F(1022043, SHAPE3_BOX, 2, 4, 3, 2),   CS_VR3F(1022023, SHAPE3_BOX, 2, 2, 3, 2),
-    CS_VR3F(1112083, SHAPE3_STAR, 1, 8, 3, 2),  CS_VR3F(1112102, SHAPE3_STAR, 1, 10, 2, 2),
-    CS_VR3F(1122043, SHAPE3_STAR, 2, 4, 3, 2),  CS_VR3F(1122022, SHAPE3_STAR, 2, 2, 2, 2),
+    CS_VR3F(1812083, SHAPE3_BOX, 1, 8, 3, 2, 8),   CS_VR3F(1012083, SHAPE3_BOX, 1, 8, 3, 2, 4),
+    CS_VR3F(1012082, SHAPE3_BOX, 1, 8, 2, 2, 4),
+    CS_VR3F(1022043, SHAPE3_BOX, 2, 4, 3, 2, 4),   CS_VR3F(1022023, SHAPE3_BOX, 2, 2, 3, 2, 4),
+    CS_VR3F(1112083, SHAPE3_STAR, 1, 8, 3, 2, 4),  CS_VR3F(1112102, SHAPE3_STAR, 1, 10, 2, 2, 4),
+    CS_VR3F(1122043, SHAPE3_STAR, 2, 4, 3, 2, 4),  CS_VR3F(1122022, SHAPE3_STAR, 2, 2, 2, 2, 4),
+    CS_VR3F(1912083, SHAPE3_STAR, 1, 8, 3, 2, 8),
     CS_VR3D(902023, SHAPE3_BOX, 2, 2, 3, 3),  CS_VR3D(904023, SHAPE3_BOX, 4, 2, 3, 2),
     CS_VR3D(902043, SHAPE3_BOX, 2, 4, 3, 2),  CS_VR3D(912023, SHAPE3_STAR, 2, 2, 3, 4),
     CS_VR3D(914023, SHAPE3_STAR, 4, 2, 3, 2), CS_VR3D(912043, SHAPE3_STAR, 2, 4, 3, 2),
