@@ -51,9 +51,11 @@ struct Sweep2D {
   // 102-182 with the buffer stores, clamped loads and ordered prologue of the others)
   static constexpr bool SPILLS = R > 1 && BOX;
   // The radius-3 star runs its first period ahead of the loop (see run); measured at 8192^2
-  // 1018 GStencil/s against 982.  For radius 1 the scheduling barriers this needs cost
-  // registers: 4 columns per lane 1831 against 1983 (31 spilled registers), 2 columns equal.
-  static constexpr bool PEEL = R > 1 && !BOX;
+  // 1018 GStencil/s against 982.  For radius 1 with 4 columns per lane only the plain path
+  // does (box2d1r 10240^2: 2021-2038 against 1974-1977); on both paths the scheduling
+  // barriers this needs cost registers (31 spilled, 1831), and 2 columns per lane gain
+  // nothing.
+  static constexpr bool PEEL = (R > 1 && !BOX) || (!EDGE && R == 1 && PX == 4);
 
   const double *__restrict__ in;
   double *__restrict__ out;
@@ -372,7 +374,7 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
 
 // ids 700000 + (ring of 2 (2R + 1) rows) * 50000 + (star) * 10000 + (R > 1) * R * 1000 + PX * 100
 // + KF.  Measured (GStencil/s over 64-step runs; boxes differ by up to 10%): box2d1r 10240^2
-// p2 kf8 ring 6 (3 waves per SIMD) 1868-1940, p4 kf8 ring 3 (2 waves) 1916-1988, p2 kf12
+// p2 kf8 ring 6 (3 waves per SIMD) 1868-1940, p4 kf8 ring 3 (2 waves) 1950-2038, p2 kf12
 // ring 6 (2 waves) 1849-1908, p2 kf10 ring 6 1840, p2 kf8 ring 3 1812-1830, p3 kf8 1715, p2
 // kf6 ring 3 (4 waves) 1546; star2d3r 8192^2 p2 kf4 ring 7 1016-1018 (kf3 797, kf6 761);
 // box2d3r p2 kf3 195, kf2 184 (these spill; the MFMA tables run it at 304-308).  Fewer
