@@ -22,8 +22,10 @@ extern "C" {
  * buffers' bytes); 7 = the register-resident 1-D FP64 kernels (info[1] = 6, info[6] = outputs
  * per workgroup, info[9] = cells per lane); 8 = the register-streaming 2-D FP64 kernels
  * (info[1] = 7, info[6] = output columns per wave, info[9] = columns per lane, info[10] =
- * rows in the load ring); 9 = the register-streaming 3-D FP64 single-step kernels (info[1] = 9,
- * info[5] = rows per wave, info[6] = columns per wave, info[9] = columns per lane, info[10] =
+ * rows in the load ring); 9 = the register-streaming 3-D FP64 kernels, single-step (info[4] = 1)
+ * and two fused steps (info[4] = 2; strips of 64 x columns-per-lane columns of which the outer
+ * two on either side are recomputed) (info[1] = 9, info[5] = rows per wave, info[6] = output
+ * columns per wave, info[7] = waves per workgroup, info[9] = columns per lane, info[10] =
  * planes in the load ring). */
 int32_t cs_debug_variant_count(int32_t ndim);
 /* info[0..11] = id, mfma shape (16 or 4), radius, shape class (0 box, 1 star), fusion depth,
@@ -95,7 +97,7 @@ int32_t cs_debug_reg2d_split(int64_t ny, int64_t nx, int32_t ox, int32_t radius,
 int32_t cs_debug_reg2d_chunk(const int64_t *split, int64_t ny, int64_t wid, int64_t *rows);
 
 /* Decomposition of one launch of the register-streaming 3-D kernels (family 9) for nz x ny x
- * nx interior cells, `ox` columns and `by` rows per wave and `slots` resident waves:
+ * nx interior cells, `ox` output columns and `by` rows per wave and `slots` resident waves:
  * split[0..3] = strips, y-tiles, z-chunks, planes per z-chunk; wave w takes y-tile w mod
  * y-tiles, then strip, then z-chunk.  Returns 0, or -1 for invalid arguments. */
 int32_t cs_debug_reg3d_split(int64_t nz, int64_t ny, int64_t nx, int32_t ox, int32_t by, int64_t slots,
