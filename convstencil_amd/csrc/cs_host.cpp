@@ -195,13 +195,14 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       // register kernels (measured at 10240^2 / 8192^2, 64 steps): 2-D radius 1 depth 8
       // (1980-2001 GStencil/s; depth 12 1890-1908), star2d3r depth 4 (1016-1018; 3: 797,
       // 6: 761); 1-D the deepest compiled
-      // the 3-D register kernels fuse two steps (their single steps are slower than the 8 x 8
-      // blocks', which a plan takes for its single steps instead)
+      // the 3-D register kernels fuse two steps for the box and three for the star (512^3:
+      // box 433-451 GStencil/s at depth 2 and 3 alike, star 584 at depth 3 against 449-472);
+      // their single steps are slower than the 8 x 8 blocks', which a plan takes instead
       if (fmf == 9 && top < 2) continue;
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
                : fmf == 6       ? top
                : fmf == 7       ? std::min(top, R == 1 ? 8 : shape == cs::SHAPE_STAR ? 4 : 2)
-               : fmf == 9       ? 2
+               : fmf == 9       ? std::min(top, shape == cs::SHAPE3_STAR ? 3 : 2)
                                 : default_kf(nd, R, top, shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
       if (v) break;

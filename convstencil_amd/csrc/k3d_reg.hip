@@ -241,28 +241,30 @@ __global__ __launch_bounds__(NW * 64, WPE) void k3d_reg(const double *__restrict
     sweep3<SH, PX, BY, RING, true>(in, out, wts, g, x0, y0, z0, z1, lane);
 }
 
-// ---- two fused steps (zero halo: the plan fuses only when both buffers carry a zero halo)
+// ---- KF fused steps (KF = 2, 3; zero halo: the plan fuses only when both buffers carry a
+// zero halo)
 //
-// A wave owns a strip of 64 PX columns whose outer two columns on either side are
-// recomputed by the neighbouring strips (64 PX - 4 outputs), and BY output rows.  Input
-// planes stream in z: rows y0 - 2 .. y0 + BY + 1 of plane z add their taps to three partial
-// planes of level 1 (rows y0 - 1 .. y0 + BY, as the single-step sweep does); the level-1
-// plane z - 1 then complete is masked to zero outside the interior (the halo of the
-// intermediate step) and adds its taps to three partial planes of level 2 (rows y0 ..
-// y0 + BY - 1), whose plane z - 2 is then complete and stored.  Column neighbours come
-// from the lanes l -+ 1; lane 0 and lane 63 read zero there, which only spoils the outer
-// columns' values.
-template <int PX, int BY>
+// A wave owns a strip of 64 PX columns whose outer KF columns on either side are recomputed
+// by the neighbouring strips (64 PX - 2 KF outputs), and BY output rows.  Input planes
+// stream in z: rows y0 - KF .. y0 + BY + KF - 1 of plane z add their taps to three partial
+// planes of level 1 (rows y0 - KF + 1 .. y0 + BY + KF - 2, as the single-step sweep does);
+// the level-1 plane z - 1 then complete is masked to zero outside the interior (the halo
+// of the intermediate step) and adds its taps to three partial planes of level 2 (two rows
+// fewer), and so on: level l's plane z - l is complete after plane z arrives, and level
+// KF's is stored.  Column neighbours come from the lanes l -+ 1; lane 0 and lane 63 read
+// zero there, which only spoils the outer columns' values.
+template <int PX, int BY, int KF>
 struct Fuse3D {
-  static constexpr int OX = 64 * PX - 4;  // output columns per wave
-  static constexpr int NR = BY + 4;       // input rows per plane
-  static constexpr int N1 = BY + 2;       // level-1 rows
+  static constexpr int OX = 64 * PX - 2 * KF;  // output columns per wave
+  static constexpr int NR = BY + 2 * KF;       // input rows per plane
+  // rows of level l (1 .. KF): BY + 2 (KF - l)
+  static constexpr int rows(int l) { return BY + 2 * (KF - l); }
 };
 
-template <int SH, int PX, int BY, int RING, bool EDGE>
+template <int SH, int PX, int BY, int KF, int RING, bool EDGE>
 struct Sweep3F {
-  using C = Fuse3D<PX, BY>;
-  static constexpr int NR = C::NR, N1 = C::N1;
+  using C = Fuse3D<PX, BY, KF>;
+  static constexpr int NR = C::NR, NA = C::rows(1);
   static constexpr bool BOX = SH == SHAPE3_BOX;
   static constexpr int PER = RING % 3 == 0 ? RING : 3 * RING;
 
@@ -276,16 +278,16 @@ struct Sweep3F {
                  // into the array
   bool colin[PX], colout[PX];
   double ring[RING][NR][PX];
-  double a1[3][N1][PX];  // level-1 partial planes
-  double a2[3][BY][PX];  // level-2 partial planes
+  // partial planes of level l + 1: a[l][slot][row], rows 0 .. rows(l + 1) - 1 in use
+  double a[KF][3][NA][PX];
 
-  // input plane z, rows y0 - 2 .. y0 + BY + 1 (the checked path clamps plane, row and column
-  // into the array: a cell outside it is at least two cells from the interior, so it
+  // input plane z, rows y0 - KF .. y0 + BY + KF - 1 (the checked path clamps plane, row and
+  // column into the array: a cell outside it is at least two cells from the interior, so it
   // reaches only level-1 cells outside the interior, which are masked)
   __device__ __forceinline__ void load_plane(int64_t z, double (&q)[NR][PX]) {
     const int64_t az = oz + z;
     if constexpr (!EDGE) {
-      const double *p = in + ((az * ey) + oy + y0 - 2) * ex + ox + cx;
+      const double *p = in + ((az * ey) + oy + y0 - KF) * ex + ox + cx;
 #pragma unroll
       for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -294,7 +296,7 @@ struct Sweep3F {
       const int64_t cz = az < 0 ? 0 : az >= ez ? ez - 1 : az;  // wave-uniform
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
-        const int64_t ay = oy + y0 - 2 + r;
+        const int64_t ay = oy + y0 - KF + r;
         const int64_t cy = ay < 0 ? 0 : ay >= ey ? ey - 1 : ay;
         const double *p = in + (cz * ey + cy) * ex + ox + cx;
 #pragma unroll
@@ -306,8 +308,8 @@ struct Sweep3F {
   // the taps of one plane offset dz from rows src (extended: src[r][1 + c], c = -1 .. PX) to
   // NO rows of the partial plane T (row i takes source rows i .. i + 2); start: the first
   // contribution to T
-  template <int NO, int NI>
-  __device__ __forceinline__ void taps(const double (&src)[NI][PX + 2], double (&T)[NO][PX], int dz, bool start) {
+  template <int NO, int NI, int NT>
+  __device__ __forceinline__ void taps(const double (&src)[NI][PX + 2], double (&T)[NT][PX], int dz, bool start) {
 #pragma unroll
     for (int i = 0; i < NO; ++i)
 #pragma unroll
@@ -327,16 +329,16 @@ struct Sweep3F {
       }
   }
 
-  template <int NO>
-  __device__ __forceinline__ void pin(double (&T)[NO][PX]) {
+  template <int NO, int NT>
+  __device__ __forceinline__ void pin(double (&T)[NT][PX]) {
 #pragma unroll
     for (int i = 0; i < NO; ++i)
 #pragma unroll
       for (int j = 0; j < PX; ++j) asm volatile("" : "+v"(T[i][j]));
   }
 
-  template <int NI>
-  __device__ __forceinline__ void extend(const double (&q)[NI][PX], double (&qe)[NI][PX + 2]) {
+  template <int NI, int NS>
+  __device__ __forceinline__ void extend(const double (&q)[NS][PX], double (&qe)[NI][PX + 2]) {
 #pragma unroll
     for (int r = 0; r < NI; ++r) {
 #pragma unroll
@@ -346,63 +348,65 @@ struct Sweep3F {
     }
   }
 
+  // Level L (1 .. KF) takes its input plane p = z - (L - 1) (rows NI) and adds its taps to
+  // its planes p + 1 (start), p and p - 1 (then complete); level L's plane q sits in slot
+  // (q - zs) mod 3 = (K - (L - 1) - dz) mod 3 for q = p - dz.
+  template <int K, int L, int NI, int NS>
+  __device__ __forceinline__ void level(int64_t z, const double (&src)[NS][PX]) {
+    constexpr int NO = C::rows(L);
+    {
+      double qe[NI][PX + 2];
+      extend<NI, NS>(src, qe);
+#pragma unroll
+      for (int dz = 1; dz >= -1; --dz) {
+        double(&T)[NA][PX] = a[L - 1][((K - (L - 1) - dz) % 3 + 3) % 3];
+        taps<NO, NI, NA>(qe, T, dz, dz == -1);
+        if (dz < 1) pin<NO, NA>(T);
+      }
+    }
+    // level L's plane z - L is complete
+    double(&P)[NA][PX] = a[L - 1][((K - L) % 3 + 3) % 3];
+    if constexpr (L < KF) {
+      // outside the interior it is the halo (zero)
+      if constexpr (EDGE) {
+        const int64_t p = z - L;
+        const bool zin = p >= 0 && p < nz;  // wave-uniform
+#pragma unroll
+        for (int i = 0; i < NO; ++i) {
+          const int64_t y = y0 - (KF - L) + i;
+          const bool rin = zin && y >= 0 && y < ny;  // wave-uniform
+#pragma unroll
+          for (int j = 0; j < PX; ++j) P[i][j] = rin && colin[j] ? P[i][j] : 0.0;
+        }
+      }
+      level<K, L + 1, NO, NA>(z, P);
+    }
+  }
+
   template <int K>
   __device__ __forceinline__ void phase(int64_t it, int64_t zs, int64_t z0, int64_t z1) {
     const int64_t z = zs + it + K;
     load_plane(z + RING - 1, ring[(K + RING - 1) % RING]);
-    // level 1: input plane z; level-1 plane z - dz sits in slot (K - dz) mod 3
-    {
-      double qe[NR][PX + 2];
-      extend(ring[K % RING], qe);
-#pragma unroll
-      for (int dz = 1; dz >= -1; --dz) {
-        double(&T)[N1][PX] = a1[((K - dz) % 3 + 3) % 3];
-        taps<N1, NR>(qe, T, dz, dz == -1);
-        if (dz < 1) pin(T);
-      }
-    }
-    // level-1 plane p = z - 1 is complete: outside the interior it is the halo (zero)
-    double(&P)[N1][PX] = a1[((K - 1) % 3 + 3) % 3];
-    if constexpr (EDGE) {
-      const int64_t p = z - 1;
-      const bool zin = p >= 0 && p < nz;  // wave-uniform
-#pragma unroll
-      for (int i = 0; i < N1; ++i) {
-        const bool rin = zin && y0 - 1 + i >= 0 && y0 - 1 + i < ny;  // wave-uniform
-#pragma unroll
-        for (int j = 0; j < PX; ++j) P[i][j] = rin && colin[j] ? P[i][j] : 0.0;
-      }
-    }
-    // level 2: level-1 plane z - 1; level-2 plane z - 1 - dz sits in slot (K - 1 - dz) mod 3
-    {
-      double pe[N1][PX + 2];
-      extend(P, pe);
-#pragma unroll
-      for (int dz = 1; dz >= -1; --dz) {
-        double(&T)[BY][PX] = a2[((K - 1 - dz) % 3 + 3) % 3];
-        taps<BY, N1>(pe, T, dz, dz == -1);
-        if (dz < 1) pin(T);
-      }
-    }
-    // level-2 plane z - 2 is complete
-    const int64_t zo = z - 2;
-    store_plane(zo, zo >= z0 && zo < z1, a2[((K - 2) % 3 + 3) % 3]);
+    level<K, 1, NR, NR>(z, ring[K % RING]);
+    // level KF's plane z - KF is complete
+    const int64_t zo = z - KF;
+    store_plane(zo, zo >= z0 && zo < z1, a[KF - 1][((K - KF) % 3 + 3) % 3]);
   }
 
   // Output plane zo's rows by buffer stores over each row of the strip: a plane outside
   // [z0, z1) or a row outside the interior gets an empty range and a cell outside the
   // outputs an offset past the range (no branch in the loop).
-  __device__ __forceinline__ void store_plane(int64_t zo, bool ok, const double (&A)[BY][PX]) {
+  __device__ __forceinline__ void store_plane(int64_t zo, bool ok, const double (&A)[NA][PX]) {
     const int base = lane * PX * 8;
 #pragma unroll
     for (int i = 0; i < BY; ++i) {
       const bool rok = ok && (!EDGE || y0 + i < ny);
       const __amdgpu_buffer_rsrc_t rs =
           buffer_over(out + ((oz + zo) * ey + oy + y0 + i) * ex + ox + xs, rok ? 64 * PX * 8 : 0);
-      if constexpr (!EDGE && PX % 2 == 0) {
-        // the plain path: the outputs start and end at even lane-run offsets (2 and
-        // 64 PX - 2), so both cells of a pair are outputs or neither (on the checked path the
-        // grid's last column can split a pair)
+      if constexpr (!EDGE && PX % 2 == 0 && KF % 2 == 0) {
+        // the plain path: the outputs start and end at even lane-run offsets (KF and
+        // 64 PX - KF), so both cells of a pair are outputs or neither (on the checked path
+        // the grid's last column can split a pair)
 #pragma unroll
         for (int j = 0; j < PX; j += 2)
           buffer_store2(rs, colout[j] ? base + j * 8 : kBufferDrop, A[i][j], A[i][j + 1]);
@@ -422,19 +426,16 @@ struct Sweep3F {
   }
 
   __device__ __forceinline__ void run(int64_t z0, int64_t z1) {
-    const int64_t zs = z0 - 2;
-    const int64_t iters = (z1 - z0) + 4;
+    const int64_t zs = z0 - KF;
+    const int64_t iters = (z1 - z0) + 2 * KF;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int l = 0; l < KF; ++l)
 #pragma unroll
-      for (int i = 0; i < N1; ++i)
+      for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int j = 0; j < PX; ++j) a1[k][i][j] = 0.0;
+        for (int i = 0; i < NA; ++i)
 #pragma unroll
-      for (int i = 0; i < BY; ++i)
-#pragma unroll
-        for (int j = 0; j < PX; ++j) a2[k][i][j] = 0.0;
-    }
+          for (int j = 0; j < PX; ++j) a[l][k][i][j] = 0.0;
 #pragma unroll
     for (int k = 0; k < RING - 1; ++k) {
       load_plane(zs + k, ring[k]);
@@ -444,11 +445,11 @@ struct Sweep3F {
   }
 };
 
-template <int SH, int PX, int BY, int RING, bool EDGE>
+template <int SH, int PX, int BY, int KF, int RING, bool EDGE>
 __device__ __forceinline__ void sweep3f(const double *__restrict__ in, double *__restrict__ out,
                                         const double *__restrict__ wts, const Geo &g, int64_t xs, int64_t y0,
                                         int64_t z0, int64_t z1, int lane) {
-  Sweep3F<SH, PX, BY, RING, EDGE> S;
+  Sweep3F<SH, PX, BY, KF, RING, EDGE> S;
   S.in = in;
   S.out = out;
 #pragma unroll
@@ -471,15 +472,15 @@ __device__ __forceinline__ void sweep3f(const double *__restrict__ in, double *_
     const int64_t c = S.cx + j, d = (int64_t)lane * PX + j;
     S.dcol[j] = (int)(S.ox + c < 0 ? -(S.ox + c) : S.ox + c >= S.ex ? S.ex - 1 - (S.ox + c) : 0);
     S.colin[j] = c >= 0 && c < S.nx;
-    S.colout[j] = d >= 2 && d < 2 + Fuse3D<PX, BY>::OX && c < S.nx;  // c >= 0 wherever d >= 2
+    S.colout[j] = d >= KF && d < KF + Fuse3D<PX, BY, KF>::OX && c < S.nx;  // c >= 0 wherever d >= KF
   }
   S.run(z0, z1);
 }
 
-template <int SH, int PX, int BY, int RING, int NW, int WPE>
+template <int SH, int PX, int BY, int KF, int RING, int NW, int WPE>
 __global__ __launch_bounds__(NW * 64, WPE) void k3f_reg(const double *__restrict__ in, double *__restrict__ out,
                                                         const double *__restrict__ wts, Geo g, Reg3Split sp) {
-  using C = Fuse3D<PX, BY>;
+  using C = Fuse3D<PX, BY, KF>;
   constexpr int PER = RING % 3 == 0 ? RING : 3 * RING;
   const int lane = threadIdx.x & 63;
   const int64_t wid = (int64_t)blockIdx.x * NW + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -489,14 +490,15 @@ __global__ __launch_bounds__(NW * 64, WPE) void k3f_reg(const double *__restrict
   const int64_t z0 = zch * sp.zc;
   if (z0 >= g.n[0]) return;
   const int64_t z1 = z0 + sp.zc < g.n[0] ? z0 + sp.zc : g.n[0];
-  const int64_t y0 = yt * BY, xs = strip * C::OX - 2;
-  // the plain path: every cell the sweep loads (planes z0 - 2 .. z1 + period + RING - 2,
-  // rows y0 - 2 .. y0 + BY + 1, the strip's columns) and every level-1 cell is interior
-  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 >= 2 && y0 + BY + 2 <= g.n[1] && z0 >= 2 &&
-      z1 + PER + RING <= g.n[0])
-    sweep3f<SH, PX, BY, RING, false>(in, out, wts, g, xs, y0, z0, z1, lane);
+  const int64_t y0 = yt * BY, xs = strip * C::OX - KF;
+  // the plain path: every cell the sweep loads (planes z0 - KF .. z1 + KF + period + RING -
+  // 3, rows y0 - KF .. y0 + BY + KF - 1, the strip's columns) and every intermediate cell is
+  // interior
+  if (xs >= 0 && xs + 64 * PX <= g.n[2] && y0 >= KF && y0 + BY + KF <= g.n[1] && z0 >= KF &&
+      z1 + KF + PER + RING - 2 <= g.n[0])
+    sweep3f<SH, PX, BY, KF, RING, false>(in, out, wts, g, xs, y0, z0, z1, lane);
   else
-    sweep3f<SH, PX, BY, RING, true>(in, out, wts, g, xs, y0, z0, z1, lane);
+    sweep3f<SH, PX, BY, KF, RING, true>(in, out, wts, g, xs, y0, z0, z1, lane);
 }
 
 template <int SH, int PX, int BY, int RING, int NW, int WPE>
@@ -530,34 +532,34 @@ static hipError_t launch_reg3d(const double *in, double *out, const double *wts,
   return hipGetLastError();
 }
 
-template <int SH, int PX, int BY, int RING, int NW, int WPE>
+template <int SH, int PX, int BY, int KF, int RING, int NW, int WPE>
 struct Fuse3Slots {
   static int64_t value;
 };
-template <int SH, int PX, int BY, int RING, int NW, int WPE>
-int64_t Fuse3Slots<SH, PX, BY, RING, NW, WPE>::value = 0;
+template <int SH, int PX, int BY, int KF, int RING, int NW, int WPE>
+int64_t Fuse3Slots<SH, PX, BY, KF, RING, NW, WPE>::value = 0;
 
-template <int SH, int PX, int BY, int RING, int NW, int WPE>
+template <int SH, int PX, int BY, int KF, int RING, int NW, int WPE>
 static hipError_t prepare_fuse3d() {
   int nb = 0, dev = 0, cus = 0;
   hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &nb, (const void *)k3f_reg<SH, PX, BY, RING, NW, WPE>, NW * 64, 0);
+      &nb, (const void *)k3f_reg<SH, PX, BY, KF, RING, NW, WPE>, NW * 64, 0);
   if (e != hipSuccess) return e;
   if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-  Fuse3Slots<SH, PX, BY, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
+  Fuse3Slots<SH, PX, BY, KF, RING, NW, WPE>::value = (int64_t)(nb > 0 ? nb : 1) * NW * (cus > 0 ? cus : 1);
   return hipSuccess;
 }
 
-template <int SH, int PX, int BY, int RING, int NW, int WPE>
+template <int SH, int PX, int BY, int KF, int RING, int NW, int WPE>
 static hipError_t launch_fuse3d(const double *in, double *out, const double *wts, const Geo &g, hipStream_t s) {
-  using C = Fuse3D<PX, BY>;
-  using S = Fuse3Slots<SH, PX, BY, RING, NW, WPE>;
+  using C = Fuse3D<PX, BY, KF>;
+  using S = Fuse3Slots<SH, PX, BY, KF, RING, NW, WPE>;
   const int64_t slots = S::value > 0 ? S::value : 2048;
   const Reg3Split sp = reg3_split(g.n[0], g.n[1], g.n[2], C::OX, BY, slots);
   const int64_t waves = sp.strips * sp.ytiles * sp.zchunks;
-  hipLaunchKernelGGL((k3f_reg<SH, PX, BY, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)), dim3(NW * 64), 0,
-                     s, in, out, wts, g, sp);
+  hipLaunchKernelGGL((k3f_reg<SH, PX, BY, KF, RING, NW, WPE>), dim3((unsigned)((waves + NW - 1) / NW)),
+                     dim3(NW * 64), 0, s, in, out, wts, g, sp);
   return hipGetLastError();
 }
 
@@ -570,29 +572,36 @@ static hipError_t launch_fuse3d(const double *in, double *out, const double *wts
         "k3d_reg<r1," #SH ",p" #PX ",by" #BY ",ring" #RING ",wpe" #WPE ">"                                  \
   }
 
-#define CS_VR3F(ID, SH, PX, BY, RING, WPE, NW)                                                               \
+#define CS_VR3F(ID, SH, PX, BY, RING, WPE, NW, KF)                                                           \
   Variant {                                                                                                  \
-    ID, 9, 1, SH, 2, BY, Fuse3D<PX, BY>::OX, NW, 0, PX, RING, &launch_fuse3d<SH, PX, BY, RING, NW, WPE>,    \
-        &prepare_fuse3d<SH, PX, BY, RING, NW, WPE>,                                                         \
-        "k3f_reg<r1," #SH ",p" #PX ",by" #BY ",kf2,ring" #RING ",wpe" #WPE ",w" #NW ">"                     \
+    ID, 9, 1, SH, KF, BY, Fuse3D<PX, BY, KF>::OX, NW, 0, PX, RING,                                           \
+        &launch_fuse3d<SH, PX, BY, KF, RING, NW, WPE>, &prepare_fuse3d<SH, PX, BY, KF, RING, NW, WPE>,      \
+        "k3f_reg<r1," #SH ",p" #PX ",by" #BY ",kf" #KF ",ring" #RING ",wpe" #WPE ",w" #NW ">"               \
   }
 
-// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING; the fused ones 1000000 + (8-wave
-// workgroups) * 800000 + (star) * 100000 + PX * 10000 + waves per SIMD * 1000 + BY * 10 + RING
-// Measured at 512^3, T = 32 (GStencil/s; the first of each shape is the default): box p1
+// ids 900000 + (star) * 10000 + PX * 1000 + BY * 10 + RING; the fused ones 1000000 (two
+// steps) or 3000000 (three) + (8-wave workgroups) * 800000 + (star) * 100000 + PX * 10000 +
+// waves per SIMD * 1000 + BY * 10 + RING
+// Two fused steps, measured at 512^3, T = 32 (GStencil/s; the first of each shape is the
+// box's default): box p1
 // by8 ring 3 449-451 (ring 2 447, by10 437, by9 427, by6 373, p2 by4 383-386, p2 by3 349, p2
 // by2 276-294), with 8-wave workgroups 440 against 428 on one box; star p1 by8 ring 3
 // 448-472 (8-wave workgroups 445 against 448, by10 449-453, by12 435, by6 371, p2 by4
-// 389-414, p2 by2 327-405).  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
+// 389-414, p2 by2 327-405).  Three fused steps: star p1 by8 ring 2 584 (11 spilled
+// registers), by6 546-563, by4 463; box by6 433 (equal to two steps: its 27 taps make it
+// FP64-issue-bound), by4 279.  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
 // load fewer halo rows ((BY + 4) / BY) per output row; one column per lane leaves the
 // registers for them.
 static const Variant kVariantsReg3D[] = {
-    CS_VR3F(1812083, SHAPE3_BOX, 1, 8, 3, 2, 8),   CS_VR3F(1012083, SHAPE3_BOX, 1, 8, 3, 2, 4),
-    CS_VR3F(1012082, SHAPE3_BOX, 1, 8, 2, 2, 4),
-    CS_VR3F(1022043, SHAPE3_BOX, 2, 4, 3, 2, 4),   CS_VR3F(1022023, SHAPE3_BOX, 2, 2, 3, 2, 4),
-    CS_VR3F(1112083, SHAPE3_STAR, 1, 8, 3, 2, 4),  CS_VR3F(1112102, SHAPE3_STAR, 1, 10, 2, 2, 4),
-    CS_VR3F(1122043, SHAPE3_STAR, 2, 4, 3, 2, 4),  CS_VR3F(1122022, SHAPE3_STAR, 2, 2, 2, 2, 4),
-    CS_VR3F(1912083, SHAPE3_STAR, 1, 8, 3, 2, 8),
+    CS_VR3F(1812083, SHAPE3_BOX, 1, 8, 3, 2, 8, 2),   CS_VR3F(1012083, SHAPE3_BOX, 1, 8, 3, 2, 4, 2),
+    CS_VR3F(1012082, SHAPE3_BOX, 1, 8, 2, 2, 4, 2),
+    CS_VR3F(1022043, SHAPE3_BOX, 2, 4, 3, 2, 4, 2),   CS_VR3F(1022023, SHAPE3_BOX, 2, 2, 3, 2, 4, 2),
+    CS_VR3F(1112083, SHAPE3_STAR, 1, 8, 3, 2, 4, 2),  CS_VR3F(1112102, SHAPE3_STAR, 1, 10, 2, 2, 4, 2),
+    CS_VR3F(1122043, SHAPE3_STAR, 2, 4, 3, 2, 4, 2),  CS_VR3F(1122022, SHAPE3_STAR, 2, 2, 2, 2, 4, 2),
+    CS_VR3F(1912083, SHAPE3_STAR, 1, 8, 3, 2, 8, 2),
+    // three fused steps: the star's default (box: equal to two steps)
+    CS_VR3F(3112082, SHAPE3_STAR, 1, 8, 2, 2, 4, 3),  CS_VR3F(3112062, SHAPE3_STAR, 1, 6, 2, 2, 4, 3),
+    CS_VR3F(3012062, SHAPE3_BOX, 1, 6, 2, 2, 4, 3),
     CS_VR3D(902023, SHAPE3_BOX, 2, 2, 3, 3),  CS_VR3D(904023, SHAPE3_BOX, 4, 2, 3, 2),
     CS_VR3D(902043, SHAPE3_BOX, 2, 4, 3, 2),  CS_VR3D(912023, SHAPE3_STAR, 2, 2, 3, 4),
     CS_VR3D(914023, SHAPE3_STAR, 4, 2, 3, 2), CS_VR3D(912043, SHAPE3_STAR, 2, 4, 3, 2),

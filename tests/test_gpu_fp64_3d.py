@@ -157,7 +157,7 @@ def test_every_variant(gpu, vid, R, shape, vname):
     _check(L, got, oracle.run(L, offs, w, x, 5))
 
 
-def _reg_variants_3d(kf=1):
+def _reg_variants_3d(fused=False):
     import ctypes
     from convstencil_amd import _lib
     lib = _lib.load()
@@ -170,13 +170,13 @@ def _reg_variants_3d(kf=1):
         info = (ctypes.c_int64 * 12)()
         name = ctypes.create_string_buffer(128)
         lib.cs_debug_variant_info(9, i, info, name, 128)
-        if int(info[4]) == kf:
+        if (int(info[4]) > 1) == fused:
             out.append((int(info[0]), int(info[3]), name.value.decode()))
     return out
 
 
 REG_3D = _reg_variants_3d()
-FUSED_3D = _reg_variants_3d(2)
+FUSED_3D = _reg_variants_3d(True)
 
 
 # the register-streaming single-step kernels (k3d_reg.hip, Variant::mf 9): plain-path
@@ -198,7 +198,7 @@ def test_every_reg_variant(gpu, vid, shape, vname):
         _check(L, got, oracle.run(L, offs, w, x, steps))
 
 
-# the fused register-streaming kernels (two steps per launch; plans fuse with a zero halo,
+# the fused register-streaming kernels (two or three steps per launch; plans fuse with a zero halo,
 # which the seeded fill leaves): plain-path waves in the large grid, ragged strips and
 # tiles, halos of odd widths, even and odd step counts (a single-step remainder)
 @pytest.mark.parametrize("vid,shape,vname", FUSED_3D, ids=[v[2] for v in FUSED_3D])

@@ -1943,21 +1943,22 @@ def test_reg3d_split_covers_once(nz, ny, nx, ox, by, slots):
 
 
 def emulate_3f_reg(v, layout, dense, x, slots):
-    """k3f_reg (two fused steps, zero halo): wave w takes y-tile w mod y-tiles, then strip,
-    then z-chunk; it owns BY output rows and a strip of 64 PX columns starting two columns
+    """k3f_reg (KF fused steps, zero halo): wave w takes y-tile w mod y-tiles, then strip,
+    then z-chunk; it owns BY output rows and a strip of 64 PX columns starting KF columns
     left of its outputs (lane-major flat row; the lane shifts bring zero past the wave's
-    ends), streams input planes z0 - 2 .., loaded RING - 1 iterations ahead, rows y0 - 2 ..
-    y0 + BY + 1; plane z adds its taps to level-1 planes z - 1, z, z + 1 (rows y0 - 1 ..
-    y0 + BY), level-1 plane z - 1 is masked to zero outside the interior on the checked path
-    and adds its taps to level-2 planes z - 2, z - 1, z, whose plane z - 2 is stored on the
-    output columns.  Checks loads and stores against the array, counts the stores per cell,
-    and that the plain path's level-1 cells are interior."""
-    PX, BY, RING = v["px"], v["by"], v["st"]
-    OX = 64 * PX - 4
-    NR, N1 = BY + 4, BY + 2
+    ends), streams input planes z0 - KF .., loaded RING - 1 iterations ahead, rows y0 - KF
+    .. y0 + BY + KF - 1; level l (rows y0 - KF + l .. y0 + BY + KF - l - 1) takes its input
+    plane p = z - (l - 1) into its planes p + 1, p, p - 1, and its plane z - l, then
+    complete, is masked to zero outside the interior on the checked path and feeds level
+    l + 1; level KF's plane z - KF is stored on the output columns (the plain path in pairs
+    of a lane's cells when PX and KF are even).  Checks loads and stores against the array,
+    counts the stores per cell, and that the plain path's intermediate cells are interior."""
+    PX, BY, RING, KF = v["px"], v["by"], v["st"], v["kf"]
+    OX = 64 * PX - 2 * KF
+    NR = BY + 2 * KF
     PER = RING if RING % 3 == 0 else 3 * RING
     box = v["shape"] == 0
-    assert v["mf"] == 9 and v["R"] == 1 and v["kf"] == 2 and v["bx"] == OX and v["lds"] == 0
+    assert v["mf"] == 9 and v["R"] == 1 and KF in (2, 3) and v["bx"] == OX and v["lds"] == 0
     ez, ey, ex = layout.ext
     oz, oy, ox = layout.org
     nz, ny, nx = layout.n
@@ -1969,6 +1970,9 @@ def emulate_3f_reg(v, layout, dense, x, slots):
     plain = 0
     W = 64 * PX
     d = np.arange(W)
+
+    def rows_of(lv):
+        return BY + 2 * (KF - lv)
 
     def shifted(q):  # q[rows][W] -> extended rows [rows][W + 2] with zero past the wave's ends
         return np.concatenate([np.zeros((q.shape[0], 1)), q, np.zeros((q.shape[0], 1))], axis=1)
@@ -1990,13 +1994,14 @@ def emulate_3f_reg(v, layout, dense, x, slots):
         if z0 >= nz:
             continue
         z1 = min(z0 + zc, nz)
-        y0, xs = yt * BY, strip * OX - 2
-        fast = xs >= 0 and xs + W <= nx and y0 >= 2 and y0 + BY + 2 <= ny and z0 >= 2 and z1 + PER + RING <= nz
+        y0, xs = yt * BY, strip * OX - KF
+        fast = (xs >= 0 and xs + W <= nx and y0 >= KF and y0 + BY + KF <= ny and z0 >= KF
+                and z1 + KF + PER + RING - 2 <= nz)
         plain += fast
         cols = xs + d
-        rows = y0 - 2 + np.arange(NR)
+        rows = y0 - KF + np.arange(NR)
         colin = (cols >= 0) & (cols < nx)
-        colout = (d >= 2) & (d < 2 + OX) & (cols < nx)
+        colout = (d >= KF) & (d < KF + OX) & (cols < nx)
 
         def load(z):
             if fast:
@@ -2005,46 +2010,44 @@ def emulate_3f_reg(v, layout, dense, x, slots):
             cz = min(max(oz + z, 0), ez - 1)
             return X[cz][np.ix_(np.clip(oy + rows, 0, ey - 1), np.clip(ox + cols, 0, ex - 1))]
 
-        zs, iters = z0 - 2, (z1 - z0) + 4
-        a1 = [np.zeros((N1, W)) for _ in range(3)]
-        a2 = [np.zeros((BY, W)) for _ in range(3)]
+        zs, iters = z0 - KF, (z1 - z0) + 2 * KF
+        acc = [[np.zeros((rows_of(lv), W)) for _ in range(3)] for lv in range(1, KF + 1)]
         ring = [load(zs + k) for k in range(RING - 1)] + [None]
         for it in range(0, iters, PER):
             for K in range(PER):
                 z = zs + it + K
                 ring[(K + RING - 1) % RING] = load(z + RING - 1)
-                qe = shifted(ring[K % RING])
-                for dz in (1, 0, -1):
-                    sl = (K - dz) % 3
-                    a1[sl] = add_taps(qe, a1[sl], dz, dz == -1, N1)
-                P = a1[(K - 1) % 3]
-                p = z - 1
-                if fast:
-                    if z0 - 1 <= p <= z1:  # the level-1 planes the stored outputs use
-                        r1 = y0 - 1 + np.arange(N1)
-                        assert 0 <= p < nz and r1[0] >= 0 and r1[-1] < ny and colin.all()
-                else:
-                    rin = (y0 - 1 + np.arange(N1) >= 0) & (y0 - 1 + np.arange(N1) < ny) & (0 <= p < nz)
-                    P = np.where(rin[:, None] & colin[None, :], P, 0.0)
-                    a1[(K - 1) % 3] = P
-                pe = shifted(P)
-                for dz in (1, 0, -1):
-                    sl = (K - 1 - dz) % 3
-                    a2[sl] = add_taps(pe, a2[sl], dz, dz == -1, BY)
-                zo = z - 2
+                src = ring[K % RING]
+                for lv in range(1, KF + 1):
+                    A = acc[lv - 1]
+                    se = shifted(src)
+                    for dz in (1, 0, -1):
+                        sl = (K - (lv - 1) - dz) % 3
+                        A[sl] = add_taps(se, A[sl], dz, dz == -1, rows_of(lv))
+                    P = A[(K - lv) % 3]
+                    if lv < KF:
+                        p = z - lv
+                        ry = y0 - (KF - lv) + np.arange(rows_of(lv))
+                        if fast:
+                            if z0 - (KF - lv) <= p <= z1 + (KF - lv) - 1:  # planes the outputs use
+                                assert 0 <= p < nz and ry[0] >= 0 and ry[-1] < ny and colin.all()
+                        else:
+                            rin = (ry >= 0) & (ry < ny) & (0 <= p < nz)
+                            P = np.where(rin[:, None] & colin[None, :], P, 0.0)
+                            A[(K - lv) % 3] = P
+                    src = P
+                zo = z - KF
                 if z0 <= zo < z1:
-                    A = a2[(K - 2) % 3]
-                    # the plain path stores pairs of a lane's cells, decided by the pair's first
-                    # cell; the checked path stores cell by cell
+                    Aout = acc[KF - 1][(K - KF) % 3]
                     st = colout.copy()
-                    if fast and PX % 2 == 0:
+                    if fast and PX % 2 == 0 and KF % 2 == 0:
                         st = np.repeat(colout[0::2], 2)
                     for i in range(BY):
                         if y0 + i >= ny:
                             continue
                         c = ox + cols[st]
                         assert 0 <= oz + zo < ez and 0 <= oy + y0 + i < ey and (c >= 0).all() and (c < ex).all()
-                        out[oz + zo, oy + y0 + i, c] = A[i, st]
+                        out[oz + zo, oy + y0 + i, c] = Aout[i, st]
                         hits[oz + zo, oy + y0 + i, c] += 1
     inner = hits[oz:oz + nz, oy:oy + ny, ox:ox + nx]
     assert (inner == 1).all(), (int(inner.min()), int(inner.max()))
@@ -2053,7 +2056,7 @@ def emulate_3f_reg(v, layout, dense, x, slots):
     return out.reshape(-1)
 
 
-VARIANTS_REG3F = [v for v in variants(9) if v["kf"] == 2]
+VARIANTS_REG3F = [v for v in variants(9) if v["kf"] > 1]
 
 
 @pytest.mark.parametrize("v", VARIANTS_REG3F, ids=[v["name"] for v in VARIANTS_REG3F])
@@ -2068,9 +2071,10 @@ def test_variant_emulation_3f_reg(v):
         w[((zz != 0).astype(int) + (yy != 0) + (xx != 0)) > 1] = 0.0
     w /= w.sum()
     dense = w.reshape(-1)
-    ny = 3 * BY + 4
-    three_chunks = 3 * (ny + BY - 1) // BY * 3  # strips x y-tiles x 3 z-chunks
-    cases = [(Layout.with_halo([40, ny, 3 * OX - 5], 1), three_chunks, True),
+    KF = v["kf"]
+    ny = 3 * BY + 2 * KF
+    three_chunks = 3 * ((ny + BY - 1) // BY) * 3  # strips x y-tiles x 3 z-chunks
+    cases = [(Layout.with_halo([48, ny, 3 * OX - 5], 1), three_chunks, True),
              (Layout(3, [29, ny + 3, 2 * OX + 9], [2, 1, 3], [25, ny, 2 * OX + 3]), 4, False),
              (Layout.with_halo([3, 1, 5], 1), 64, False), (Layout.with_halo([1, 2, 130], 2), 8, False)]
     for layout, slots, want_plain in cases:
@@ -2079,7 +2083,7 @@ def test_variant_emulation_3f_reg(v):
         if want_plain:
             assert emulate_3f_reg.plain_waves > 0
         offs, ww = oracle.dense_to_taps(3, 1, dense)
-        want = oracle.run(layout, offs, ww, x, 2)
+        want = oracle.run(layout, offs, ww, x, KF)
         gi, wi = layout.interior(got), layout.interior(want)
         err = np.max(np.abs(gi - wi)) / np.max(np.abs(wi))
         assert err < 1e-13, (v["name"], slots, err)
