@@ -196,7 +196,7 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       // (1980-2001 GStencil/s; depth 12 1890-1908), star2d3r depth 4 (1016-1018; 3: 797,
       // 6: 761); 1-D the deepest compiled
       // the 3-D register kernels fuse two steps for the box and three for the star (512^3:
-      // box 433-451 GStencil/s at depth 2 and 3 alike, star 584 at depth 3 against 449-472);
+      // box 433-451 GStencil/s at depth 2 and 3 alike, star 621 at depth 3 against 449-472);
       // their single steps are slower than the 8 x 8 blocks', which a plan takes instead
       if (fmf == 9 && top < 2) continue;
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)

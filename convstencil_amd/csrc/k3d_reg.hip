@@ -587,9 +587,9 @@ static hipError_t launch_fuse3d(const double *in, double *out, const double *wts
 // by8 ring 3 449-451 (ring 2 447, by10 437, by9 427, by6 373, p2 by4 383-386, p2 by3 349, p2
 // by2 276-294), with 8-wave workgroups 440 against 428 on one box; star p1 by8 ring 3
 // 448-472 (8-wave workgroups 445 against 448, by10 449-453, by12 435, by6 371, p2 by4
-// 389-414, p2 by2 327-405).  Three fused steps: star p1 by8 ring 2 584 (11 spilled
-// registers), by6 546-563, by4 463; box by6 433 (equal to two steps: its 27 taps make it
-// FP64-issue-bound), by4 279.  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
+// 389-414, p2 by2 327-405).  Three fused steps: star p1 by7 ring 2 621, by8 584 (11 spilled
+// registers; 8-wave workgroups 585), by6 546-563, by4 463, p2 by4 spills; box by6 433 (equal
+// to two steps: its 27 taps make it FP64-issue-bound), by4 279.  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
 // load fewer halo rows ((BY + 4) / BY) per output row; one column per lane leaves the
 // registers for them.
 static const Variant kVariantsReg3D[] = {
@@ -600,8 +600,8 @@ static const Variant kVariantsReg3D[] = {
     CS_VR3F(1122043, SHAPE3_STAR, 2, 4, 3, 2, 4, 2),  CS_VR3F(1122022, SHAPE3_STAR, 2, 2, 2, 2, 4, 2),
     CS_VR3F(1912083, SHAPE3_STAR, 1, 8, 3, 2, 8, 2),
     // three fused steps: the star's default (box: equal to two steps)
-    CS_VR3F(3112082, SHAPE3_STAR, 1, 8, 2, 2, 4, 3),  CS_VR3F(3112062, SHAPE3_STAR, 1, 6, 2, 2, 4, 3),
-    CS_VR3F(3012062, SHAPE3_BOX, 1, 6, 2, 2, 4, 3),
+    CS_VR3F(3112072, SHAPE3_STAR, 1, 7, 2, 2, 4, 3),  CS_VR3F(3112082, SHAPE3_STAR, 1, 8, 2, 2, 4, 3),
+    CS_VR3F(3112062, SHAPE3_STAR, 1, 6, 2, 2, 4, 3),  CS_VR3F(3012062, SHAPE3_BOX, 1, 6, 2, 2, 4, 3),
     CS_VR3D(902023, SHAPE3_BOX, 2, 2, 3, 3),  CS_VR3D(904023, SHAPE3_BOX, 4, 2, 3, 2),
     CS_VR3D(902043, SHAPE3_BOX, 2, 4, 3, 2),  CS_VR3D(912023, SHAPE3_STAR, 2, 2, 3, 4),
     CS_VR3D(914023, SHAPE3_STAR, 4, 2, 3, 2), CS_VR3D(912043, SHAPE3_STAR, 2, 4, 3, 2),
