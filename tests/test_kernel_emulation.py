@@ -1958,7 +1958,7 @@ def emulate_3f_reg(v, layout, dense, x, slots):
     NR = BY + 2 * KF
     PER = RING if RING % 3 == 0 else 3 * RING
     box = v["shape"] == 0
-    assert v["mf"] == 9 and v["R"] == 1 and KF in (2, 3) and v["bx"] == OX and v["lds"] == 0
+    assert v["mf"] == 9 and v["R"] == 1 and KF >= 2 and v["bx"] == OX and v["lds"] == 0
     ez, ey, ex = layout.ext
     oz, oy, ox = layout.org
     nz, ny, nx = layout.n
