@@ -585,7 +585,8 @@ static hipError_t launch_fuse3d(const double *in, double *out, const double *wts
 // Two fused steps, measured at 512^3, T = 32 (GStencil/s; the first of each shape is the
 // box's default): box p1
 // by8 ring 3 449-451 (ring 2 447, by10 437, by9 427, by6 373, p2 by4 383-386, p2 by3 349, p2
-// by2 276-294), with 8-wave workgroups 440 against 428 on one box; star p1 by8 ring 3
+// by2 276-294, 3 waves per SIMD at by5 / by6 400 / 347), with 8-wave workgroups 440
+// against 428 on one box; star p1 by8 ring 3
 // 448-472 (8-wave workgroups 445 against 448, by10 449-453, by12 435, by6 371, p2 by4
 // 389-414, p2 by2 327-405).  Three fused steps: star p1 by7 ring 2 621, by8 584 (11 spilled
 // registers; 8-wave workgroups 585), by6 546-563, by4 463, p2 by4 spills; box by6 433 (equal
