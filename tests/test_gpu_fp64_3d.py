@@ -219,6 +219,32 @@ def test_every_fused_reg_variant(gpu, vid, shape, vname):
         _check(L, got, oracle.run(L, offs, w, x, steps))
 
 
+@pytest.mark.parametrize("name", ["star3d1r", "box3d1r"])
+def test_nonzero_halo_runs_single_steps(gpu, name):
+    # a non-zero halo: the plan (whose main variant fuses) runs every step on its single-step
+    # kernel, which reads the B fragments the plan appends after the fused kernel's weights;
+    # the halo stays as uploaded in both buffers' reads, as in the reference's ping-pong
+    print(f"case {name} nonzero halo", flush=True)
+    ndim, r, kind = SHAPES[name]
+    L = Layout.with_halo([9, 13, 150], r)
+    offs = shape_offsets(name)
+    w = oracle.fill_weights(len(offs), 91)
+    x = oracle.fill_data(L, 91)
+    rng = np.random.default_rng(91)
+    a = x.reshape(L.ext)
+    m = np.ones(L.ext, dtype=bool)
+    m[1:-1, 1:-1, 1:-1] = False
+    a[m] = rng.uniform(-1.0, 1.0, int(m.sum()))
+    x = a.reshape(-1)
+    with Plan(L, r, dense_weights(3, r, offs, w)) as p:
+        assert "halo_uniform" in p.describe()
+        p.upload(x)
+        assert "halo_uniform=0" in p.describe()
+        p.run(5)
+        got = p.download()
+    _check(L, got, oracle.run(L, offs, w, x, 5))
+
+
 # the register-blocked FMA kernels (k3d_fma.hip, Variant::mf 3): radius 2 and 3, each
 # shape class, partial tiles and z-chunks
 FMA_3D = [(35302, 3, "box"), (35322, 3, "diamond"), (35312, 3, "star"), (35303, 3, "box"), (35323, 3, "diamond"),
