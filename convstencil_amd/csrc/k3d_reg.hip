@@ -591,7 +591,9 @@ static hipError_t launch_fuse3d(const double *in, double *out, const double *wts
 // 389-414, p2 by2 327-405).  Three fused steps: star p1 by7 ring 2 621, by8 584 (11 spilled
 // registers; 8-wave workgroups 585), by6 546-563, by4 463, p2 by4 spills; box by6 433 (equal
 // to two steps: its 27 taps make it FP64-issue-bound), by4 279.  Four fused steps: star by4
-// 498, by5 458 (the recomputed rows outgrow the saved bytes).  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
+// 498, by5 458 (the recomputed rows outgrow the saved bytes).  Also slower: the star's three
+// steps with 8-wave workgroups (543 against 620), the box with a ring of 4 planes (416-418
+// against 432).  Taller tiles recompute fewer level-1 rows ((BY + 2) / BY) and
 // load fewer halo rows ((BY + 4) / BY) per output row; one column per lane leaves the
 // registers for them.
 static const Variant kVariantsReg3D[] = {
