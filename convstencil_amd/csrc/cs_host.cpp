@@ -161,8 +161,14 @@ const Variant *single_step(int nd, int R, int shape, const Variant *v) {
   return find_variant(nd, v->mf, R, shape, 1, 0, CS_PREC_FP64);
 }
 
-// The main variant of a plan: a forced id, else the preferred family's default depth.
-const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, int tile_variant) {
+// 2-D grids of at most this many interior cells take depth 4 from the radius-1 box register
+// kernels: their launches run short of waves and depth 8's longer tail dominates (2048^2:
+// 825 GStencil/s at depth 4 against 635 at depth 8; 3072^2: 1136 against 1183).
+constexpr int64_t kRegSmallGrid = 2560LL * 2560LL;
+
+// The main variant of a plan: a forced id, else the preferred family's default depth (for
+// 2-D, `cells` = the interior cells of one grid, 0 when no single grid decides).
+const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, int tile_variant, int64_t cells = 0) {
   const int mf = prec != CS_PREC_FP64 ? prec : 4;
   const Variant *v = nullptr;
   if (tile_variant > 0) {
@@ -201,7 +207,9 @@ const Variant *pick_main(int nd, int R, int shape, int prec, int fusion_depth, i
       if (fmf == 9 && top < 2) continue;
       int kf = fusion_depth > 0 ? std::min(fusion_depth, top)
                : fmf == 6       ? top
-               : fmf == 7       ? std::min(top, R == 1 ? 8 : shape == cs::SHAPE_STAR ? 4 : 2)
+               : fmf == 7       ? std::min(top, R == 1 ? (shape == cs::SHAPE_BOX && cells > 0 &&
+                                                          cells <= kRegSmallGrid ? 4 : 8)
+                                                 : shape == cs::SHAPE_STAR ? 4 : 2)
                : fmf == 9       ? std::min(top, shape == cs::SHAPE3_STAR ? 3 : 2)
                                 : default_kf(nd, R, top, shape, true);
       for (; kf >= 1 && !v; --kf) v = find_variant(nd, fmf, R, shape, kf, 0, prec);
@@ -226,7 +234,7 @@ cs_status select_variants(cs_plan *p) {
     v = pick_main(nd, p->R, p->shape, prec, p->opt.fusion_depth, p->opt.tile_variant);
     if (!v) return CS_ERR_INVALID;
   } else if (prec == CS_PREC_FP64) {
-    v = pick_main(nd, p->R, p->shape, prec, p->opt.fusion_depth, 0);
+    v = pick_main(nd, p->R, p->shape, prec, p->opt.fusion_depth, 0, nd == 2 ? p->L.n[1] * p->L.n[2] : 0);
   } else {
     const int tries[3][2] = {{p->shape, 2}, {cs::SHAPE_BOX, 2}, {cs::SHAPE_BOX, 0}};
     for (const auto &tr : tries) {
@@ -561,7 +569,8 @@ namespace cs {
 
 // The FP64 kernel choice of a plan, made without a plan (nothing allocated, no device
 // touched): the main variant, the single-step variant and one variant per remainder depth
-// 1 .. kf - 1 (vrem[r], null where the family has none), as select_variants picks them.
+// 1 .. kf - 1 (vrem[r], null where the family has none), as select_variants picks them for
+// a grid of `cells` interior cells.
 // The launch sequence of the register-streaming 2-D family for `steps` steps of a radius-R
 // stencil of this shape class: the plan's default depth as often as it fits, then the
 // family's largest depth that fits the rest; empty when the plan would not take the family.
@@ -587,9 +596,9 @@ void reg_sequence(int R, int shape, int64_t steps, std::vector<const Variant *> 
   }
 }
 
-cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int tile_variant, const Variant **vmain,
-                             const Variant **vone, const Variant **vrem, int vrem_cap) {
-  const Variant *v = pick_main(ndim, R, shape, CS_PREC_FP64, fusion_depth, tile_variant);
+cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int tile_variant, int64_t cells,
+                             const Variant **vmain, const Variant **vone, const Variant **vrem, int vrem_cap) {
+  const Variant *v = pick_main(ndim, R, shape, CS_PREC_FP64, fusion_depth, tile_variant, cells);
   if (!v) return tile_variant > 0 ? CS_ERR_INVALID : CS_ERR_UNSUPPORTED;
   if (v->kf > vrem_cap) return CS_ERR_UNSUPPORTED;
   *vmain = v;

@@ -17,8 +17,8 @@
 
 namespace cs {
 
-cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int tile_variant, const Variant **vmain,
-                             const Variant **vone, const Variant **vrem, int vrem_cap);
+cs_status pick_fp64_variants(int ndim, int R, int shape, int fusion_depth, int tile_variant, int64_t cells,
+                             const Variant **vmain, const Variant **vone, const Variant **vrem, int vrem_cap);
 
 namespace {
 
@@ -91,7 +91,8 @@ extern "C" cs_status cs_run_2d_noalloc(const cs_layout *layout, const cs_stencil
   CS_TRY(check_device(&dev));
   const int shape = classify_2d(R, st->weights);
   const cs::Variant *vmain = nullptr, *vone = nullptr, *vrem[32] = {};
-  CS_TRY(cs::pick_fp64_variants(2, R, shape, o.fusion_depth, o.tile_variant, &vmain, &vone, vrem, 32));
+  CS_TRY(cs::pick_fp64_variants(2, R, shape, o.fusion_depth, o.tile_variant, layout->n[1] * layout->n[2], &vmain,
+                                &vone, vrem, 32));
   // the dilated tables (mf 4: B fragments) and the register-streaming kernels (mf 7: dense
   // weights, their single steps on the mf 4 kernel as in a plan): scratch holds the dense
   // weights first when the main variant is mf 7, then the B fragments when an mf 4 kernel runs
