@@ -373,8 +373,9 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
   }
 
 // ids 700000 + (ring of 2 (2R + 1) rows) * 50000 + (star) * 10000 + (R > 1) * R * 1000 + PX * 100
-// + KF; 780408: the box radius-1 kernel with a ring of 4 rows (1% faster launches than a
-// ring of 3, 0.410-0.414 against 0.417 ms, within the autotuner's margin; a ring of 2 1982).  Measured (GStencil/s over 64-step runs; boxes differ by up to 10%): box2d1r 10240^2
+// + KF; 780408: the box radius-1 kernel with a ring of 4 rows, the default (launches of
+// 0.400-0.414 ms against 0.417 for a ring of 3, box2d1r 2012-2074 GStencil/s; a ring of 2
+// 1982).  Measured (GStencil/s over 64-step runs; boxes differ by up to 10%): box2d1r 10240^2
 // p2 kf8 ring 6 (3 waves per SIMD) 1868-1940, p4 kf8 ring 3 (2 waves) 1950-2038, p2 kf12
 // ring 6 (2 waves) 1849-1908, p2 kf10 ring 6 1840, p2 kf8 ring 3 1812-1830, p3 kf8 1715, p2
 // kf6 ring 3 (4 waves) 1546; star2d3r 8192^2 p2 kf4 ring 7 1016-1018 (kf3 797, kf6 761);
@@ -383,8 +384,8 @@ static hipError_t launch_multi_reg2d(const RegMulti &mm, hipStream_t s) {
 // registers for more waves and a deeper load ring, which the FP64 FMA issue needs: one wave
 // issues an FP64 FMA about every 12 cycles, two waves about every 6.
 static const Variant kVariantsReg2D[] = {
-    CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),   CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),
-    CS_VR2D(780408, 1, SHAPE_BOX, 4, 8, 4, 2),
+    CS_VR2D(780408, 1, SHAPE_BOX, 4, 8, 4, 2),   CS_VR2D(700408, 1, SHAPE_BOX, 4, 8, 3, 2),
+    CS_VR2D(750208, 1, SHAPE_BOX, 2, 8, 6, 3),
     CS_VR2D(750212, 1, SHAPE_BOX, 2, 12, 6, 2),  CS_VR2D(700212, 1, SHAPE_BOX, 2, 12, 3, 3),
     CS_VR2D(750216, 1, SHAPE_BOX, 2, 16, 6, 2),  CS_VR2D(700404, 1, SHAPE_BOX, 4, 4, 3, 2),
     CS_VR2D(700402, 1, SHAPE_BOX, 4, 2, 3, 2),   CS_VR2D(700401, 1, SHAPE_BOX, 4, 1, 3, 2),
